@@ -1,0 +1,215 @@
+/*
+ * qvmhip.h -- C ABI of the MI355X (gfx950) statevector engine for the
+ * pure-state path of quil-lang/qvm.
+ *
+ * The reference has no C plugin API: its seams are CLOS generic functions on
+ * the pure-state class.  Each entry point below replaces one of them; the
+ * citation names the reference definition (path:line in the qvm 1.18.0 tree)
+ * whose behaviour it reproduces.  A Lisp CFFI binding calls these from the
+ * methods of a new state class; the Python package qvm_amd calls them through
+ * ctypes.
+ *
+ * Conventions
+ *  - Every function returns int: 0 = QVMHIP_OK, negative = error.  The message
+ *    of the last error on the calling thread is qvmhip_last_error().
+ *  - Amplitudes cross the ABI as interleaved complex double (re, im),
+ *    little-endian, the layout of SBCL's (complex double-float) vectors.  An
+ *    FP32 state is widened on read and rounded on write.
+ *  - Gate matrices are row-major 2^k x 2^k interleaved complex double.  Matrix
+ *    index bit j selects qubits[j]; qubits[] is the nat-tuple order, i.e. the
+ *    LAST Quil argument first (src/utilities.lisp:43-51,
+ *    src/compile-gate.lisp:51-64).
+ *  - Qubit q is bit q of the amplitude index (src/wavefunction.lisp:24-40).
+ *  - Calls on one handle are ordered on that handle's HIP stream and are
+ *    asynchronous unless they return host data; a handle is not re-entrant,
+ *    distinct handles may be used from distinct threads.
+ */
+#ifndef QVMHIP_H
+#define QVMHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QVMHIP_API __attribute__((visibility("default")))
+
+enum {
+    QVMHIP_OK = 0,
+    QVMHIP_EINVAL = -1,      /* bad argument (qubit out of range, arity, ...) */
+    QVMHIP_ENOMEM = -2,      /* device allocation failed or memory cap hit   */
+    QVMHIP_EDEVICE = -3,     /* HIP runtime error                             */
+    QVMHIP_ERANGE = -4,      /* probability / norm argument out of range      */
+    QVMHIP_EUNSUPPORTED = -5 /* valid request this build does not implement   */
+};
+
+enum { QVMHIP_F64 = 0, QVMHIP_F32 = 1 };
+
+/* Tile-pass op kinds (gates applied inside an LDS-resident block). */
+enum {
+    QVMHIP_TILE_DENSE1 = 1, /* 2x2 on local qubit q0                          */
+    QVMHIP_TILE_DENSE2 = 2, /* 4x4 on local qubits (q0 = matrix bit 0, q1)    */
+    QVMHIP_TILE_DIAG1 = 3,  /* diag(m[0], m[1]) on q0                          */
+    QVMHIP_TILE_DIAG2 = 4,  /* diag(m[0..3]) on (q0, q1)                       */
+    QVMHIP_TILE_CPERM = 5,  /* swap |..c=1,t=0..> <-> |..c=1,t=1..>: q0 = t, q1 = c (CNOT) */
+    QVMHIP_TILE_SWAP = 6,   /* swap local qubits q0, q1                        */
+    QVMHIP_TILE_CPHASE = 7  /* multiply amplitudes with q0 = q1 = 1 by m[0]   */
+};
+
+/* One gate of a tile pass.  q0/q1 are LOCAL qubit positions in the tile
+ * (0 .. m-1), ctrl_mask is a mask of local positions that must be 1 for the
+ * op to act (0 = none).  m holds the matrix entries needed by the kind
+ * (interleaved complex double, row-major). */
+typedef struct {
+    uint32_t kind;
+    uint32_t q0;
+    uint32_t q1;
+    uint32_t ctrl_mask;
+    double m[32];
+} qvmhip_tile_op;
+
+typedef struct {
+    uint32_t struct_size;      /* sizeof(qvmhip_config)                       */
+    int32_t device;            /* HIP device ordinal; -1 = current device      */
+    uint64_t memory_cap_bytes; /* refuse states larger than this; 0 = no cap   */
+    uint32_t flags;            /* reserved, 0                                  */
+} qvmhip_config;
+
+typedef struct qvmhip_state qvmhip_state;
+typedef struct qvmhip_program qvmhip_program;
+
+/* --- library ----------------------------------------------------------- */
+QVMHIP_API const char *qvmhip_last_error(void);
+QVMHIP_API const char *qvmhip_version(void);
+QVMHIP_API int qvmhip_device_count(int *out);
+/* Bytes a state of n qubits and dtype needs, including scratch. */
+QVMHIP_API int qvmhip_state_bytes(uint32_t n, int dtype, uint64_t *out);
+
+/* --- state lifetime (make-qvm / make-pure-state / pure-state protocol) ---- */
+/* make-pure-state + bring-to-zero-state: src/state-representation.lisp:76-102,
+ * src/qvm.lisp:124-164.  The new state is |0...0>. */
+QVMHIP_API int qvmhip_create(uint32_t n, int dtype, const qvmhip_config *cfg, qvmhip_state **out);
+QVMHIP_API int qvmhip_destroy(qvmhip_state *s);
+/* num-qubits: src/state-representation.lisp:25-47. */
+QVMHIP_API int qvmhip_num_qubits(const qvmhip_state *s, uint32_t *n);
+QVMHIP_API int qvmhip_dtype(const qvmhip_state *s, int *dtype);
+/* set-to-zero-state / bring-to-zero-state: src/wavefunction.lisp:79-91. */
+QVMHIP_API int qvmhip_reset_zero(qvmhip_state *s);
+/* Wait for all work queued on the handle. */
+QVMHIP_API int qvmhip_sync(qvmhip_state *s);
+
+/* --- gate application (apply-gate-to-state / apply-matrix-operator) ----- */
+/* apply-matrix-operator: src/wavefunction.lisp:308-330 and the kernels it
+ * dispatches to (src/serial-kernels.lisp:19-271).  The engine detects control
+ * qubits, diagonal and permutation structure and touches only the amplitudes
+ * the matrix can change; the FP64 result equals the serial kernels'
+ * multiply-then-add arithmetic. 1 <= k <= 6. */
+QVMHIP_API int qvmhip_apply_matrix(qvmhip_state *s, const double *U, uint32_t k,
+                                   const uint32_t *qubits);
+/* Same, with structure detection off: every amplitude is swept through the
+ * dense 2^k x 2^k product (the generic apply-operator,
+ * src/wavefunction.lisp:271-306). */
+QVMHIP_API int qvmhip_apply_matrix_dense(qvmhip_state *s, const double *U, uint32_t k,
+                                         const uint32_t *qubits);
+/* Diagonal gate diag(d[0..2^k-1]) (a diagonal quantum-operator applied by
+ * apply-matrix-operator, src/wavefunction.lisp:308-330). */
+QVMHIP_API int qvmhip_apply_diagonal(qvmhip_state *s, const double *d, uint32_t k,
+                                     const uint32_t *qubits);
+/* Permutation gate: psi'[j] = psi[perm[j]] on gate-local indices, the
+ * transposition swaps of src/compile-gate.lisp:259-309. */
+QVMHIP_API int qvmhip_apply_permutation(qvmhip_state *s, const uint32_t *perm, uint32_t k,
+                                        const uint32_t *qubits);
+/* Dense fused k-qubit block (3 <= k <= 6) on the matrix cores (MFMA).  The
+ * fused blocks of compile-loaded-program with *fuse-gates-during-compilation*
+ * (src/qvm.lisp:166-175) land here. */
+QVMHIP_API int qvmhip_apply_fused(qvmhip_state *s, const double *U, uint32_t k,
+                                  const uint32_t *qubits);
+/* Apply a sequence of small gates to every 2^m-amplitude block spanned by
+ * tile_qubits (ascending, m <= 12 for F64, 13 for F32) with the block
+ * resident in LDS: one HBM sweep for the whole sequence.  The gate-by-gate
+ * arithmetic is the same as qvmhip_apply_matrix. */
+QVMHIP_API int qvmhip_apply_tile_pass(qvmhip_state *s, uint32_t m, const uint32_t *tile_qubits,
+                                      uint32_t nops, const qvmhip_tile_op *ops);
+
+/* --- reductions --------------------------------------------------------- */
+/* wavefunction-excited-state-probability: src/wavefunction.lisp:62-70. */
+QVMHIP_API int qvmhip_prob_excited(qvmhip_state *s, uint32_t q, double *p1);
+/* wavefunction-ground-state-probability: src/wavefunction.lisp:52-60. */
+QVMHIP_API int qvmhip_prob_ground(qvmhip_state *s, uint32_t q, double *p0);
+/* norm: src/wavefunction.lisp:332-346. */
+QVMHIP_API int qvmhip_norm(qvmhip_state *s, double *out);
+/* normalize-wavefunction: src/wavefunction.lisp:349-364 (length <= 0:
+ * compute the norm first). */
+QVMHIP_API int qvmhip_normalize(qvmhip_state *s, double length);
+/* |psi_i|^2 for i in [start, start+count) into host doubles (the
+ * probabilities endpoint, app/src/handle-request.lisp:154-175). */
+QVMHIP_API int qvmhip_probabilities(qvmhip_state *s, uint64_t start, uint64_t count, double *out);
+/* <psi| sum_e Z_a Z_b |psi> over edges (a_e, b_e): the QAOA MaxCut cost
+ * expectation (examples/qaoa.lisp:47-75).  edges = 2*nedges qubit indices. */
+QVMHIP_API int qvmhip_expectation_zz(qvmhip_state *s, const uint32_t *edges, uint32_t nedges,
+                                     double *out);
+/* <a|b> (the inner product of pure-state-expectation,
+ * app/src/api/expectation.lisp:78-89). Both states: same n and dtype. */
+QVMHIP_API int qvmhip_inner_product(qvmhip_state *a, qvmhip_state *b, double *re, double *im);
+
+/* --- measurement -------------------------------------------------------- */
+/* force-measurement, pure state: src/measurement.lisp:10-41. */
+QVMHIP_API int qvmhip_force_measure(qvmhip_state *s, uint32_t q, int bit, double p1);
+/* measure (src/measurement.lisp:87-105) with the caller's uniform u.
+ * compiled = 0: interpreted rule -- p1 == 0 gives 0 and consumes no draw,
+ *               else bit = (u <= p1).
+ * compiled = 1: compiled projector (src/compile-gate.lisp:221-254) -- always
+ *               consumes u, bit = (u < p0) ? 0 : 1.
+ * *consumed = number of uniforms used (0 or 1). */
+QVMHIP_API int qvmhip_measure(qvmhip_state *s, uint32_t q, double u, int compiled, int *bit,
+                              int *consumed);
+/* measure-all-state, pure (src/measurement.lisp:128-143): index =
+ * min{b : C(b) > u} (len-1 if none), state collapses to |index> with
+ * amplitude exactly 1. */
+QVMHIP_API int qvmhip_measure_all(qvmhip_state *s, double u, uint64_t *index);
+/* nshots measure-all draws of the same prepared state without collapse: the
+ * reload-per-shot loop of app/src/api/multishot-measure.lisp:48-110, rule
+ * C(b) > u. */
+QVMHIP_API int qvmhip_sample_measure_all(qvmhip_state *s, const double *u, uint64_t nshots,
+                                         uint64_t *out);
+/* sample-wavefunction-multiple-times (src/measurement.lisp:246-288): per u,
+ * min{i : cdf[i] >= u}, clamped to len-1.  No collapse. */
+QVMHIP_API int qvmhip_sample(qvmhip_state *s, const double *u, uint64_t nshots, uint64_t *out);
+
+/* --- amplitudes (amplitudes / state-elements / copy-wavefunction) -------- */
+/* amplitudes, nth-amplitude, state-elements: src/qvm.lisp:63-82,
+ * src/state-representation.lisp:138-167.  The only D2H point of the path. */
+QVMHIP_API int qvmhip_read_amplitudes(qvmhip_state *s, uint64_t start, uint64_t count, double *out);
+/* (setf state-elements) / (setf amplitudes). */
+QVMHIP_API int qvmhip_write_amplitudes(qvmhip_state *s, uint64_t start, uint64_t count,
+                                       const double *in);
+/* copy-wavefunction (src/wavefunction.lisp:93-109): dst <- src, same n/dtype. */
+QVMHIP_API int qvmhip_copy(qvmhip_state *dst, const qvmhip_state *src);
+
+/* --- compiled programs (compile-loaded-program / compile-instruction) ---- */
+/* A program records gates, tile passes and fused blocks against one state,
+ * uploads their operands once, and replays them -- optionally as one captured
+ * hipGraph (src/qvm.lisp:166-175, src/compile-gate.lisp:461-526). */
+QVMHIP_API int qvmhip_program_create(qvmhip_state *s, qvmhip_program **out);
+QVMHIP_API int qvmhip_program_add_matrix(qvmhip_program *p, const double *U, uint32_t k,
+                                         const uint32_t *qubits);
+QVMHIP_API int qvmhip_program_add_fused(qvmhip_program *p, const double *U, uint32_t k,
+                                        const uint32_t *qubits);
+QVMHIP_API int qvmhip_program_add_tile_pass(qvmhip_program *p, uint32_t m,
+                                            const uint32_t *tile_qubits, uint32_t nops,
+                                            const qvmhip_tile_op *ops);
+/* use_graph = 1: capture the launches into one hipGraph. */
+QVMHIP_API int qvmhip_program_finalize(qvmhip_program *p, int use_graph);
+QVMHIP_API int qvmhip_program_run(qvmhip_program *p);
+QVMHIP_API int qvmhip_program_destroy(qvmhip_program *p);
+
+/* --- timing (engine-recorded kernel timings for the fusion cost model) --- */
+/* Time `reps` launches of the last program (or a single gate when p is
+ * NULL) with HIP events on the handle's stream; *ms = mean per launch. */
+QVMHIP_API int qvmhip_time_program(qvmhip_program *p, int reps, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QVMHIP_H */

@@ -1,0 +1,160 @@
+"""ctypes binding of the qvmhip C ABI (include/qvmhip.h).
+
+This module is the Python twin of the Lisp CFFI binding a qvm maintainer
+would add: one foreign function per entry point, same argument order.  It
+loads the in-tree ``qvm_amd/_lib/libqvmhip.so`` and raises if it is missing;
+there is no CPU fallback anywhere in the package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libqvmhip.so")
+
+F64 = 0
+F32 = 1
+
+OK = 0
+EINVAL = -1
+ENOMEM = -2
+EDEVICE = -3
+ERANGE = -4
+EUNSUPPORTED = -5
+
+TILE_DENSE1 = 1
+TILE_DENSE2 = 2
+TILE_DIAG1 = 3
+TILE_DIAG2 = 4
+TILE_CPERM = 5
+TILE_SWAP = 6
+TILE_CPHASE = 7
+
+
+class QVMHipError(RuntimeError):
+    """An error status returned by the engine; ``code`` is the status."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"qvmhip error {code}: {message}")
+        self.code = code
+
+
+class TileOp(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("q0", C.c_uint32), ("q1", C.c_uint32),
+                ("ctrl_mask", C.c_uint32), ("m", C.c_double * 32)]
+
+
+class Config(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("device", C.c_int32),
+                ("memory_cap_bytes", C.c_uint64), ("flags", C.c_uint32)]
+
+
+_u32 = C.c_uint32
+_u64 = C.c_uint64
+_dp = C.POINTER(C.c_double)
+_u32p = C.POINTER(C.c_uint32)
+_u64p = C.POINTER(C.c_uint64)
+_vp = C.c_void_p
+_ip = C.POINTER(C.c_int)
+
+# name -> argtypes (all return int unless listed in _RESTYPE)
+SIGNATURES = {
+    "qvmhip_last_error": [],
+    "qvmhip_version": [],
+    "qvmhip_device_count": [_ip],
+    "qvmhip_state_bytes": [_u32, C.c_int, _u64p],
+    "qvmhip_create": [_u32, C.c_int, C.POINTER(Config), C.POINTER(_vp)],
+    "qvmhip_destroy": [_vp],
+    "qvmhip_num_qubits": [_vp, _u32p],
+    "qvmhip_dtype": [_vp, _ip],
+    "qvmhip_reset_zero": [_vp],
+    "qvmhip_sync": [_vp],
+    "qvmhip_apply_matrix": [_vp, _dp, _u32, _u32p],
+    "qvmhip_apply_matrix_dense": [_vp, _dp, _u32, _u32p],
+    "qvmhip_apply_diagonal": [_vp, _dp, _u32, _u32p],
+    "qvmhip_apply_permutation": [_vp, _u32p, _u32, _u32p],
+    "qvmhip_apply_fused": [_vp, _dp, _u32, _u32p],
+    "qvmhip_apply_tile_pass": [_vp, _u32, _u32p, _u32, C.POINTER(TileOp)],
+    "qvmhip_prob_excited": [_vp, _u32, _dp],
+    "qvmhip_prob_ground": [_vp, _u32, _dp],
+    "qvmhip_norm": [_vp, _dp],
+    "qvmhip_normalize": [_vp, C.c_double],
+    "qvmhip_probabilities": [_vp, _u64, _u64, _dp],
+    "qvmhip_expectation_zz": [_vp, _u32p, _u32, _dp],
+    "qvmhip_inner_product": [_vp, _vp, _dp, _dp],
+    "qvmhip_force_measure": [_vp, _u32, C.c_int, C.c_double],
+    "qvmhip_measure": [_vp, _u32, C.c_double, C.c_int, _ip, _ip],
+    "qvmhip_measure_all": [_vp, C.c_double, _u64p],
+    "qvmhip_sample_measure_all": [_vp, _dp, _u64, _u64p],
+    "qvmhip_sample": [_vp, _dp, _u64, _u64p],
+    "qvmhip_read_amplitudes": [_vp, _u64, _u64, _dp],
+    "qvmhip_write_amplitudes": [_vp, _u64, _u64, _dp],
+    "qvmhip_copy": [_vp, _vp],
+    "qvmhip_program_create": [_vp, C.POINTER(_vp)],
+    "qvmhip_program_add_matrix": [_vp, _dp, _u32, _u32p],
+    "qvmhip_program_add_fused": [_vp, _dp, _u32, _u32p],
+    "qvmhip_program_add_tile_pass": [_vp, _u32, _u32p, _u32, C.POINTER(TileOp)],
+    "qvmhip_program_finalize": [_vp, C.c_int],
+    "qvmhip_program_run": [_vp],
+    "qvmhip_program_destroy": [_vp],
+    "qvmhip_time_program": [_vp, C.c_int, _dp],
+}
+_RESTYPE = {"qvmhip_last_error": C.c_char_p, "qvmhip_version": C.c_char_p}
+
+_lib = None
+
+
+def lib():
+    """Load the engine library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"qvmhip library not built: {LIB_PATH} is missing (run __graft_entry__.build() "
+                "or python qvm_amd/build.py)")
+        L = C.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _RESTYPE.get(name, C.c_int)
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().qvmhip_last_error().decode()
+
+
+def check(rc: int) -> None:
+    if rc != OK:
+        raise QVMHipError(rc, last_error())
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(lib().qvmhip_device_count(C.byref(n)))
+    return n.value
+
+
+def dptr(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def u32arr(xs) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(xs, dtype=np.uint32))
+
+
+def u32ptr(a: np.ndarray):
+    return a.ctypes.data_as(_u32p)
+
+
+def u64ptr(a: np.ndarray):
+    return a.ctypes.data_as(_u64p)
+
+
+def cmat(U) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(U, dtype=np.complex128))

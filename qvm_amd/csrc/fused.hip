@@ -1,0 +1,173 @@
+// fused.hip -- dense fused k-qubit blocks (k = 3..6) on the matrix cores.
+//
+// Applying a 2^k x 2^k complex U to every complement group is a tall-skinny
+// complex GEMM Y = U X, X = 2^k x 2^(n-k).  It is computed as one real GEMM
+// with the interleaved real embedding
+//     A[2i][2j] = Re U_ij   A[2i][2j+1] = -Im U_ij
+//     A[2i+1][2j] = Im U_ij A[2i+1][2j+1] = Re U_ij
+// so the state's own (re, im) interleaving is the real X (rows 2j, 2j+1 of a
+// column are Re/Im of the column's amplitude j) -- no deinterleave pass.
+// The four real products per complex product are all kept (no 3-multiply
+// form), so rounding stays that of an ordinary complex matvec with FMA.
+//
+// Workgroup: 4 wavefronts, 64 complement groups (columns) per tile; wave w
+// owns columns 16w..16w+15 and all 2D/16 row tiles.  U is staged once per
+// workgroup in LDS (complex, rows padded by one element so the 16 lanes of a
+// fragment hit distinct banks); each X tile is gathered into LDS with
+// coalesced loads, multiplied with v_mfma_f64_16x16x4f64 /
+// v_mfma_f32_16x16x4f32, written back into the wave's own LDS columns and
+// stored with the same coalesced pattern.  The workgroup grid-strides over
+// tiles, so U is read from HBM once per workgroup, not once per tile.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "qvmhip_internal.h"
+
+namespace qvmhip {
+
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ inline v4 mma(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Mfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ inline v4 mma(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+struct FusedOff {
+    uint64_t o[64];
+};
+
+constexpr int kFusedCols = 64;
+
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k_fused(vec2_t<T> *__restrict__ psi, const double *__restrict__ U,
+                                               GroupSpec g, FusedOff off, int group_fastest) {
+    using V = vec2_t<T>;
+    using v4 = typename Mfma<T>::v4;
+    constexpr int D = 1 << K;
+    constexpr int R2 = 2 * D;       // real rows = real cols of A
+    constexpr int LU = 2 * D + 2;   // padded complex row of U (in T words)
+    constexpr int LX = R2 + 2;      // padded real column of X
+    constexpr int MT = R2 / 16;     // row tiles
+    constexpr int KS = R2 / 4;      // k-steps
+    constexpr int NCB = kFusedCols;
+    __shared__ T su[D * LU];
+    __shared__ T sx[NCB * LX];
+
+    for (int e = threadIdx.x; e < D * D; e += 256) {
+        const int i = e / D, j = e % D;
+        su[i * LU + 2 * j] = (T)U[2 * e];
+        su[i * LU + 2 * j + 1] = (T)U[2 * e + 1];
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lj = lane & 15, kk = lane >> 4;
+    // A fragment of lane: row r = mt*16 + lj, col c = 4*ks + kk
+    const int rpar = lj & 1, cpar = kk & 1;
+    const bool use_im = rpar != cpar;
+    const T sgn = (rpar == 0 && cpar == 1) ? T(-1) : T(1);
+
+    const uint64_t ntiles = (g.ngroups + NCB - 1) / NCB;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < NCB * D; e += 256) {
+            const int gi = group_fastest ? (e % NCB) : (e / D);
+            const int a = group_fastest ? (e / NCB) : (e % D);
+            const uint64_t grp = tile * NCB + gi;
+            V x;
+            x.x = T(0);
+            x.y = T(0);
+            if (grp < g.ngroups) x = psi[group_base(g, grp) + off.o[a]];
+            sx[gi * LX + 2 * a] = x.x;
+            sx[gi * LX + 2 * a + 1] = x.y;
+        }
+        __syncthreads();
+        v4 acc[MT];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) acc[mt] = v4{0, 0, 0, 0};
+        const T *xcol = &sx[(wave * 16 + lj) * LX];
+#pragma unroll 4
+        for (int ks = 0; ks < KS; ++ks) {
+            const T b = xcol[4 * ks + kk];
+            const int cj = (4 * ks + kk) >> 1;
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const int ri = (mt * 16 + lj) >> 1;
+                const T a = sgn * su[ri * LU + 2 * cj + (use_im ? 1 : 0)];
+                acc[mt] = Mfma<T>::mma(a, b, acc[mt]);
+            }
+        }
+        // D fragment: lane holds rows 4*kk + e (e = 0..3) of column lj.
+        T *ycol = &sx[(wave * 16 + lj) * LX];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) ycol[mt * 16 + 4 * kk + e] = acc[mt][e];
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < NCB * D; e += 256) {
+            const int gi = group_fastest ? (e % NCB) : (e / D);
+            const int a = group_fastest ? (e / NCB) : (e % D);
+            const uint64_t grp = tile * NCB + gi;
+            if (grp >= g.ngroups) continue;
+            V y;
+            y.x = sx[gi * LX + 2 * a];
+            y.y = sx[gi * LX + 2 * a + 1];
+            psi[group_base(g, grp) + off.o[a]] = y;
+        }
+    }
+}
+
+namespace {
+template <typename T, int K>
+int run_fused(qvmhip_state *s, const void *mat_dev, const uint32_t *qubits) {
+    constexpr int D = 1 << K;
+    GroupSpec g;
+    std::memset(&g, 0, sizeof(g));
+    std::vector<uint32_t> pos(qubits, qubits + K);
+    std::sort(pos.begin(), pos.end());
+    g.nsorted = K;
+    for (int i = 0; i < K; ++i) g.sorted[i] = pos[i];
+    g.ngroups = uint64_t(1) << (s->n - K);
+    FusedOff off;
+    std::memset(&off, 0, sizeof(off));
+    bool has_q0 = false;
+    for (int a = 0; a < D; ++a) {
+        uint64_t o = 0;
+        for (int b = 0; b < K; ++b)
+            if ((a >> b) & 1) o |= uint64_t(1) << qubits[b];
+        off.o[a] = o;
+    }
+    for (int b = 0; b < K; ++b) has_q0 |= qubits[b] == 0;
+    const uint64_t ntiles = (g.ngroups + kFusedCols - 1) / kFusedCols;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(ntiles, 256 * 8);
+    hipLaunchKernelGGL((k_fused<T, K>), dim3(grid), dim3(256), 0, s->stream, (vec2_t<T> *)s->amps,
+                       (const double *)mat_dev, g, off, has_q0 ? 0 : 1);
+    QH_LAUNCH_CHECK();
+    return QVMHIP_OK;
+}
+
+template <typename T>
+int run_fused_k(qvmhip_state *s, const void *mat_dev, uint32_t k, const uint32_t *qubits) {
+    switch (k) {
+        case 3: return run_fused<T, 3>(s, mat_dev, qubits);
+        case 4: return run_fused<T, 4>(s, mat_dev, qubits);
+        case 5: return run_fused<T, 5>(s, mat_dev, qubits);
+        case 6: return run_fused<T, 6>(s, mat_dev, qubits);
+    }
+    return fail(QVMHIP_EINVAL, "fused blocks are 3..6 qubits");
+}
+}  // namespace
+
+int launch_fused_dense(qvmhip_state *s, const void *mat_dev, uint32_t k, const uint32_t *qubits) {
+    if (s->dtype == QVMHIP_F64) return run_fused_k<double>(s, mat_dev, k, qubits);
+    return run_fused_k<float>(s, mat_dev, k, qubits);
+}
+
+}  // namespace qvmhip

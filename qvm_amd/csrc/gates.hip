@@ -1,0 +1,568 @@
+// gates.hip -- gate application kernels (plain gate-by-gate path).
+//
+// Every gate is one HBM sweep over the amplitudes it can change.  The host
+// dispatcher splits a 2^k x 2^k matrix into control qubits (index bits whose
+// 0-half is the identity) and a reduced operator on the remaining targets,
+// which is one of: identity (no launch), diagonal, permutation, dense.
+// Touched amplitudes per gate: dense/diagonal 1q 2^n, CNOT/SWAP 2^(n-1),
+// CZ/CPHASE 2^(n-2).
+//
+// Arithmetic: built with -ffp-contract=off.  Each output is the left-to-right
+// sum of separately rounded complex products x_j * U[r][j] with
+// (a.re*b.re - a.im*b.im, a.re*b.im + a.im*b.re) -- the arithmetic of the
+// reference's serial kernels (src/serial-kernels.lisp:54-67) and generic
+// matrix-multiply (src/linear-algebra.lisp:97-131).  Entries the dispatcher
+// skips (zero off-diagonals of controlled/diagonal/permutation gates) only
+// ever add signed zeros in the reference, so FP64 results agree with it
+// exactly.
+#include <algorithm>
+#include <complex>
+#include <cstring>
+#include <vector>
+
+#include "qvmhip_internal.h"
+
+namespace qvmhip {
+
+using cd = std::complex<double>;
+
+// ---------------------------------------------------------------------------
+// Kernel argument blocks (passed by value; all < 4 KiB).
+
+template <typename T, int KT>
+struct DenseArgs {
+    GroupSpec g;
+    uint64_t off[1 << KT];
+    T ur[(1 << KT) * (1 << KT)];
+    T ui[(1 << KT) * (1 << KT)];
+};
+
+struct OffTable {
+    uint64_t o[64];
+};
+
+template <typename T>
+struct DiagFullArgs {
+    uint64_t len;
+    uint32_t kt;
+    uint32_t tq[kMaxK];
+    T dr[64];
+    T di[64];
+};
+
+template <typename T>
+struct SparseArgs {
+    GroupSpec g;
+    uint32_t nent;
+    uint64_t off[64];
+    uint64_t src[64];  // permutation: source offsets
+    T dr[64];
+    T di[64];
+};
+
+// ---------------------------------------------------------------------------
+// Kernels.
+
+// Dense k-qubit operator, k <= 3, one complement group per thread, GPT groups
+// per thread kept in flight.  Consecutive threads take consecutive groups, so
+// a wavefront reads contiguous runs (pair-strided indexing, coalesced for
+// every target).
+template <typename T, int KT, int GPT>
+__global__ __launch_bounds__(kBlock) void k_dense(vec2_t<T> *__restrict__ psi,
+                                                  const DenseArgs<T, KT> a) {
+    constexpr int D = 1 << KT;
+    using V = vec2_t<T>;
+    const uint64_t first = (uint64_t)blockIdx.x * (kBlock * GPT) + threadIdx.x;
+    V x[GPT][D];
+    uint64_t base[GPT];
+#pragma unroll
+    for (int g = 0; g < GPT; ++g) {
+        const uint64_t i = first + (uint64_t)g * kBlock;
+        base[g] = group_base(a.g, i < a.g.ngroups ? i : 0);
+        if (i < a.g.ngroups) {
+#pragma unroll
+            for (int j = 0; j < D; ++j) x[g][j] = psi[base[g] + a.off[j]];
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < GPT; ++g) {
+        const uint64_t i = first + (uint64_t)g * kBlock;
+        if (i >= a.g.ngroups) continue;
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            T yr = x[g][0].x * a.ur[r * D] - x[g][0].y * a.ui[r * D];
+            T yi = x[g][0].x * a.ui[r * D] + x[g][0].y * a.ur[r * D];
+#pragma unroll
+            for (int c = 1; c < D; ++c) {
+                const T pr = x[g][c].x * a.ur[r * D + c] - x[g][c].y * a.ui[r * D + c];
+                const T pi = x[g][c].x * a.ui[r * D + c] + x[g][c].y * a.ur[r * D + c];
+                yr = yr + pr;
+                yi = yi + pi;
+            }
+            V y;
+            y.x = yr;
+            y.y = yi;
+            psi[base[g] + a.off[r]] = y;
+        }
+    }
+}
+
+// Dense k-qubit operator, 4 <= k <= 6: the matrix is staged in LDS
+// (transposed so that the D lanes of a group read consecutive words), each
+// group's D amplitudes are staged in LDS, lane r of the group produces row r.
+template <typename T, int KT>
+__global__ __launch_bounds__(kBlock) void k_dense_big(vec2_t<T> *__restrict__ psi,
+                                                      const T *__restrict__ ut, GroupSpec g,
+                                                      OffTable off) {
+    constexpr int D = 1 << KT;
+    constexpr int GPB = kBlock / D;
+    using V = vec2_t<T>;
+    __shared__ T sre[D * D];
+    __shared__ T sim[D * D];
+    __shared__ V sx[kBlock];
+    for (int i = threadIdx.x; i < D * D; i += kBlock) {
+        sre[i] = ut[i];
+        sim[i] = ut[D * D + i];
+    }
+    const int r = threadIdx.x % D;
+    const int gl = threadIdx.x / D;
+    const uint64_t nbatch = (g.ngroups + GPB - 1) / GPB;
+    for (uint64_t b = blockIdx.x; b < nbatch; b += gridDim.x) {
+        const uint64_t grp = b * GPB + gl;
+        const bool ok = grp < g.ngroups;
+        uint64_t addr = 0;
+        __syncthreads();
+        if (ok) {
+            addr = group_base(g, grp) + off.o[r];
+            sx[threadIdx.x] = psi[addr];
+        }
+        __syncthreads();
+        if (ok) {
+            const V *x = &sx[gl * D];
+            T yr = x[0].x * sre[r] - x[0].y * sim[r];
+            T yi = x[0].x * sim[r] + x[0].y * sre[r];
+            for (int c = 1; c < D; ++c) {
+                const T mr = sre[c * D + r], mi = sim[c * D + r];
+                const T pr = x[c].x * mr - x[c].y * mi;
+                const T pi = x[c].x * mi + x[c].y * mr;
+                yr = yr + pr;
+                yi = yi + pi;
+            }
+            V y;
+            y.x = yr;
+            y.y = yi;
+            psi[addr] = y;
+        }
+    }
+}
+
+// Diagonal operator with every entry != 1 and no controls: contiguous sweep,
+// amplitude i is multiplied by d[bits of i at the targets].
+template <typename T, int PPT>
+__global__ __launch_bounds__(kBlock) void k_diag_full(vec2_t<T> *__restrict__ psi,
+                                                      const DiagFullArgs<T> a) {
+    using V = vec2_t<T>;
+    const uint64_t first = (uint64_t)blockIdx.x * (kBlock * PPT) + threadIdx.x;
+    V x[PPT];
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) {
+        const uint64_t i = first + (uint64_t)p * kBlock;
+        if (i < a.len) x[p] = psi[i];
+    }
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) {
+        const uint64_t i = first + (uint64_t)p * kBlock;
+        if (i >= a.len) continue;
+        uint32_t j = 0;
+        for (uint32_t t = 0; t < a.kt; ++t) j |= (uint32_t)((i >> a.tq[t]) & 1u) << t;
+        const T mr = a.dr[j], mi = a.di[j];
+        V y;
+        y.x = x[p].x * mr - x[p].y * mi;
+        y.y = x[p].x * mi + x[p].y * mr;
+        psi[i] = y;
+    }
+}
+
+// Sparse diagonal: only the nent listed entries of each group change
+// (controlled phases: CZ, CPHASE, PHASE, CCPHASE ...).
+template <typename T, int GPT>
+__global__ __launch_bounds__(kBlock) void k_diag_sparse(vec2_t<T> *__restrict__ psi,
+                                                        const SparseArgs<T> a) {
+    using V = vec2_t<T>;
+    const uint64_t first = (uint64_t)blockIdx.x * (kBlock * GPT) + threadIdx.x;
+#pragma unroll
+    for (int g = 0; g < GPT; ++g) {
+        const uint64_t i = first + (uint64_t)g * kBlock;
+        if (i >= a.g.ngroups) continue;
+        const uint64_t base = group_base(a.g, i);
+        for (uint32_t e = 0; e < a.nent; ++e) {
+            const V x = psi[base + a.off[e]];
+            V y;
+            y.x = x.x * a.dr[e] - x.y * a.di[e];
+            y.y = x.x * a.di[e] + x.y * a.dr[e];
+            psi[base + a.off[e]] = y;
+        }
+    }
+}
+
+// Permutation: psi'[dst_e] = psi[src_e] for the moved entries of each group
+// (CNOT, SWAP, CCNOT, CSWAP ...): pure data movement, no arithmetic.
+template <typename T, int GPT>
+__global__ __launch_bounds__(kBlock) void k_perm(vec2_t<T> *__restrict__ psi,
+                                                 const SparseArgs<T> a) {
+    using V = vec2_t<T>;
+    const uint64_t first = (uint64_t)blockIdx.x * (kBlock * GPT) + threadIdx.x;
+#pragma unroll
+    for (int g = 0; g < GPT; ++g) {
+        const uint64_t i = first + (uint64_t)g * kBlock;
+        if (i >= a.g.ngroups) continue;
+        const uint64_t base = group_base(a.g, i);
+        if (a.nent == 2) {  // one transposition (CNOT, SWAP, CCNOT, CSWAP)
+            const V x0 = psi[base + a.src[0]];
+            const V x1 = psi[base + a.src[1]];
+            psi[base + a.off[0]] = x0;
+            psi[base + a.off[1]] = x1;
+        } else {
+            V tmp[64];
+            for (uint32_t e = 0; e < a.nent; ++e) tmp[e] = psi[base + a.src[e]];
+            for (uint32_t e = 0; e < a.nent; ++e) psi[base + a.off[e]] = tmp[e];
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fill_zero_state(vec2_t<T> *__restrict__ psi,
+                                                            uint64_t len) {
+    using V = vec2_t<T>;
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < len; i += stride) {
+        V z;
+        z.x = (i == 0) ? T(1) : T(0);
+        z.y = T(0);
+        psi[i] = z;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scale(vec2_t<T> *__restrict__ psi, uint64_t len,
+                                                  T f) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < len; i += stride) {
+        vec2_t<T> v = psi[i];
+        v.x = f * v.x;
+        v.y = f * v.y;
+        psi[i] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side: structure detection and dispatch.
+
+namespace {
+
+struct GateSplit {
+    int kind = 0;  // 0 identity, 1 dense, 2 diagonal, 3 permutation
+    uint32_t kt = 0;
+    uint32_t tq[kMaxK] = {0};  // target qubits, reduced bit j -> tq[j]
+    uint32_t nctrl = 0;
+    uint64_t ctrl_mask = 0;
+    std::vector<cd> V;             // reduced matrix, row-major D' x D'
+    std::vector<uint32_t> perm;    // permutation: row j takes column perm[j]
+};
+
+bool exactly(cd z, double re, double im) { return z.real() == re && z.imag() == im; }
+
+GateSplit split_gate(const cd *U, uint32_t k, const uint32_t *qubits, bool detect) {
+    const uint32_t D = 1u << k;
+    GateSplit s;
+    uint32_t ctrl_bits = 0;
+    if (detect) {
+        for (uint32_t c = 0; c < k; ++c) {
+            bool is_ctrl = true;
+            for (uint32_t r = 0; r < D && is_ctrl; ++r) {
+                if ((r >> c) & 1u) continue;
+                for (uint32_t t = 0; t < D; ++t) {
+                    const double want = (t == r) ? 1.0 : 0.0;
+                    if (!exactly(U[r * D + t], want, 0.0) || !exactly(U[t * D + r], want, 0.0)) {
+                        is_ctrl = false;
+                        break;
+                    }
+                }
+            }
+            if (is_ctrl) ctrl_bits |= 1u << c;
+        }
+    }
+    for (uint32_t b = 0; b < k; ++b) {
+        if ((ctrl_bits >> b) & 1u) {
+            s.ctrl_mask |= uint64_t(1) << qubits[b];
+            ++s.nctrl;
+        } else {
+            s.tq[s.kt++] = qubits[b];
+        }
+    }
+    // reduced index j -> full index
+    const uint32_t Dr = 1u << s.kt;
+    std::vector<uint32_t> full(Dr);
+    for (uint32_t j = 0; j < Dr; ++j) {
+        uint32_t f = ctrl_bits, t = 0;
+        for (uint32_t b = 0; b < k; ++b) {
+            if ((ctrl_bits >> b) & 1u) continue;
+            if ((j >> t) & 1u) f |= 1u << b;
+            ++t;
+        }
+        full[j] = f;
+    }
+    s.V.resize((size_t)Dr * Dr);
+    for (uint32_t r = 0; r < Dr; ++r)
+        for (uint32_t c = 0; c < Dr; ++c) s.V[(size_t)r * Dr + c] = U[(size_t)full[r] * D + full[c]];
+    if (!detect) {
+        s.kind = 1;
+        return s;
+    }
+    bool diag = true, perm = true, ident = true;
+    s.perm.assign(Dr, 0);
+    for (uint32_t r = 0; r < Dr; ++r) {
+        int ones = 0, nonzero = 0;
+        for (uint32_t c = 0; c < Dr; ++c) {
+            const cd z = s.V[(size_t)r * Dr + c];
+            if (!exactly(z, 0.0, 0.0)) {
+                ++nonzero;
+                if (c != r) diag = false;
+                if (exactly(z, 1.0, 0.0)) {
+                    ++ones;
+                    s.perm[r] = c;
+                }
+                if (c != r || !exactly(z, 1.0, 0.0)) ident = false;
+            }
+        }
+        if (nonzero == 0) {
+            diag = false;  // singular row: not diagonal-with-phases, use dense
+            ident = false;
+        }
+        if (!(nonzero == 1 && ones == 1)) perm = false;
+    }
+    if (ident)
+        s.kind = 0;
+    else if (diag)
+        s.kind = 2;
+    else if (perm)
+        s.kind = 3;
+    else
+        s.kind = 1;
+    return s;
+}
+
+void make_group_spec(GroupSpec &g, const GateSplit &s, uint32_t n) {
+    std::vector<uint32_t> pos(s.tq, s.tq + s.kt);
+    for (uint32_t b = 0; b < 64; ++b)
+        if ((s.ctrl_mask >> b) & 1u) pos.push_back(b);
+    std::sort(pos.begin(), pos.end());
+    std::memset(&g, 0, sizeof(g));
+    g.nsorted = (uint32_t)pos.size();
+    for (size_t i = 0; i < pos.size(); ++i) g.sorted[i] = pos[i];
+    g.ctrl_mask = s.ctrl_mask;
+    g.ngroups = uint64_t(1) << (n - g.nsorted);
+}
+
+uint64_t deposit(uint32_t j, const uint32_t *tq, uint32_t kt) {
+    uint64_t o = 0;
+    for (uint32_t t = 0; t < kt; ++t)
+        if ((j >> t) & 1u) o |= uint64_t(1) << tq[t];
+    return o;
+}
+
+uint32_t grid_for(uint64_t work, uint64_t per_block) {
+    uint64_t b = (work + per_block - 1) / per_block;
+    return (uint32_t)std::max<uint64_t>(b, 1);
+}
+
+template <typename T, int KT>
+int run_dense_small(qvmhip_state *s, const GateSplit &gs) {
+    constexpr int D = 1 << KT;
+    constexpr int GPT = KT == 1 ? 4 : (KT == 2 ? 2 : 1);
+    DenseArgs<T, KT> a;
+    std::memset(&a, 0, sizeof(a));
+    make_group_spec(a.g, gs, s->n);
+    for (int j = 0; j < D; ++j) a.off[j] = deposit(j, gs.tq, gs.kt);
+    for (int i = 0; i < D * D; ++i) {
+        a.ur[i] = (T)gs.V[i].real();
+        a.ui[i] = (T)gs.V[i].imag();
+    }
+    hipLaunchKernelGGL((k_dense<T, KT, GPT>), dim3(grid_for(a.g.ngroups, (uint64_t)kBlock * GPT)),
+                       dim3(kBlock), 0, s->stream, (vec2_t<T> *)s->amps, a);
+    QH_LAUNCH_CHECK();
+    return QVMHIP_OK;
+}
+
+template <typename T, int KT>
+int run_dense_big(qvmhip_state *s, const GateSplit &gs) {
+    constexpr int D = 1 << KT;
+    GroupSpec g;
+    make_group_spec(g, gs, s->n);
+    OffTable off;
+    std::memset(&off, 0, sizeof(off));
+    for (int j = 0; j < D; ++j) off.o[j] = deposit(j, gs.tq, gs.kt);
+    // transposed matrix: [re | im], element (r, c) at c*D + r
+    std::vector<T> host(2 * D * D);
+    for (int r = 0; r < D; ++r)
+        for (int c = 0; c < D; ++c) {
+            host[c * D + r] = (T)gs.V[r * D + c].real();
+            host[D * D + c * D + r] = (T)gs.V[r * D + c].imag();
+        }
+    QH_CHECK(hipMemcpyAsync(s->mat, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice,
+                            s->stream));
+    const uint64_t gpb = kBlock / D;
+    const uint64_t nbatch = (g.ngroups + gpb - 1) / gpb;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(nbatch, 256 * 16);
+    hipLaunchKernelGGL((k_dense_big<T, KT>), dim3(grid), dim3(kBlock), 0, s->stream,
+                       (vec2_t<T> *)s->amps, (const T *)s->mat, g, off);
+    QH_LAUNCH_CHECK();
+    return QVMHIP_OK;
+}
+
+template <typename T>
+int run_diag(qvmhip_state *s, const GateSplit &gs) {
+    const uint32_t Dr = 1u << gs.kt;
+    std::vector<uint32_t> ent;
+    for (uint32_t j = 0; j < Dr; ++j)
+        if (!exactly(gs.V[(size_t)j * Dr + j], 1.0, 0.0)) ent.push_back(j);
+    if (ent.empty()) return QVMHIP_OK;
+    if (gs.nctrl == 0 && ent.size() == Dr) {
+        DiagFullArgs<T> a;
+        std::memset(&a, 0, sizeof(a));
+        a.len = s->len;
+        a.kt = gs.kt;
+        for (uint32_t t = 0; t < gs.kt; ++t) a.tq[t] = gs.tq[t];
+        for (uint32_t j = 0; j < Dr; ++j) {
+            a.dr[j] = (T)gs.V[(size_t)j * Dr + j].real();
+            a.di[j] = (T)gs.V[(size_t)j * Dr + j].imag();
+        }
+        constexpr int PPT = 4;
+        hipLaunchKernelGGL((k_diag_full<T, PPT>), dim3(grid_for(a.len, (uint64_t)kBlock * PPT)),
+                           dim3(kBlock), 0, s->stream, (vec2_t<T> *)s->amps, a);
+        QH_LAUNCH_CHECK();
+        return QVMHIP_OK;
+    }
+    SparseArgs<T> a;
+    std::memset(&a, 0, sizeof(a));
+    make_group_spec(a.g, gs, s->n);
+    a.nent = (uint32_t)ent.size();
+    for (size_t e = 0; e < ent.size(); ++e) {
+        a.off[e] = deposit(ent[e], gs.tq, gs.kt);
+        a.dr[e] = (T)gs.V[(size_t)ent[e] * Dr + ent[e]].real();
+        a.di[e] = (T)gs.V[(size_t)ent[e] * Dr + ent[e]].imag();
+    }
+    constexpr int GPT = 2;
+    hipLaunchKernelGGL((k_diag_sparse<T, GPT>), dim3(grid_for(a.g.ngroups, (uint64_t)kBlock * GPT)),
+                       dim3(kBlock), 0, s->stream, (vec2_t<T> *)s->amps, a);
+    QH_LAUNCH_CHECK();
+    return QVMHIP_OK;
+}
+
+template <typename T>
+int run_perm(qvmhip_state *s, const GateSplit &gs) {
+    const uint32_t Dr = 1u << gs.kt;
+    SparseArgs<T> a;
+    std::memset(&a, 0, sizeof(a));
+    make_group_spec(a.g, gs, s->n);
+    for (uint32_t j = 0; j < Dr; ++j) {
+        if (gs.perm[j] == j) continue;
+        a.off[a.nent] = deposit(j, gs.tq, gs.kt);
+        a.src[a.nent] = deposit(gs.perm[j], gs.tq, gs.kt);
+        ++a.nent;
+    }
+    if (a.nent == 0) return QVMHIP_OK;
+    constexpr int GPT = 2;
+    hipLaunchKernelGGL((k_perm<T, GPT>), dim3(grid_for(a.g.ngroups, (uint64_t)kBlock * GPT)),
+                       dim3(kBlock), 0, s->stream, (vec2_t<T> *)s->amps, a);
+    QH_LAUNCH_CHECK();
+    return QVMHIP_OK;
+}
+
+template <typename T>
+int run_split(qvmhip_state *s, const GateSplit &gs) {
+    switch (gs.kind) {
+        case 0:
+            return QVMHIP_OK;
+        case 2:
+            return run_diag<T>(s, gs);
+        case 3:
+            return run_perm<T>(s, gs);
+        default:
+            break;
+    }
+    switch (gs.kt) {
+        case 0: {  // dense 1x1 is a phase on the control subspace
+            GateSplit d = gs;
+            d.kind = 2;
+            return run_diag<T>(s, d);
+        }
+        case 1:
+            return run_dense_small<T, 1>(s, gs);
+        case 2:
+            return run_dense_small<T, 2>(s, gs);
+        case 3:
+            return run_dense_small<T, 3>(s, gs);
+        case 4:
+            return run_dense_big<T, 4>(s, gs);
+        case 5:
+            return run_dense_big<T, 5>(s, gs);
+        case 6:
+            return run_dense_big<T, 6>(s, gs);
+    }
+    return fail(QVMHIP_EINVAL, "gate wider than 6 qubits");
+}
+
+}  // namespace
+
+int launch_gate(qvmhip_state *s, const double *U, uint32_t k, const uint32_t *qubits,
+                int force_dense) {
+    const GateSplit gs = split_gate(reinterpret_cast<const cd *>(U), k, qubits, !force_dense);
+    if (s->dtype == QVMHIP_F64) return run_split<double>(s, gs);
+    return run_split<float>(s, gs);
+}
+
+int launch_diagonal(qvmhip_state *s, const double *d, uint32_t k, const uint32_t *qubits) {
+    const uint32_t D = 1u << k;
+    std::vector<cd> U((size_t)D * D, cd(0, 0));
+    for (uint32_t j = 0; j < D; ++j) U[(size_t)j * D + j] = cd(d[2 * j], d[2 * j + 1]);
+    return launch_gate(s, reinterpret_cast<const double *>(U.data()), k, qubits, 0);
+}
+
+int launch_permutation(qvmhip_state *s, const uint32_t *perm, uint32_t k, const uint32_t *qubits) {
+    const uint32_t D = 1u << k;
+    std::vector<char> seen(D, 0);
+    for (uint32_t j = 0; j < D; ++j) {
+        if (perm[j] >= D || seen[perm[j]]) return fail(QVMHIP_EINVAL, "not a permutation");
+        seen[perm[j]] = 1;
+    }
+    std::vector<cd> U((size_t)D * D, cd(0, 0));
+    for (uint32_t j = 0; j < D; ++j) U[(size_t)j * D + perm[j]] = cd(1, 0);
+    return launch_gate(s, reinterpret_cast<const double *>(U.data()), k, qubits, 0);
+}
+
+int launch_fill_zero_state(qvmhip_state *s) {
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((s->len + kBlock - 1) / kBlock, 256 * 64);
+    if (s->dtype == QVMHIP_F64)
+        hipLaunchKernelGGL(k_fill_zero_state<double>, dim3(grid), dim3(kBlock), 0, s->stream,
+                           (double2 *)s->amps, s->len);
+    else
+        hipLaunchKernelGGL(k_fill_zero_state<float>, dim3(grid), dim3(kBlock), 0, s->stream,
+                           (float2 *)s->amps, s->len);
+    QH_LAUNCH_CHECK();
+    return QVMHIP_OK;
+}
+
+int launch_scale(qvmhip_state *s, double factor) {
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((s->len + kBlock - 1) / kBlock, 256 * 64);
+    if (s->dtype == QVMHIP_F64)
+        hipLaunchKernelGGL(k_scale<double>, dim3(grid), dim3(kBlock), 0, s->stream,
+                           (double2 *)s->amps, s->len, factor);
+    else
+        hipLaunchKernelGGL(k_scale<float>, dim3(grid), dim3(kBlock), 0, s->stream, (float2 *)s->amps,
+                           s->len, (float)factor);
+    QH_LAUNCH_CHECK();
+    return QVMHIP_OK;
+}
+
+}  // namespace qvmhip

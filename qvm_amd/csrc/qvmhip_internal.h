@@ -1,0 +1,119 @@
+// qvmhip_internal.h -- shared definitions of the qvmhip engine (CDNA4 / gfx950).
+//
+// Statevector layout in HBM: 2^n interleaved complex amplitudes, qubit q =
+// bit q of the index.  FP64 states are double2 (16 B/amp), FP32 states are
+// float2 (8 B/amp).  All index arithmetic is 64-bit so 34-qubit FP32 states
+// (2^34 amps, 128 GiB) are addressable.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/qvmhip.h"
+
+namespace qvmhip {
+
+constexpr int kBlock = 256;         // threads per workgroup for streaming kernels
+constexpr int kMaxK = 6;            // widest dense block (64 x 64)
+constexpr int kMaxSorted = 16;      // targets + controls of one gate
+constexpr int kRedBlocks = 2048;    // partial sums of one reduction launch
+constexpr int kChunkLog2 = 8;       // sampling chunk: 256 amplitudes
+
+template <typename T> struct Vec2;
+template <> struct Vec2<double> { using type = double2; };
+template <> struct Vec2<float> { using type = float2; };
+template <typename T> using vec2_t = typename Vec2<T>::type;
+
+// Complement iteration: inject zeros at `sorted` (ascending) positions, then
+// OR the control mask.  This is map-complement (src/wavefunction.lisp:155-175)
+// with the control qubits of a controlled gate pinned to 1.
+struct GroupSpec {
+    uint32_t nsorted;
+    uint32_t sorted[kMaxSorted];
+    uint64_t ctrl_mask;
+    uint64_t ngroups;
+};
+
+__host__ __device__ inline uint64_t insert_zero(uint64_t x, uint32_t q) {
+    const uint64_t lo = x & ((uint64_t(1) << q) - 1);
+    return ((x >> q) << (q + 1)) | lo;
+}
+
+__device__ inline uint64_t group_base(const GroupSpec &g, uint64_t i) {
+#pragma unroll 1
+    for (uint32_t j = 0; j < g.nsorted; ++j) i = insert_zero(i, g.sorted[j]);
+    return i | g.ctrl_mask;
+}
+
+// Error reporting: thread-local message + status code.
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+}  // namespace qvmhip
+
+// The state handle behind the opaque qvmhip_state of the C ABI.
+struct qvmhip_state {
+    uint32_t n = 0;
+    int dtype = QVMHIP_F64;
+    int device = 0;
+    uint64_t len = 0;          // 2^n
+    size_t amp_bytes = 16;     // 16 (F64) or 8 (F32)
+    void *amps = nullptr;      // device statevector
+    hipStream_t stream = nullptr;
+    bool own_stream = true;
+    // scratch
+    double *red = nullptr;          // kRedBlocks partials + results
+    double *host = nullptr;         // pinned host scalars (64 doubles)
+    void *mat = nullptr;            // device matrix slot (2 x 64 KiB)
+    double *chunk_sum = nullptr;    // sampling: per-chunk probability sums
+    double *chunk_pre = nullptr;    // sampling: exclusive prefix of chunk sums
+    double *scan_tmp = nullptr;     // scan block sums
+    uint64_t nchunks = 0;
+    uint64_t *shot_out = nullptr;   // device shot results
+    double *shot_u = nullptr;       // device uniforms
+    uint64_t shot_cap = 0;
+    uint64_t *host_shots = nullptr; // pinned
+    double *host_u = nullptr;       // pinned
+    void *stage = nullptr;          // pinned host staging for read/write
+    size_t stage_bytes = 0;
+    uint32_t flags = 0;
+    uint64_t counters[4] = {0, 0, 0, 0};
+};
+
+namespace qvmhip {
+// Launch helpers implemented in the kernel files.
+int launch_gate(qvmhip_state *s, const double *U, uint32_t k, const uint32_t *qubits, int force_dense);
+int launch_diagonal(qvmhip_state *s, const double *d, uint32_t k, const uint32_t *qubits);
+int launch_permutation(qvmhip_state *s, const uint32_t *perm, uint32_t k, const uint32_t *qubits);
+int launch_fill_zero_state(qvmhip_state *s);
+int launch_scale(qvmhip_state *s, double factor);
+int reduce_probability(qvmhip_state *s, int mode, uint32_t q, double *out_dev);
+int reduce_norm2(qvmhip_state *s, double *out_dev);
+int launch_force_measure(qvmhip_state *s, uint32_t q, int bit, double inv_norm, int compiled_form);
+int sample_shots(qvmhip_state *s, const double *u, uint64_t nshots, uint64_t *out, int rule_ge);
+int collapse_to_basis(qvmhip_state *s, const uint64_t *index_dev);
+int launch_tile_pass(qvmhip_state *s, uint32_t m, const uint32_t *tile_qubits, uint32_t nops,
+                     const qvmhip_tile_op *ops_dev_or_host, int ops_on_device);
+int launch_fused_dense(qvmhip_state *s, const void *mat_dev, uint32_t k, const uint32_t *qubits);
+int expectation_zz(qvmhip_state *s, const uint32_t *edges, uint32_t nedges, double *out);
+int inner_product(qvmhip_state *a, qvmhip_state *b, double *re, double *im);
+int probabilities(qvmhip_state *s, uint64_t start, uint64_t count, double *host_out);
+int convert_f32_to_f64(qvmhip_state *s, uint64_t start, uint64_t count, double *dev_out);
+int convert_f64_to_f32(qvmhip_state *s, uint64_t start, uint64_t count, const double *dev_in);
+}  // namespace qvmhip
+
+#define QH_CHECK(expr)                                                                    \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess)                                                             \
+            return ::qvmhip::fail(QVMHIP_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+#define QH_LAUNCH_CHECK()                                                                 \
+    do {                                                                                  \
+        hipError_t _e = hipGetLastError();                                                \
+        if (_e != hipSuccess)                                                             \
+            return ::qvmhip::fail(QVMHIP_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(_e)); \
+    } while (0)

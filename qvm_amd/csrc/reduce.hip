@@ -1,0 +1,583 @@
+// reduce.hip -- probability reductions, measurement collapse and sampling.
+//
+// Reductions accumulate |psi_i|^2 in FP64 for both state dtypes: each thread
+// sums a grid-strided slice, a wavefront reduces with cross-lane shuffles, the
+// waves of a block combine in LDS, and one block adds the block partials in a
+// fixed order (deterministic: same input -> same bits).
+//
+// Sampling (measure-all and multi-shot) is a two-level search over the
+// cumulative distribution C(b) = sum_{i<=b} |psi_i|^2:
+//   1. chunk sums S[c] over chunks of 256 amplitudes (one wavefront per chunk),
+//   2. exclusive prefix P[c] of S (device-wide scan),
+//   3. per draw p, one wavefront: binary search over P for the chunk, then a
+//      wavefront prefix scan of the chunk's 256 probabilities, C(b) =
+//      P[c] + in-chunk prefix, and a ballot picks the first b that satisfies
+//      the rule: C(b) > p for measure-all (src/measurement.lisp:179-225) or
+//      C(b) >= p for multi-shot sampling (src/measurement.lisp:246-288).
+//   The result is clamped to len-1 when no b qualifies, as in the reference.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "qvmhip_internal.h"
+
+namespace qvmhip {
+
+constexpr int kWave = 64;
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Inclusive prefix over the 64 lanes (Hillis-Steele with shuffles).
+__device__ inline double wave_inclusive_scan(double v, int lane) {
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const double t = __shfl_up(v, o, kWave);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+__device__ inline double block_sum(double v, double *sm) {
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    v = wave_sum(v);
+    if (lane == 0) sm[wid] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += sm[w];
+    }
+    return t;  // valid in thread 0
+}
+
+template <typename T>
+__device__ inline double prob(const vec2_t<T> &a) {
+    const double re = (double)a.x, im = (double)a.y;
+    return re * re + im * im;
+}
+
+// mode 0: all amplitudes; 1: bit q = 1 half; 2: bit q = 0 half.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_prob_partial(const vec2_t<T> *__restrict__ psi,
+                                                         uint64_t count, int mode, uint32_t q,
+                                                         double *__restrict__ partial) {
+    __shared__ double sm[kBlock / kWave];
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    const uint64_t qbit = uint64_t(1) << q;
+    double acc = 0.0;
+    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    for (; i + 3 * stride < count; i += 4 * stride) {
+        vec2_t<T> v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            uint64_t a = i + u * stride;
+            if (mode == 1) a = insert_zero(a, q) | qbit;
+            if (mode == 2) a = insert_zero(a, q);
+            v[u] = psi[a];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc += prob<T>(v[u]);
+    }
+    for (; i < count; i += stride) {
+        uint64_t a = i;
+        if (mode == 1) a = insert_zero(a, q) | qbit;
+        if (mode == 2) a = insert_zero(a, q);
+        acc += prob<T>(psi[a]);
+    }
+    const double t = block_sum(acc, sm);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// sum over edges of <Z_a Z_b>: weight(i) = nedges - 2 * #cut edges.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_zz_partial(const vec2_t<T> *__restrict__ psi,
+                                                       uint64_t count, const uint32_t *edges,
+                                                       uint32_t nedges, double *__restrict__ partial) {
+    __shared__ double sm[kBlock / kWave];
+    __shared__ uint32_t se[2 * 256];
+    for (uint32_t e = threadIdx.x; e < 2 * nedges && e < 512; e += kBlock) se[e] = edges[e];
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    double acc = 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += stride) {
+        int cut = 0;
+        for (uint32_t e = 0; e < nedges; ++e) cut += (int)(((i >> se[2 * e]) ^ (i >> se[2 * e + 1])) & 1u);
+        acc += prob<T>(psi[i]) * (double)((int)nedges - 2 * cut);
+    }
+    const double t = block_sum(acc, sm);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dot_partial(const vec2_t<T> *__restrict__ a,
+                                                        const vec2_t<T> *__restrict__ b,
+                                                        uint64_t count, double *__restrict__ partial) {
+    __shared__ double sm[kBlock / kWave];
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    double re = 0.0, im = 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += stride) {
+        const double ar = a[i].x, ai = a[i].y, br = b[i].x, bi = b[i].y;
+        re += ar * br + ai * bi;  // conj(a) * b
+        im += ar * bi - ai * br;
+    }
+    const double tr = block_sum(re, sm);
+    __syncthreads();
+    const double ti = block_sum(im, sm);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = tr;
+        partial[2 * blockIdx.x + 1] = ti;
+    }
+}
+
+// One block adds `n` partials (stride `stride`, `width` interleaved values) in
+// a fixed order.
+__global__ __launch_bounds__(kBlock) void k_final_sum(const double *__restrict__ partial, int n,
+                                                      int width, double *__restrict__ out) {
+    __shared__ double sm[kBlock / kWave];
+    for (int w = 0; w < width; ++w) {
+        double acc = 0.0;
+        for (int i = threadIdx.x; i < n; i += kBlock) acc += partial[(size_t)i * width + w];
+        const double t = block_sum(acc, sm);
+        if (threadIdx.x == 0) out[w] = t;
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_force_measure(vec2_t<T> *__restrict__ psi, uint64_t len,
+                                                          uint32_t q, int bit, double inv_norm,
+                                                          int compiled_form) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    const T f = (T)inv_norm;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < len; i += stride) {
+        vec2_t<T> v = psi[i];
+        const int ab = (int)((i >> q) & 1u);
+        if (compiled_form) {
+            // src/compile-gate.lisp:248-254: amp *= inv_norm * [bit matches]
+            const T g = f * (T)(ab == bit ? 1 : 0);
+            v.x = g * v.x;
+            v.y = g * v.y;
+        } else if (ab != bit) {
+            // src/measurement.lisp:29-39: annihilated half -> 0, kept half *= inv_norm
+            v.x = T(0);
+            v.y = T(0);
+        } else {
+            v.x = f * v.x;
+            v.y = f * v.y;
+        }
+        psi[i] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Sampling.
+
+// Lane l of the chunk's wavefront holds amplitudes 4l .. 4l+3.  The chunk
+// total S[c] is lane 63's inclusive scan value; k_sample_search rebuilds the
+// same lane sums, so its in-chunk prefix agrees with S[c] to rounding.
+template <typename T>
+__device__ inline void chunk_lane_probs(const vec2_t<T> *__restrict__ psi, uint64_t len,
+                                        uint64_t chunk, int lane, double p[4]) {
+    const uint64_t first = (chunk << kChunkLog2) + (uint64_t)lane * 4;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) p[u] = (first + u < len) ? prob<T>(psi[first + u]) : 0.0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_chunk_sums(const vec2_t<T> *__restrict__ psi,
+                                                       uint64_t len, uint64_t nchunks,
+                                                       double *__restrict__ sums) {
+    const int lane = threadIdx.x % kWave;
+    const uint64_t chunk = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    if (chunk >= nchunks) return;
+    double p[4];
+    chunk_lane_probs<T>(psi, len, chunk, lane, p);
+    const double local = ((p[0] + p[1]) + p[2]) + p[3];
+    const double incl = wave_inclusive_scan(local, lane);
+    if (lane == kWave - 1) sums[chunk] = incl;
+}
+
+// Device-wide exclusive scan of doubles, three phases.  Phase 1: totals of
+// segments of kScanSeg values.
+constexpr int kScanSeg = 1024;
+
+__global__ __launch_bounds__(kBlock) void k_scan_seg_totals(const double *__restrict__ in, uint64_t n,
+                                                            double *__restrict__ tot) {
+    __shared__ double sm[kBlock / kWave];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanSeg;
+    double acc = 0.0;
+    // each thread: 4 consecutive values, sequential
+    const uint64_t j = base + (uint64_t)threadIdx.x * 4;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+        if (j + u < n) acc += in[j + u];
+    // totals must be the same sums phase 3 builds: lane scan order
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    const double incl = wave_inclusive_scan(acc, lane);
+    if (lane == kWave - 1) sm[wid] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) t += sm[w];
+        tot[blockIdx.x] = t;
+    }
+}
+
+// Phase 2: one block, exclusive scan of the segment totals in place.
+__global__ __launch_bounds__(kBlock) void k_scan_tops(double *__restrict__ tot, uint64_t nseg) {
+    __shared__ double sm[kBlock];
+    // each thread owns a contiguous run
+    const uint64_t per = (nseg + kBlock - 1) / kBlock;
+    const uint64_t lo = (uint64_t)threadIdx.x * per;
+    const uint64_t hi = std::min<uint64_t>(lo + per, nseg);
+    double acc = 0.0;
+    for (uint64_t i = lo; i < hi; ++i) acc += tot[i];
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double run = 0.0;
+        for (int t = 0; t < kBlock; ++t) {
+            const double v = sm[t];
+            sm[t] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    double run = sm[threadIdx.x];
+    for (uint64_t i = lo; i < hi; ++i) {
+        const double v = tot[i];
+        tot[i] = run;
+        run += v;
+    }
+}
+
+// Phase 3: exclusive prefix of every value = segment offset + in-segment scan.
+__global__ __launch_bounds__(kBlock) void k_scan_apply(const double *__restrict__ in, uint64_t n,
+                                                       const double *__restrict__ seg_off,
+                                                       double *__restrict__ out) {
+    __shared__ double sm[kBlock / kWave];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanSeg;
+    const uint64_t j = base + (uint64_t)threadIdx.x * 4;
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = (j + u < n) ? in[j + u] : 0.0;
+    const double acc = ((v[0] + v[1]) + v[2]) + v[3];
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    const double incl = wave_inclusive_scan(acc, lane);
+    const double up = __shfl_up(incl, 1, kWave);
+    if (lane == kWave - 1) sm[wid] = incl;
+    __syncthreads();
+    double woff = 0.0;
+    for (int w = 0; w < wid; ++w) woff += sm[w];
+    double run = seg_off[blockIdx.x] + (woff + (lane ? up : 0.0));
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        if (j + u < n) out[j + u] = run;
+        run += v[u];
+    }
+}
+
+// One wavefront per draw.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sample_search(const vec2_t<T> *__restrict__ psi,
+                                                          uint64_t len, uint64_t nchunks,
+                                                          const double *__restrict__ pre,
+                                                          const double *__restrict__ u,
+                                                          uint64_t nshots, int rule_ge,
+                                                          uint64_t *__restrict__ out) {
+    const int lane = threadIdx.x % kWave;
+    const uint64_t shot = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    if (shot >= nshots) return;
+    const double p = u[shot];
+    // last chunk c with pre[c] <= p (rule >) or pre[c] < p (rule >=); 0 if none
+    uint64_t lo = 0, hi = nchunks;  // invariant: answer in [lo, hi)
+    while (hi - lo > 1) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        const double v = pre[mid];
+        const bool ok = rule_ge ? (v < p) : (v <= p);
+        if (ok)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    uint64_t result = len - 1;
+    bool found = false;
+    for (uint64_t c = lo; c < nchunks && !found; ++c) {
+        double pr[4];
+        chunk_lane_probs<T>(psi, len, c, lane, pr);
+        const double local = ((pr[0] + pr[1]) + pr[2]) + pr[3];
+        const double incl = wave_inclusive_scan(local, lane);
+        const double up = __shfl_up(incl, 1, kWave);
+        double run = lane ? up : 0.0;  // exclusive lane offset
+        int hit = -1;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            run += pr[k];
+            const double C = pre[c] + run;
+            const bool ok = rule_ge ? (C >= p) : (C > p);
+            const uint64_t idx = (c << kChunkLog2) + (uint64_t)lane * 4 + k;
+            if (ok && hit < 0 && idx < len) hit = k;
+        }
+        const unsigned long long mask = __ballot(hit >= 0);
+        if (mask) {
+            const int first = __ffsll((long long)mask) - 1;
+            const int k = __shfl(hit, first, kWave);
+            result = (c << kChunkLog2) + (uint64_t)first * 4 + (uint64_t)k;
+            found = true;
+        }
+    }
+    if (lane == 0) out[shot] = result;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_collapse_basis(vec2_t<T> *__restrict__ psi, uint64_t len,
+                                                           const uint64_t *__restrict__ index) {
+    const uint64_t b = *index;
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < len; i += stride) {
+        vec2_t<T> z;
+        z.x = (i == b) ? T(1) : T(0);
+        z.y = T(0);
+        psi[i] = z;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_probs(const vec2_t<T> *__restrict__ psi, uint64_t start,
+                                                  uint64_t count, double *__restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += stride)
+        out[i] = prob<T>(psi[start + i]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_widen(const float2 *__restrict__ in, uint64_t count,
+                                                  double2 *__restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += stride) {
+        const float2 v = in[i];
+        out[i] = make_double2((double)v.x, (double)v.y);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_narrow(const double2 *__restrict__ in, uint64_t count,
+                                                   float2 *__restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += stride) {
+        const double2 v = in[i];
+        out[i] = make_float2((float)v.x, (float)v.y);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+
+namespace {
+uint32_t red_grid(uint64_t count) {
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((count + kBlock - 1) / kBlock, kRedBlocks));
+}
+uint32_t sweep_grid(uint64_t count) {
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((count + kBlock - 1) / kBlock, 256 * 64));
+}
+}  // namespace
+
+int reduce_probability(qvmhip_state *s, int mode, uint32_t q, double *out_dev) {
+    const uint64_t count = mode == 0 ? s->len : s->len / 2;
+    const uint32_t grid = red_grid(count);
+    if (s->dtype == QVMHIP_F64)
+        hipLaunchKernelGGL(k_prob_partial<double>, dim3(grid), dim3(kBlock), 0, s->stream,
+                           (const double2 *)s->amps, count, mode, q, s->red);
+    else
+        hipLaunchKernelGGL(k_prob_partial<float>, dim3(grid), dim3(kBlock), 0, s->stream,
+                           (const float2 *)s->amps, count, mode, q, s->red);
+    QH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(kBlock), 0, s->stream, s->red, (int)grid, 1, out_dev);
+    QH_LAUNCH_CHECK();
+    return QVMHIP_OK;
+}
+
+int reduce_norm2(qvmhip_state *s, double *out_dev) { return reduce_probability(s, 0, 0, out_dev); }
+
+int launch_force_measure(qvmhip_state *s, uint32_t q, int bit, double inv_norm, int compiled_form) {
+    const uint32_t grid = sweep_grid(s->len);
+    if (s->dtype == QVMHIP_F64)
+        hipLaunchKernelGGL(k_force_measure<double>, dim3(grid), dim3(kBlock), 0, s->stream,
+                           (double2 *)s->amps, s->len, q, bit, inv_norm, compiled_form);
+    else
+        hipLaunchKernelGGL(k_force_measure<float>, dim3(grid), dim3(kBlock), 0, s->stream,
+                           (float2 *)s->amps, s->len, q, bit, inv_norm, compiled_form);
+    QH_LAUNCH_CHECK();
+    return QVMHIP_OK;
+}
+
+// Builds chunk sums + exclusive chunk prefix in s->chunk_sum / s->chunk_pre.
+static int build_chunk_prefix(qvmhip_state *s) {
+    const uint64_t nch = s->nchunks;
+    const uint32_t g1 = (uint32_t)((nch + (kBlock / kWave) - 1) / (kBlock / kWave));
+    if (s->dtype == QVMHIP_F64)
+        hipLaunchKernelGGL(k_chunk_sums<double>, dim3(g1), dim3(kBlock), 0, s->stream,
+                           (const double2 *)s->amps, s->len, nch, s->chunk_sum);
+    else
+        hipLaunchKernelGGL(k_chunk_sums<float>, dim3(g1), dim3(kBlock), 0, s->stream,
+                           (const float2 *)s->amps, s->len, nch, s->chunk_sum);
+    QH_LAUNCH_CHECK();
+    const uint64_t nseg = (nch + kScanSeg - 1) / kScanSeg;
+    hipLaunchKernelGGL(k_scan_seg_totals, dim3((uint32_t)nseg), dim3(kBlock), 0, s->stream, s->chunk_sum,
+                       nch, s->scan_tmp);
+    QH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_tops, dim3(1), dim3(kBlock), 0, s->stream, s->scan_tmp, nseg);
+    QH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_apply, dim3((uint32_t)nseg), dim3(kBlock), 0, s->stream, s->chunk_sum, nch,
+                       s->scan_tmp, s->chunk_pre);
+    QH_LAUNCH_CHECK();
+    return QVMHIP_OK;
+}
+
+static int ensure_shots(qvmhip_state *s, uint64_t nshots) {
+    if (nshots <= s->shot_cap) return QVMHIP_OK;
+    if (s->shot_out) hipFree(s->shot_out);
+    if (s->shot_u) hipFree(s->shot_u);
+    s->shot_out = nullptr;
+    s->shot_u = nullptr;
+    s->shot_cap = 0;
+    QH_CHECK(hipMalloc((void **)&s->shot_out, nshots * sizeof(uint64_t)));
+    QH_CHECK(hipMalloc((void **)&s->shot_u, nshots * sizeof(double)));
+    s->shot_cap = nshots;
+    return QVMHIP_OK;
+}
+
+// Draws nshots samples; u and out are host arrays (out may be null when the
+// caller only needs the device copy in s->shot_out).
+int sample_shots(qvmhip_state *s, const double *u, uint64_t nshots, uint64_t *out, int rule_ge) {
+    if (nshots == 0) return QVMHIP_OK;
+    // Draws and results travel through the pinned staging buffer in batches.
+    const uint64_t batch = std::min<uint64_t>(nshots, s->stage_bytes / 16);
+    int rc = ensure_shots(s, batch);
+    if (rc) return rc;
+    rc = build_chunk_prefix(s);
+    if (rc) return rc;
+    double *stage_u = (double *)s->stage;
+    uint64_t *stage_out = (uint64_t *)((char *)s->stage + s->stage_bytes / 2);
+    for (uint64_t first = 0; first < nshots; first += batch) {
+        const uint64_t c = std::min<uint64_t>(batch, nshots - first);
+        QH_CHECK(hipStreamSynchronize(s->stream));  // staging buffer is free
+        std::memcpy(stage_u, u + first, c * sizeof(double));
+        QH_CHECK(hipMemcpyAsync(s->shot_u, stage_u, c * sizeof(double), hipMemcpyHostToDevice, s->stream));
+        const uint64_t wpb = kBlock / kWave;
+        const uint64_t nb = (c + wpb - 1) / wpb;
+        if (s->dtype == QVMHIP_F64)
+            hipLaunchKernelGGL(k_sample_search<double>, dim3((uint32_t)nb), dim3(kBlock), 0, s->stream,
+                               (const double2 *)s->amps, s->len, s->nchunks, s->chunk_pre, s->shot_u, c,
+                               rule_ge, s->shot_out);
+        else
+            hipLaunchKernelGGL(k_sample_search<float>, dim3((uint32_t)nb), dim3(kBlock), 0, s->stream,
+                               (const float2 *)s->amps, s->len, s->nchunks, s->chunk_pre, s->shot_u, c,
+                               rule_ge, s->shot_out);
+        QH_LAUNCH_CHECK();
+        if (out) {
+            QH_CHECK(hipMemcpyAsync(stage_out, s->shot_out, c * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                    s->stream));
+            QH_CHECK(hipStreamSynchronize(s->stream));
+            std::memcpy(out + first, stage_out, c * sizeof(uint64_t));
+        }
+    }
+    return QVMHIP_OK;
+}
+
+int collapse_to_basis(qvmhip_state *s, const uint64_t *index_dev) {
+    const uint32_t grid = sweep_grid(s->len);
+    if (s->dtype == QVMHIP_F64)
+        hipLaunchKernelGGL(k_collapse_basis<double>, dim3(grid), dim3(kBlock), 0, s->stream,
+                           (double2 *)s->amps, s->len, index_dev);
+    else
+        hipLaunchKernelGGL(k_collapse_basis<float>, dim3(grid), dim3(kBlock), 0, s->stream,
+                           (float2 *)s->amps, s->len, index_dev);
+    QH_LAUNCH_CHECK();
+    return QVMHIP_OK;
+}
+
+int expectation_zz(qvmhip_state *s, const uint32_t *edges, uint32_t nedges, double *out) {
+    if (nedges > 256) return fail(QVMHIP_EINVAL, "at most 256 edges");
+    uint32_t *dedges = nullptr;
+    QH_CHECK(hipMalloc((void **)&dedges, sizeof(uint32_t) * 2 * std::max<uint32_t>(nedges, 1)));
+    if (nedges)
+        QH_CHECK(hipMemcpyAsync(dedges, edges, sizeof(uint32_t) * 2 * nedges, hipMemcpyHostToDevice,
+                                s->stream));
+    const uint32_t grid = red_grid(s->len);
+    if (s->dtype == QVMHIP_F64)
+        hipLaunchKernelGGL(k_zz_partial<double>, dim3(grid), dim3(kBlock), 0, s->stream,
+                           (const double2 *)s->amps, s->len, dedges, nedges, s->red);
+    else
+        hipLaunchKernelGGL(k_zz_partial<float>, dim3(grid), dim3(kBlock), 0, s->stream,
+                           (const float2 *)s->amps, s->len, dedges, nedges, s->red);
+    QH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(kBlock), 0, s->stream, s->red, (int)grid, 1,
+                       s->red + kRedBlocks * 2);
+    QH_LAUNCH_CHECK();
+    QH_CHECK(hipMemcpyAsync(s->host, s->red + kRedBlocks * 2, sizeof(double), hipMemcpyDeviceToHost,
+                            s->stream));
+    QH_CHECK(hipStreamSynchronize(s->stream));
+    hipFree(dedges);
+    *out = s->host[0];
+    return QVMHIP_OK;
+}
+
+int inner_product(qvmhip_state *a, qvmhip_state *b, double *re, double *im) {
+    const uint32_t grid = red_grid(a->len);
+    if (a->dtype == QVMHIP_F64)
+        hipLaunchKernelGGL(k_dot_partial<double>, dim3(grid), dim3(kBlock), 0, a->stream,
+                           (const double2 *)a->amps, (const double2 *)b->amps, a->len, a->red);
+    else
+        hipLaunchKernelGGL(k_dot_partial<float>, dim3(grid), dim3(kBlock), 0, a->stream,
+                           (const float2 *)a->amps, (const float2 *)b->amps, a->len, a->red);
+    QH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(kBlock), 0, a->stream, a->red, (int)grid, 2,
+                       a->red + kRedBlocks * 2);
+    QH_LAUNCH_CHECK();
+    QH_CHECK(hipMemcpyAsync(a->host, a->red + kRedBlocks * 2, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                            a->stream));
+    QH_CHECK(hipStreamSynchronize(a->stream));
+    *re = a->host[0];
+    *im = a->host[1];
+    return QVMHIP_OK;
+}
+
+int probabilities(qvmhip_state *s, uint64_t start, uint64_t count, double *host_out) {
+    // chunked through the pinned staging buffer
+    const uint64_t per = s->stage_bytes / sizeof(double);
+    double *dev = nullptr;
+    QH_CHECK(hipMalloc((void **)&dev, per * sizeof(double)));
+    for (uint64_t off = 0; off < count; off += per) {
+        const uint64_t c = std::min<uint64_t>(per, count - off);
+        if (s->dtype == QVMHIP_F64)
+            hipLaunchKernelGGL(k_probs<double>, dim3(sweep_grid(c)), dim3(kBlock), 0, s->stream,
+                               (const double2 *)s->amps, start + off, c, dev);
+        else
+            hipLaunchKernelGGL(k_probs<float>, dim3(sweep_grid(c)), dim3(kBlock), 0, s->stream,
+                               (const float2 *)s->amps, start + off, c, dev);
+        QH_LAUNCH_CHECK();
+        QH_CHECK(hipMemcpyAsync(s->stage, dev, c * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        QH_CHECK(hipStreamSynchronize(s->stream));
+        std::memcpy(host_out + off, s->stage, c * sizeof(double));
+    }
+    hipFree(dev);
+    return QVMHIP_OK;
+}
+
+int convert_f32_to_f64(qvmhip_state *s, uint64_t start, uint64_t count, double *dev_out) {
+    hipLaunchKernelGGL(k_widen, dim3(sweep_grid(count)), dim3(kBlock), 0, s->stream,
+                       (const float2 *)s->amps + start, count, (double2 *)dev_out);
+    QH_LAUNCH_CHECK();
+    return QVMHIP_OK;
+}
+
+int convert_f64_to_f32(qvmhip_state *s, uint64_t start, uint64_t count, const double *dev_in) {
+    hipLaunchKernelGGL(k_narrow, dim3(sweep_grid(count)), dim3(kBlock), 0, s->stream,
+                       (const double2 *)dev_in, count, (float2 *)s->amps + start);
+    QH_LAUNCH_CHECK();
+    return QVMHIP_OK;
+}
+
+}  // namespace qvmhip

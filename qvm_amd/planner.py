@@ -1,0 +1,354 @@
+"""Execution planners: turn a gate list into engine launches.
+
+Three plans, all consumed by ``compile_gates``:
+
+* ``plain``  -- one engine launch per gate (apply-matrix-operator per gate,
+  src/wavefunction.lisp:308-330).  The judge of the other two.
+* ``tiled``  -- gates grouped into LDS tile passes: a pass names up to m tile
+  qubits and applies every gate that fits, in dependency order, to each
+  2^m-amplitude block while it sits in LDS.  Per-gate arithmetic is the
+  plain kernels', so FP64 results are bit-identical to ``plain`` (unless
+  ``merge_1q`` multiplies runs of 1-qubit gates together first).
+* ``fused``  -- gates multiplied into dense 2^k x 2^k blocks (k <= kmax) that
+  the MFMA kernel applies in one sweep each; the fusion of
+  compile-loaded-program with *fuse-gates-during-compilation*
+  (src/qvm.lisp:166-175).  Rounding differs from plain at the 1e-16 level.
+
+Both grouping planners use the same dependency rule: a gate may join the
+current group once every earlier gate that shares a qubit with it has been
+scheduled, so gates on disjoint qubits are reordered freely and the result is
+the same unitary.
+
+Gates are (U, quil_qubits) pairs: U in the Quil convention (first argument is
+the most significant matrix bit).
+"""
+from __future__ import annotations
+
+from collections import deque
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _native as N
+from . import gates as G
+
+Gate = Tuple[np.ndarray, Sequence[int]]
+
+
+# ---------------------------------------------------------------------------
+# Dependency-ordered grouping
+
+
+class _Ready:
+    """Tracks which gates are ready: all earlier gates on their qubits done."""
+
+    def __init__(self, gates_qubits: List[Tuple[int, ...]], nq: int):
+        self.qs = gates_qubits
+        self.queues = [deque() for _ in range(nq)]
+        for i, qs in enumerate(gates_qubits):
+            for q in qs:
+                self.queues[q].append(i)
+        self.done = [False] * len(gates_qubits)
+        self.remaining = len(gates_qubits)
+
+    def is_ready(self, i: int) -> bool:
+        return all(self.queues[q] and self.queues[q][0] == i for q in self.qs[i])
+
+    def ready(self) -> List[int]:
+        heads = set()
+        for dq in self.queues:
+            if dq:
+                heads.add(dq[0])
+        return sorted(i for i in heads if self.is_ready(i))
+
+    def take(self, i: int) -> None:
+        for q in self.qs[i]:
+            assert self.queues[q][0] == i
+            self.queues[q].popleft()
+        self.done[i] = True
+        self.remaining -= 1
+
+
+def group_gates(gates_qubits: List[Tuple[int, ...]], nq: int, width: int, seed_qubits=(),
+                standalone=lambda i: False) -> List[Tuple[str, List[int], List[int]]]:
+    """Greedy dependency-aware grouping.
+
+    Returns a list of (kind, qubit set, gate indices in application order);
+    kind is "group" or "single" (a gate that ``standalone`` excludes from
+    groups, applied on its own).
+    """
+    R = _Ready(gates_qubits, nq)
+    out = []
+    while R.remaining:
+        ready = R.ready()
+        single = [i for i in ready if standalone(i)]
+        if single and all(standalone(i) for i in ready):
+            i = single[0]
+            R.take(i)
+            out.append(("single", list(gates_qubits[i]), [i]))
+            continue
+        T = set(q for q in seed_qubits if q < nq)
+        members: List[int] = []
+        while True:
+            progressed = False
+            for i in R.ready():
+                if standalone(i):
+                    continue
+                if set(gates_qubits[i]) <= T:
+                    R.take(i)
+                    members.append(i)
+                    progressed = True
+            if progressed:
+                continue
+            # extend the tile with the qubits of the earliest ready gate that fits
+            best = None
+            for i in R.ready():
+                if standalone(i):
+                    continue
+                new = set(gates_qubits[i]) - T
+                if len(T) + len(new) <= width:
+                    if best is None or len(new) < len(set(gates_qubits[best]) - T):
+                        best = i
+            if best is None:
+                break
+            T |= set(gates_qubits[best])
+        if not members:
+            # nothing fits an empty group: take the earliest ready gate alone
+            i = R.ready()[0]
+            R.take(i)
+            out.append(("single", list(gates_qubits[i]), [i]))
+            continue
+        out.append(("group", sorted(T), members))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Tile ops
+
+
+def _is_diag(U):
+    return np.count_nonzero(U - np.diag(np.diag(U))) == 0
+
+
+def tile_op(U: np.ndarray, quil_qubits: Sequence[int], local) -> N.TileOp:
+    """Translate one 1- or 2-qubit gate into a tile op on local positions."""
+    op = N.TileOp()
+    m = np.zeros(32)
+    nat = G.nat_tuple(*quil_qubits)
+    k = len(nat)
+    if k == 1:
+        op.q0 = local[nat[0]]
+        if _is_diag(U):
+            op.kind = N.TILE_DIAG1
+            d = np.diag(U)
+            m[0:4] = [d[0].real, d[0].imag, d[1].real, d[1].imag]
+        else:
+            op.kind = N.TILE_DENSE1
+            m[0:8] = np.ascontiguousarray(U).reshape(-1).view(np.float64)
+    elif k == 2:
+        op.q0, op.q1 = local[nat[0]], local[nat[1]]
+        I4 = np.eye(4)
+        d = np.diag(U)
+        if _is_diag(U) and d[0] == 1 and d[1] == 1 and d[2] == 1:
+            op.kind = N.TILE_CPHASE
+            m[0:2] = [d[3].real, d[3].imag]
+        elif _is_diag(U):
+            op.kind = N.TILE_DIAG2
+            m[0:8] = np.ascontiguousarray(d).view(np.float64)
+        elif np.array_equal(U, G.gate_matrix("CNOT")):
+            op.kind = N.TILE_CPERM  # q0 = target (nat[0]), q1 = control (nat[1])
+        elif np.array_equal(U, G.gate_matrix("SWAP")):
+            op.kind = N.TILE_SWAP
+        elif (np.array_equal(U[:2, :2], I4[:2, :2]) and not U[:2, 2:].any() and not U[2:, :2].any()):
+            # CONTROLLED 1q: control = first Quil argument = nat[1]
+            op.kind = N.TILE_DENSE1
+            op.q0 = local[nat[0]]
+            op.q1 = 0
+            op.ctrl_mask = 1 << local[nat[1]]
+            m[0:8] = np.ascontiguousarray(U[2:, 2:]).reshape(-1).view(np.float64)
+        else:
+            op.kind = N.TILE_DENSE2
+            m[0:32] = np.ascontiguousarray(U).reshape(-1).view(np.float64)
+    else:
+        raise ValueError("tile ops are 1- or 2-qubit gates")
+    for i in range(32):
+        op.m[i] = m[i]
+    return op
+
+
+# ---------------------------------------------------------------------------
+# Dense block fusion
+
+
+def embed(U: np.ndarray, quil_qubits: Sequence[int], block: Sequence[int]) -> np.ndarray:
+    """Matrix of U on ``block`` (index bit j <-> block[j])."""
+    nat = G.nat_tuple(*quil_qubits)
+    kb = len(block)
+    pos = [block.index(q) for q in nat]
+    D = 1 << kb
+    idx = np.arange(D)
+    gin = np.zeros(D, dtype=np.int64)
+    for j, p in enumerate(pos):
+        gin |= ((idx >> p) & 1) << j
+    rest = idx.copy()
+    for p in pos:
+        rest &= ~(1 << p)
+    E = np.zeros((D, D), dtype=np.complex128)
+    kg = len(nat)
+    for r in range(1 << kg):
+        dep = 0
+        for j, p in enumerate(pos):
+            if (r >> j) & 1:
+                dep |= 1 << p
+        E[rest | dep, idx] = U[r, gin]
+    return E
+
+
+def fuse_groups(gates: List[Gate], nq: int, kmax: int):
+    """Dense blocks: list of (U_block, block_qubits) in engine order
+    (block_qubits[j] = index bit j)."""
+    qs = [tuple(q) for _, q in gates]
+    groups = group_gates(qs, nq, kmax, standalone=lambda i: len(qs[i]) > kmax)
+    blocks = []
+    for kind, T, members in groups:
+        block = list(T)
+        M = np.eye(1 << len(block), dtype=np.complex128)
+        for i in members:
+            U, q = gates[i]
+            M = embed(U, q, block) @ M
+        blocks.append((M, block, members))
+    return blocks
+
+
+# ---------------------------------------------------------------------------
+# Plans
+
+
+@dataclass
+class PlanItem:
+    kind: str                 # "gate" | "tile" | "fused"
+    qubits: List[int]         # engine order (gate/fused) or tile qubits
+    U: Optional[np.ndarray] = None
+    ops: list = field(default_factory=list)
+    gates: List[int] = field(default_factory=list)
+
+
+def plan_plain(gates: List[Gate], nq: int) -> List[PlanItem]:
+    return [PlanItem("gate", list(G.nat_tuple(*q)), U=U, gates=[i]) for i, (U, q) in enumerate(gates)]
+
+
+def _merge_runs_1q(gates: List[Gate]) -> List[Gate]:
+    """Multiply consecutive 1-qubit gates on the same qubit (in dependency
+    order) into one matrix."""
+    # pending[q] = index in `merged` of the open 1-qubit run on q; a gate on
+    # q with more qubits closes the run.
+    merged: List[Gate] = []
+    pending = {}
+    for U, q in gates:
+        q = list(q)
+        if len(q) == 1:
+            j = pending.get(q[0])
+            if j is not None:
+                merged[j] = (U @ merged[j][0], merged[j][1])
+                continue
+            merged.append((U, q))
+            pending[q[0]] = len(merged) - 1
+        else:
+            for x in q:
+                pending.pop(x, None)
+            merged.append((U, q))
+    return merged
+
+
+def plan_tiled(gates: List[Gate], nq: int, m: int, low: int = 4, merge_1q: bool = False) -> List[PlanItem]:
+    """LDS tile passes of up to m qubits that always include qubits 0..low-1."""
+    if merge_1q:
+        gates = _merge_runs_1q(gates)
+    if nq < 8:
+        return plan_plain(gates, nq)  # tile passes need at least 8 qubits
+    m = min(m, nq)
+    low = min(low, m)
+    qs = [tuple(q) for _, q in gates]
+    groups = group_gates(qs, nq, m, seed_qubits=range(low), standalone=lambda i: len(qs[i]) > 2)
+    plan = []
+    for kind, T, members in groups:
+        if kind == "single":
+            i = members[0]
+            plan.append(PlanItem("gate", list(G.nat_tuple(*gates[i][1])), U=gates[i][0], gates=[i]))
+            continue
+        # pad the tile to m qubits with the lowest unused qubits (contiguity)
+        T = set(T)
+        q = 0
+        while len(T) < m:
+            if q not in T:
+                T.add(q)
+            q += 1
+        tq = sorted(T)
+        local = {x: j for j, x in enumerate(tq)}
+        ops = [tile_op(gates[i][0], gates[i][1], local) for i in members]
+        plan.append(PlanItem("tile", tq, ops=ops, gates=members))
+    return plan
+
+
+def plan_fused(gates: List[Gate], nq: int, kmax: int) -> List[PlanItem]:
+    plan = []
+    for M, block, members in fuse_groups(gates, nq, kmax):
+        if len(block) >= 3:
+            plan.append(PlanItem("fused", block, U=M, gates=members))
+        else:
+            # engine order: block[j] = index bit j
+            plan.append(PlanItem("gate", block, U=M, gates=members))
+    return plan
+
+
+def execute_plan(state, plan: List[PlanItem]) -> None:
+    """Run a plan eagerly (one C-ABI call per item)."""
+    for it in plan:
+        if it.kind == "gate":
+            state.apply_matrix(it.U, it.qubits)
+        elif it.kind == "fused":
+            state.apply_fused(it.U, it.qubits)
+        else:
+            state.apply_tile_pass(it.qubits, it.ops)
+
+
+def build_program(state, plan: List[PlanItem], use_graph: bool = True):
+    from .state import Program
+    p = Program(state)
+    for it in plan:
+        if it.kind == "gate":
+            p.add_matrix(it.U, it.qubits)
+        elif it.kind == "fused":
+            p.add_fused(it.U, it.qubits)
+        else:
+            p.add_tile_pass(it.qubits, it.ops)
+    p.finalize(use_graph=use_graph)
+    return p
+
+
+def default_tile_m(dtype: str) -> int:
+    return 11 if dtype == "f64" else 12
+
+
+def default_low(dtype: str) -> int:
+    return 4 if dtype == "f64" else 5
+
+
+def compile_gates(state, gates: List[Gate], mode: str = "tiled", use_graph: bool = True, m: int = 0,
+                  kmax: int = 4, merge_1q: bool = False):
+    from .qvm import CompiledProgram
+    nq = state.num_qubits
+    if mode == "plain":
+        plan = plan_plain(gates, nq)
+    elif mode == "tiled":
+        plan = plan_tiled(gates, nq, m or default_tile_m(state.dtype), default_low(state.dtype), merge_1q)
+    elif mode == "fused":
+        plan = plan_fused(gates, nq, kmax)
+    else:
+        raise ValueError(f"unknown plan mode {mode!r}")
+    return CompiledProgram(build_program(state, plan, use_graph), plan, mode)
+
+
+def sweeps(plan: List[PlanItem]) -> int:
+    return len(plan)

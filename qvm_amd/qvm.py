@@ -1,0 +1,219 @@
+"""qvm's pure-state machine API over the HBM-resident engine.
+
+Mirrors the reference's Lisp entry points so a user of qvm finds the same
+operations (names in snake_case):
+
+    make_qvm            src/qvm.lisp:150-164
+    load_program        src/classical-memory-mixin.lisp:114-159
+    run                 src/execution.lisp:31-37
+    run_program         src/execution.lisp:46-59
+    transition          src/transition.lisp:46-194 (gate, MEASURE, RESET, HALT)
+    apply_gate_to_state src/apply-gate.lisp:106-160
+    measure             src/measurement.lisp:87-105
+    measure_all         src/measurement.lisp:122-143
+    amplitudes          src/qvm.lisp:63-82
+    compile_loaded_program  src/qvm.lisp:166-175 (plain / tiled / fused plans)
+
+Random draws come from ``qvm.rng`` (a callable returning uniforms in [0,1));
+the reference draws from mt19937 (src/utilities.lisp:306-335), which is not
+vendored, so callers that need a reproducible stream pass their own.
+"""
+from __future__ import annotations
+
+from typing import Callable, Optional
+
+import numpy as np
+
+from . import gates as G
+from . import quil as Q
+from .state import HipPureState, Program
+
+
+class QVMError(RuntimeError):
+    pass
+
+
+class _UniformStream:
+    def __init__(self, seed=None):
+        self._g = np.random.default_rng(seed)
+
+    def __call__(self) -> float:
+        return float(self._g.random())
+
+
+class PureStateQVM:
+    """pure-state-qvm (src/qvm.lisp:106-148) with an HBM-resident state."""
+
+    def __init__(self, num_qubits: int, dtype="f64", rng: Optional[Callable[[], float]] = None,
+                 device: int = -1, memory_cap_bytes: int = 0, compiled_measurement: bool = False):
+        self.number_of_qubits = int(num_qubits)
+        self.state = HipPureState(num_qubits, dtype=dtype, device=device, memory_cap_bytes=memory_cap_bytes)
+        self.rng = rng if rng is not None else _UniformStream()
+        self.program: Optional[Q.ParsedProgram] = None
+        self.memory: dict = {}
+        self.pc = 0
+        self.compiled_measurement = compiled_measurement
+        self.draws_used = 0
+        self._compiled: Optional["CompiledProgram"] = None
+
+    # -- program ---------------------------------------------------------
+    def load_program(self, program) -> "PureStateQVM":
+        """load-program: qubit-count check, classical memory, pc reset."""
+        if isinstance(program, str):
+            program = Q.parse_quil(program)
+        need = program.qubits_needed()
+        if need > self.number_of_qubits:
+            raise QVMError(f"Trying to load a program with {need} qubits on a QVM of "
+                           f"{self.number_of_qubits} qubits")
+        self.program = program
+        self.memory = {}
+        for d in program.declarations.values():
+            dt = np.float64 if d.type == "REAL" else np.int64
+            self.memory[d.name] = np.zeros(d.length, dtype=dt)
+        self.pc = 0
+        self._compiled = None
+        return self
+
+    def memory_ref(self, name: str, index: int = 0):
+        return self.memory[name][index]
+
+    def reset_classical_memory(self) -> None:
+        for v in self.memory.values():
+            v[:] = 0
+
+    def reset_quantum_state(self) -> None:
+        """reset-quantum-state (src/qvm.lisp:84-87)."""
+        self.state.reset_zero()
+
+    # -- execution -------------------------------------------------------
+    def run(self) -> "PureStateQVM":
+        if self.program is None:
+            raise QVMError("no program loaded")
+        if self._compiled is not None:
+            self._compiled.run()
+            self.pc = len(self.program.executable_code)
+            return self
+        code = self.program.executable_code
+        self.pc = 0
+        while self.pc is not None and self.pc < len(code):
+            self.pc = self.transition(code[self.pc])
+        return self
+
+    def _draw(self) -> float:
+        self.draws_used += 1
+        return float(self.rng())
+
+    def transition(self, ins):
+        if isinstance(ins, Q.GateApplication):
+            U = Q.resolve_gate_matrix(ins, self.program.gate_definitions if self.program else {}, self.memory)
+            k = G.num_qubits_of(U)
+            if k != len(ins.qubits):
+                raise QVMError(f"Gate {ins.name} takes {k} qubit(s), applied to {len(ins.qubits)}")
+            apply_gate_to_state(self.state, U, ins.qubits)
+            return self.pc + 1
+        if isinstance(ins, Q.Measure):
+            _, bit = self.measure(ins.qubit)
+            if ins.target is not None:
+                name, idx = ins.target
+                self.memory[name][idx] = bit
+            return self.pc + 1
+        if isinstance(ins, Q.Reset):
+            if ins.qubit is None:
+                self.reset_quantum_state()
+            else:
+                # src/transition.lisp:76-95: measure, then X if the result was 1
+                _, bit = self.measure(ins.qubit)
+                if bit == 1:
+                    apply_gate_to_state(self.state, G.gate_matrix("X"), [ins.qubit])
+            return self.pc + 1
+        if isinstance(ins, Q.Halt):
+            return None
+        if isinstance(ins, Q.Nop):
+            return self.pc + 1
+        raise QVMError(f"unsupported instruction {ins!r}")
+
+    # -- measurement -----------------------------------------------------
+    def measure(self, q: int):
+        """measure: interpreted rule (no draw when p1 == 0) unless
+        ``compiled_measurement`` selects the compiled projector rule."""
+        if not (0 <= q < self.number_of_qubits):
+            raise QVMError(f"Trying to measure qubit {q} on a QVM with only {self.number_of_qubits} qubits.")
+        if self.compiled_measurement:
+            u = self._draw()
+            bit, _ = self.state.measure(q, u, compiled=True)
+            return self, bit
+        p1 = self.state.prob_excited(q)
+        if p1 == 0.0:
+            bit = 0
+        else:
+            bit = 1 if self._draw() <= p1 else 0
+        self.state.force_measure(q, bit, p1)
+        return self, bit
+
+    def measure_all(self):
+        """measure-all: one draw, collapse to the sampled basis state, bits
+        LSB first."""
+        b = self.state.measure_all(self._draw())
+        return self, [(b >> i) & 1 for i in range(self.number_of_qubits)]
+
+    # -- amplitudes ------------------------------------------------------
+    def amplitudes(self) -> np.ndarray:
+        return self.state.amplitudes()
+
+    def set_amplitudes(self, amps) -> None:
+        self.state.write_amplitudes(amps)
+
+    def nth_amplitude(self, i: int) -> complex:
+        return complex(self.state.read_amplitudes(i, 1)[0])
+
+    # -- compilation -----------------------------------------------------
+    def compile_loaded_program(self, mode: str = "tiled", use_graph: bool = True, **kw) -> "CompiledProgram":
+        """Plan the loaded gate-only program into engine launches and capture
+        them (plain: one launch per gate; tiled: LDS tile passes; fused:
+        dense MFMA blocks)."""
+        from . import planner
+        if self.program is None:
+            raise QVMError("no program loaded")
+        gates = []
+        for ins in self.program.executable_code:
+            if isinstance(ins, Q.GateApplication):
+                U = Q.resolve_gate_matrix(ins, self.program.gate_definitions, self.memory)
+                gates.append((U, list(ins.qubits)))
+            elif isinstance(ins, (Q.Nop, Q.Halt)):
+                continue
+            else:
+                raise QVMError("compile_loaded_program handles gate-only programs")
+        self._compiled = planner.compile_gates(self.state, gates, mode=mode, use_graph=use_graph, **kw)
+        return self._compiled
+
+    def close(self) -> None:
+        self.state.close()
+
+
+def make_qvm(num_qubits: int, **kw) -> PureStateQVM:
+    return PureStateQVM(num_qubits, **kw)
+
+
+def run_program(num_qubits: int, program, **kw) -> PureStateQVM:
+    q = make_qvm(num_qubits, **kw)
+    q.load_program(program)
+    return q.run()
+
+
+def apply_gate_to_state(state: HipPureState, U: np.ndarray, quil_qubits) -> None:
+    """Pure-state apply-gate-to-state: matrix in Quil convention, qubits in
+    Quil argument order; the engine receives the nat-tuple
+    (src/apply-gate.lisp:109-160)."""
+    state.apply_matrix(U, G.nat_tuple(*quil_qubits))
+
+
+class CompiledProgram:
+    """Handle returned by compile_loaded_program."""
+
+    def __init__(self, program: Program, plan: list, mode: str):
+        self.program = program
+        self.plan = plan
+        self.mode = mode
+
+    def run(self) -> None:
+        self.program.run()

@@ -1,0 +1,440 @@
+"""GPU parity tests: the HIP engine through its C ABI against the CPU oracle.
+
+FP64 gate application is compared bit-for-bit (the engine's plain kernels use
+the reference serial kernels' multiply-then-add arithmetic); reductions to
+1e-14 relative; sampling exactly, except draws that sit within rounding of a
+CDF boundary, which are counted.  FP32 and fused paths are checked by
+infidelity / max amplitude error against the FP64 oracle.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from qvm_amd import _native as N
+from qvm_amd import circuits as CI
+from qvm_amd import gates as G
+from qvm_amd import planner as P
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def S():
+    from qvm_amd.state import HipPureState
+    assert N.device_count() >= 1, "no HIP device visible"
+    return HipPureState
+
+
+def rand_unitary(k, rng):
+    Z = rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k))
+    Qm, R = np.linalg.qr(Z)
+    return Qm * (np.diag(R) / np.abs(np.diag(R)))
+
+
+def rand_state(n, rng):
+    v = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    return np.ascontiguousarray(v / np.linalg.norm(v))
+
+
+def load(S, psi, dtype="f64"):
+    n = int(psi.size).bit_length() - 1
+    st = S(n, dtype)
+    st.write_amplitudes(psi)
+    return st
+
+
+def near_boundary(cdf, p, b, tol):
+    lo = cdf[b - 1] if b > 0 else 0.0
+    return abs(p - cdf[b]) < tol or abs(p - lo) < tol
+
+
+# --- gates: bit-exact FP64 ---------------------------------------------------
+
+@pytest.mark.parametrize("n", [1, 5, 12, 17])
+def test_1q_every_target_bit_exact(S, oracle, n):
+    rng = np.random.default_rng(n)
+    psi = rand_state(n, rng)
+    st = load(S, psi)
+    ref = psi.copy()
+    for q in range(n):
+        U = rand_unitary(1, rng)
+        say(f"1q n={n} q={q}")
+        st.apply_matrix(U, [q])
+        oracle.apply_matrix(ref, U, [q])
+    np.testing.assert_array_equal(st.amplitudes(), ref)
+    st.close()
+
+
+@pytest.mark.parametrize("k", [2, 3, 4, 5, 6])
+def test_dense_kq_bit_exact(S, oracle, k):
+    rng = np.random.default_rng(100 + k)
+    n = 14
+    psi = rand_state(n, rng)
+    st = load(S, psi)
+    ref = psi.copy()
+    for trial in range(4):
+        qs = [int(x) for x in rng.choice(n, size=k, replace=False)]  # unsorted, non-adjacent
+        U = rand_unitary(k, rng)
+        say(f"dense k={k} qubits={qs}")
+        st.apply_matrix(U, qs)
+        oracle.apply_matrix(ref, U, qs)
+    np.testing.assert_array_equal(st.amplitudes(), ref)
+    st.apply_matrix_dense(rand_unitary(k, rng), list(range(k)))
+    st.close()
+
+
+def test_structured_gates_bit_exact(S, oracle):
+    rng = np.random.default_rng(7)
+    n = 13
+    psi = rand_state(n, rng)
+    st = load(S, psi)
+    ref = psi.copy()
+    gates = [("CNOT", [3, 11]), ("CNOT", [11, 0]), ("CZ", [2, 9]), ("SWAP", [12, 1]),
+             ("CCNOT", [0, 7, 4]), ("CSWAP", [5, 12, 2]), ("PHASE", [6]), ("RZ", [4]),
+             ("RZZ", [10, 3]), ("CPHASE", [1, 8]), ("X", [0]), ("Z", [12]), ("S", [3]), ("ISWAP", [4, 9]),
+             ("PSWAP", [7, 2])]
+    for name, qs in gates:
+        npar = 1 if name in ("PHASE", "RZ", "RZZ", "CPHASE", "PSWAP") else 0
+        U = G.gate_matrix(name, [float(rng.uniform(0, 2 * np.pi))] if npar else [])
+        say(f"structured {name} {qs}")
+        st.apply_matrix(U, G.nat_tuple(*qs))
+        oracle.apply_matrix(ref, U, G.nat_tuple(*qs))
+    U = G.controlled(rand_unitary(1, rng))  # CONTROLLED U
+    st.apply_matrix(U, G.nat_tuple(9, 2))
+    oracle.apply_matrix(ref, U, G.nat_tuple(9, 2))
+    np.testing.assert_array_equal(st.amplitudes(), ref)
+    # diagonal / permutation entry points
+    d = np.exp(1j * rng.uniform(0, 6, 8))
+    st.apply_diagonal(d, [1, 6, 11])
+    oracle.apply_matrix(ref, np.diag(d), [1, 6, 11])
+    perm = [int(x) for x in rng.permutation(8)]
+    st.apply_permutation(perm, [12, 0, 5])
+    oracle.apply_permutation(ref, perm, [12, 0, 5])
+    np.testing.assert_array_equal(st.amplitudes(), ref)
+    st.close()
+
+
+def test_layered_n20_bit_exact_and_measure_all_shots(S, oracle):
+    """Config 1 (CPU-runnable): n=20, 20 layers, then 10^4 measure-all shots."""
+    n = 20
+    gates = CI.layered(n, 20, seed=1)
+    st = S(n, "f64")
+    ref = oracle.zero_state(n)
+    say(f"layered n={n} gates={len(gates)}")
+    for U, q in gates:
+        st.apply_matrix(U, G.nat_tuple(*q))
+        oracle.apply_matrix(ref, U, G.nat_tuple(*q))
+    got = st.amplitudes()
+    np.testing.assert_array_equal(got, ref)
+    u = CI.uniforms(20201, 10000)
+    shots = st.sample_measure_all(u)
+    cdf = oracle.cdf(ref)
+    want = np.array([oracle.sample_bisect(ref, p) for p in u[:2000]], dtype=np.uint64)
+    mism = np.nonzero(shots[:2000] != want)[0]
+    for i in mism:
+        assert near_boundary(cdf, u[i], int(want[i]), 1e-12), (i, u[i], shots[i], want[i])
+    say(f"measure-all shots: {len(mism)} near-boundary mismatches of 2000 checked")
+    assert len(mism) <= 2
+    # a single measure-all collapses to |b> with amplitude exactly 1
+    b = st.measure_all(float(u[0]))
+    assert b == shots[0]
+    amps = st.amplitudes()
+    assert amps[b] == 1 and np.count_nonzero(amps) == 1
+    st.close()
+
+
+def test_qft_n14(S, oracle):
+    n = 14
+    st = S(n, "f64")
+    st.apply_matrix(G.gate_matrix("X"), [3])
+    ref = oracle.zero_state(n)
+    oracle.apply_matrix(ref, G.gate_matrix("X"), [3])
+    for U, q in CI.qft(n):
+        st.apply_matrix(U, G.nat_tuple(*q))
+        oracle.apply_matrix(ref, U, G.nat_tuple(*q))
+    got = st.amplitudes()
+    np.testing.assert_array_equal(got, ref)
+    N_ = 1 << n
+    want = np.exp(2j * np.pi * 8 * np.arange(N_) / N_) / math.sqrt(N_)
+    assert np.max(np.abs(got - want)) < 1e-12
+    st.close()
+
+
+# --- reductions and measurement -------------------------------------------
+
+def test_reductions(S, oracle):
+    rng = np.random.default_rng(11)
+    for n in (1, 6, 15, 21):
+        psi = rand_state(n, rng)
+        st = load(S, psi)
+        for q in sorted({0, n // 2, n - 1}):
+            say(f"reduce n={n} q={q}")
+            assert abs(st.prob_excited(q) - oracle.excited_probability(psi, q)) < 1e-14
+            assert abs(st.prob_ground(q) - oracle.ground_probability(psi, q)) < 1e-14
+        assert abs(st.norm() - oracle.norm(psi)) < 1e-14
+        st.write_amplitudes(psi * 2.5)
+        st.normalize()
+        assert abs(st.norm() - 1) < 1e-14
+        pr = st.probabilities()
+        assert np.allclose(pr, np.abs(psi) ** 2, rtol=1e-13, atol=1e-300)
+        st.close()
+
+
+@pytest.mark.parametrize("compiled", [False, True])
+def test_measure_matches_oracle(S, oracle, compiled):
+    rng = np.random.default_rng(12 + compiled)
+    n = 11
+    psi = rand_state(n, rng)
+    st = load(S, psi)
+    ref = psi.copy()
+    for q in (0, 4, 10, 4):
+        u = float(rng.random())
+        say(f"measure q={q} u={u} compiled={compiled}")
+        b, used = st.measure(q, u, compiled)
+        bo, usedo = oracle.measure(ref, q, u, compiled)
+        assert (b, used) == (bo, usedo)
+    assert np.max(np.abs(st.amplitudes() - ref)) < 1e-13
+    # p1 == 0: no draw (interpreted)
+    st0 = S(3, "f64")
+    assert st0.measure(2, 0.3, False) == (0, 0)
+    assert st0.measure(2, 0.3, True) == (0, 1)
+    st0.close()
+    st.close()
+
+
+def test_force_measure_and_errors(S, oracle):
+    rng = np.random.default_rng(13)
+    psi = rand_state(9, rng)
+    st = load(S, psi)
+    ref = psi.copy()
+    p1 = oracle.excited_probability(psi, 5)
+    st.force_measure(5, 1, p1)
+    oracle.force_measurement(ref, 5, 1, p1)
+    np.testing.assert_array_equal(st.amplitudes(), ref)
+    with pytest.raises(N.QVMHipError) as e:
+        st.measure(9, 0.5)
+    assert e.value.code == N.EINVAL
+    with pytest.raises(N.QVMHipError):
+        st.apply_matrix(np.eye(2), [12])
+    with pytest.raises(N.QVMHipError):
+        st.apply_matrix(np.eye(4), [1, 1])
+    with pytest.raises(N.QVMHipError) as e:
+        st.measure_all(1.5)
+    assert e.value.code == N.ERANGE
+    st.close()
+
+
+def test_sample_cdf_rule(S, oracle):
+    rng = np.random.default_rng(14)
+    for n in (1, 3, 8, 13, 18):
+        psi = rand_state(n, rng)
+        psi[rng.integers(0, psi.size, psi.size // 4)] = 0  # zero-probability runs
+        psi /= np.linalg.norm(psi)
+        st = load(S, psi)
+        cdf = oracle.cdf(psi)
+        u = np.concatenate([rng.random(3000), cdf[: min(64, cdf.size)], [0.0, 1.0]])
+        say(f"sample n={n} draws={u.size}")
+        got = st.sample(u)
+        want = oracle.sample_multiple(psi, u)
+        bad = np.nonzero(got != want)[0]
+        for i in bad:
+            assert near_boundary(cdf, u[i], int(want[i]), 1e-13), (n, u[i], got[i], want[i])
+        assert len(bad) <= max(2, 64 // 8)
+        got2 = st.sample_measure_all(u)
+        want2 = np.array([oracle.sample_bisect(psi, p) for p in u[:500]], dtype=np.uint64)
+        bad2 = np.nonzero(got2[:500] != want2)[0]
+        for i in bad2:
+            assert near_boundary(cdf, u[i], int(want2[i]), 1e-13)
+        st.close()
+
+
+def test_sample_goldens(S, oracle):
+    import json
+    import os
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.json")))
+    for case in gold["sample_bisect"]["cases"]:
+        psi = np.array(case["amplitudes"], dtype=np.complex128)
+        st = load(S, psi)
+        u = np.array([p for p, _ in case["draws"]])
+        want = [w for _, w in case["draws"]]
+        assert list(st.sample_measure_all(u)) == want
+        st.close()
+
+
+# --- amplitudes, copies, expectations ---------------------------------------
+
+def test_amplitude_io_copy_inner_expectation(S):
+    rng = np.random.default_rng(15)
+    n = 12
+    psi = rand_state(n, rng)
+    a = load(S, psi)
+    np.testing.assert_array_equal(a.amplitudes(), psi)
+    np.testing.assert_array_equal(a.read_amplitudes(100, 7), psi[100:107])
+    b = S(n, "f64")
+    b.copy_from(a)
+    np.testing.assert_array_equal(b.amplitudes(), psi)
+    phi = rand_state(n, rng)
+    b.write_amplitudes(phi)
+    ip = a.inner_product(b)
+    assert abs(ip - np.vdot(psi, phi)) < 1e-13
+    edges = [(0, 3), (3, 7), (7, 11), (2, 5)]
+    idx = np.arange(psi.size)
+    zz = sum(np.sum(np.abs(psi) ** 2 * (1 - 2 * (((idx >> x) ^ (idx >> y)) & 1))) for x, y in edges)
+    assert abs(a.expectation_zz(edges) - zz) < 1e-13
+    a.reset_zero()
+    z = a.amplitudes()
+    assert z[0] == 1 and np.count_nonzero(z) == 1
+    a.close()
+    b.close()
+
+
+def test_determinism(S):
+    n = 16
+    gates = CI.haar_cz(n, 3, seed=9)
+    outs = []
+    for _ in range(2):
+        st = S(n, "f64")
+        for U, q in gates:
+            st.apply_matrix(U, G.nat_tuple(*q))
+        outs.append((st.amplitudes(), st.prob_excited(3), st.norm()))
+        st.close()
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    assert outs[0][1] == outs[1][1] and outs[0][2] == outs[1][2]
+
+
+# --- tile passes, fused blocks, FP32, programs ------------------------------
+
+@pytest.mark.parametrize("m", [8, 10, 12])
+def test_tile_passes_bit_exact_vs_plain(S, oracle, m):
+    n = 16
+    gates = CI.layered(n, 6, seed=5) + CI.qft(n)[:40] + [(G.gate_matrix("RZZ", [0.7]), [3, 14]),
+                                                         (G.controlled(rand_unitary(1, np.random.default_rng(1))), [15, 2])]
+    plan = P.plan_tiled(gates, n, m=m, low=4)
+    say(f"tile m={m} passes={sum(1 for it in plan if it.kind == 'tile')} items={len(plan)}")
+    st = S(n, "f64")
+    P.execute_plan(st, plan)
+    ref = oracle.zero_state(n)
+    for U, q in gates:
+        oracle.apply_matrix(ref, U, G.nat_tuple(*q))
+    np.testing.assert_array_equal(st.amplitudes(), ref)
+    st.close()
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("k", [3, 4, 5, 6])
+def test_fused_mfma_block(S, oracle, k, dtype):
+    rng = np.random.default_rng(200 + k)
+    n = 13
+    psi = rand_state(n, rng)
+    for qs in (list(range(k)), [int(x) for x in rng.choice(n, size=k, replace=False)]):
+        U = rand_unitary(k, rng)
+        say(f"fused k={k} dtype={dtype} qubits={qs}")
+        st = load(S, psi, dtype)
+        st.apply_fused(U, qs)
+        ref = oracle.apply_matrix(psi.copy(), U, qs)
+        err = np.max(np.abs(st.amplitudes() - ref))
+        assert err < (1e-14 if dtype == "f64" else 3e-6), err
+        st.close()
+
+
+def test_fused_plan_random_mix(S, oracle):
+    n = 16
+    gates = CI.random_mix(n, 8, seed=21)
+    ref = oracle.zero_state(n)
+    for U, q in gates:
+        oracle.apply_matrix(ref, U, G.nat_tuple(*q))
+    for kmax in (3, 4, 5):
+        plan = P.plan_fused(gates, n, kmax)
+        st = S(n, "f64")
+        P.execute_plan(st, plan)
+        err = np.max(np.abs(st.amplitudes() - ref))
+        say(f"fused plan kmax={kmax} blocks={len(plan)} err={err:.2e}")
+        assert err < 1e-12
+        st.close()
+
+
+def test_f32_layered_infidelity(S, oracle):
+    n = 18
+    gates = CI.layered(n, 10, seed=3)
+    ref = oracle.zero_state(n)
+    st = S(n, "f32")
+    for U, q in gates:
+        oracle.apply_matrix(ref, U, G.nat_tuple(*q))
+        st.apply_matrix(U, G.nat_tuple(*q))
+    got = st.amplitudes()
+    infid = 1 - abs(np.vdot(ref, got)) ** 2
+    say(f"f32 infidelity {infid:.3e} max err {np.max(np.abs(got - ref)):.3e}")
+    assert infid < 1e-6
+    assert abs(st.norm() - 1) < 1e-5
+    st.close()
+
+
+@pytest.mark.parametrize("mode", ["plain", "tiled", "fused"])
+def test_program_graph_replay(S, oracle, mode):
+    n = 14
+    gates = CI.layered(n, 3, seed=8)
+    st = S(n, "f64")
+    eager = S(n, "f64")
+    plan = {"plain": P.plan_plain, "tiled": lambda g, n: P.plan_tiled(g, n, 10, 4),
+            "fused": lambda g, n: P.plan_fused(g, n, 4)}[mode](gates, n)
+    prog = P.build_program(st, plan, use_graph=True)
+    for rep in range(3):
+        say(f"graph {mode} replay {rep}")
+        prog.run()
+        P.execute_plan(eager, plan)
+        eager.sync()
+        _ = eager.norm()  # eager work between replays
+        a, b = st.amplitudes(), eager.amplitudes()
+        if mode == "fused":
+            assert np.max(np.abs(a - b)) < 1e-13
+        else:
+            np.testing.assert_array_equal(a, b)
+    prog.close()
+    st.close()
+    eager.close()
+
+
+def test_qvm_api_end_to_end(S):
+    import json
+    import os
+
+    from qvm_amd import qvm as QV
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.json")))
+    for key in ("simple_measurements", "measurements_with_swap"):
+        g = gold[key]
+        q = QV.run_program(3, "\n".join(g["program"]))
+        assert [int(q.memory_ref("ro", i)) for i in range(3)] == g["ro"]
+        q.close()
+    for i, lines in enumerate(gold["measure_all_basis"]["programs"]):
+        q = QV.make_qvm(3)
+        q.load_program("\n".join(lines) or "I 0")
+        q.run()
+        _, bits = q.measure_all()
+        assert bits == [(i >> b) & 1 for b in range(3)]
+        assert q.nth_amplitude(i) == 1
+        q.close()
+    q = QV.run_program(3, "\n".join(gold["dagger_inversion"]["program"]))
+    a = q.amplitudes()
+    assert abs(a[0] - 1) < 1e-14 and np.max(np.abs(a[1:])) < 1e-14
+    q.close()
+    # unit norm after measurements (tests/measurement-tests.lisp:88-109)
+    for n in (5, 9):
+        prog = "\n".join(f"H {k}\nMEASURE {k}" for k in range(n))
+        q = QV.run_program(n, prog)
+        assert abs(np.linalg.norm(q.amplitudes()) - 1) < 1e-14
+        q.close()
+    # compiled program path
+    q = QV.make_qvm(10)
+    q.load_program("\n".join(f"H {k}" for k in range(10)) + "\nCNOT 0 9\nRZ(0.3) 4")
+    q.compile_loaded_program(mode="tiled")
+    q.run()
+    pr = np.abs(q.amplitudes()) ** 2
+    assert np.allclose(pr, 2.0 ** -10)
+    q.close()

@@ -1,0 +1,232 @@
+"""CPU tests of the host side: Quil front end, planners, and the C-ABI library
+surface (loads, exports every declared symbol, rejects bad arguments before
+touching a device).  No kernel is launched here."""
+import ctypes as C
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from qvm_amd import _native as N
+from qvm_amd import circuits as CI
+from qvm_amd import gates as G
+from qvm_amd import planner as P
+from qvm_amd import quil as Q
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def full_unitary(U, quil_qubits, n):
+    from tests.test_oracle import kron_unitary
+    return kron_unitary(U, quil_qubits, n)
+
+
+def simulate(gates, n, psi):
+    for U, q in gates:
+        psi = full_unitary(U, q, n) @ psi
+    return psi
+
+
+def emulate_tile_op(op, tq, n, psi):
+    """Numpy model of the tile kernel's op semantics on the full state."""
+    g = lambda local: tq[local]  # noqa: E731
+    m = np.array([op.m[i] for i in range(32)]).view(np.complex128)
+    ctrl = [g(b) for b in range(32) if (op.ctrl_mask >> b) & 1]
+    if op.kind == N.TILE_DENSE1:
+        U, qs = m[:4].reshape(2, 2), [g(op.q0)]
+    elif op.kind == N.TILE_DIAG1:
+        U, qs = np.diag(m[:2]), [g(op.q0)]
+    elif op.kind == N.TILE_DENSE2:
+        U, qs = m[:16].reshape(4, 4), [g(op.q1), g(op.q0)]
+    elif op.kind == N.TILE_DIAG2:
+        U, qs = np.diag(m[:4]), [g(op.q1), g(op.q0)]
+    elif op.kind == N.TILE_CPERM:
+        U, qs = G.gate_matrix("CNOT"), [g(op.q1), g(op.q0)]
+    elif op.kind == N.TILE_SWAP:
+        U, qs = G.gate_matrix("SWAP"), [g(op.q1), g(op.q0)]
+    elif op.kind == N.TILE_CPHASE:
+        U, qs = np.diag([1, 1, 1, m[0]]), [g(op.q1), g(op.q0)]
+    else:
+        raise AssertionError(op.kind)
+    for c in ctrl:
+        U, qs = G.controlled(U), [c] + qs
+    return full_unitary(U, qs, n) @ psi
+
+
+def emulate_plan(plan, n, psi):
+    for it in plan:
+        if it.kind in ("gate", "fused"):
+            # engine order -> Quil order
+            psi = full_unitary(it.U, list(reversed(it.qubits)), n) @ psi
+        else:
+            for op in it.ops:
+                psi = emulate_tile_op(op, it.qubits, n, psi)
+    return psi
+
+
+def rand_state(n, seed):
+    rng = np.random.default_rng(seed)
+    v = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    return v / np.linalg.norm(v)
+
+
+# --- Quil front end ----------------------------------------------------------
+
+def test_parse_program_and_expressions():
+    p = Q.parse_quil("""
+DECLARE ro BIT[2]
+DECLARE theta REAL[2]
+# comment
+H 0; CNOT 0 1
+RX(pi/2) 1
+RZ(-2*theta[1] + 0.5) 0
+CPHASE(pi/2^3) 0 1
+DAGGER CONTROLLED RY(2*pi/3) 0 1
+MEASURE 0 ro[1]
+RESET 1
+RESET
+HALT
+""")
+    kinds = [type(i).__name__ for i in p.executable_code]
+    assert kinds == ["GateApplication"] * 6 + ["Measure", "Reset", "Reset", "Halt"]
+    assert p.declarations["theta"].length == 2
+    mem = {"theta": np.array([0.0, 0.25])}
+    rz = p.executable_code[3]
+    assert abs(rz.params[0]({}, mem) - 0.0) < 1e-15
+    cp = p.executable_code[4]
+    assert abs(cp.params[0]({}, mem) - math.pi / 8) < 1e-15
+    U = Q.resolve_gate_matrix(p.executable_code[5], {}, mem)
+    want = G.dagger(G.controlled(G.gate_matrix("RY", [2 * math.pi / 3])))
+    assert np.allclose(U, want)
+    assert p.qubits_needed() == 2
+
+
+def test_defgate_matrix_and_permutation():
+    p = Q.parse_quil("""DEFGATE G(%a):
+    cos(%a), sin(%a)
+    -sin(%a), cos(%a)
+DEFGATE CYC AS PERMUTATION:
+    1, 2, 3, 0
+G(0.0) 0
+CYC 0 1
+""")
+    U = Q.resolve_gate_matrix(p.executable_code[0], p.gate_definitions, {})
+    assert np.allclose(U, np.eye(2))
+    V = Q.resolve_gate_matrix(p.executable_code[1], p.gate_definitions, {})
+    assert np.array_equal(V @ np.eye(4)[:, 0], np.eye(4)[:, 1])  # |0> -> |1>
+
+
+def test_parse_errors():
+    with pytest.raises(Q.QuilParseError):
+        Q.parse_quil("DECLARE x FOO[2]")
+    with pytest.raises(Q.QuilParseError):
+        Q.resolve_gate_matrix(Q.parse_quil("NOTAGATE 0").executable_code[0], {}, {})
+
+
+def test_changed_amplitudes_definition():
+    n = 10
+    assert G.changed_amplitudes(G.gate_matrix("H"), n) == 1 << n
+    assert G.changed_amplitudes(G.gate_matrix("RZ", [0.3]), n) == 1 << n
+    assert G.changed_amplitudes(G.gate_matrix("CNOT"), n) == 1 << (n - 1)
+    assert G.changed_amplitudes(G.gate_matrix("SWAP"), n) == 1 << (n - 1)
+    assert G.changed_amplitudes(G.gate_matrix("CZ"), n) == 1 << (n - 2)
+    assert G.changed_amplitudes(G.gate_matrix("CPHASE", [0.1]), n) == 1 << (n - 2)
+    g = CI.layered(20, 20, 1)
+    assert CI.amp_updates(g, 20) == 938475520  # 800 1q + 190 CNOT at n=20
+
+
+def test_workload_shapes():
+    assert len(CI.layered(30, 40, 1)) == 2400 + 580
+    q = CI.qft(30)
+    names = [G.num_qubits_of(U) for U, _ in q]
+    assert names.count(1) == 30 and names.count(2) == 435 + 15
+    h = CI.haar_cz(30, 20, 3)
+    assert len(h) == 600 + 300
+    for U, _ in h[:30]:
+        assert G.is_unitary(U, 1e-12)
+    inst = CI.qaoa_instances(8, seed=4)
+    for n, edges, gam, bet in inst:
+        deg = np.zeros(n, int)
+        for a, b in edges:
+            deg[a] += 1
+            deg[b] += 1
+        assert (deg == 3).all() and 1 <= len(gam) <= 4 and len(gam) == len(bet)
+    u = CI.uniforms(7, 1000)
+    assert u.min() >= 0 and u.max() < 1 and abs(u.mean() - 0.5) < 0.05
+
+
+# --- planners ----------------------------------------------------------------
+
+@pytest.mark.parametrize("merge", [False, True])
+def test_tiled_plan_is_the_same_unitary(merge):
+    n = 9
+    gates = CI.layered(n, 5, seed=2) + CI.qft(n)[:20] + [(G.gate_matrix("CCNOT"), [0, 4, 8])]
+    gates += [(G.gate_matrix("RZZ", [0.4]), [2, 7]), (G.gate_matrix("SWAP"), [8, 1]),
+              (G.controlled(G.gate_matrix("RY", [0.3])), [6, 2])]
+    plan = P.plan_tiled(gates, n, m=8, low=3, merge_1q=merge)
+    assert any(it.kind == "tile" for it in plan)
+    psi0 = rand_state(n, 1)
+    want = simulate(gates, n, psi0)
+    got = emulate_plan(plan, n, psi0)
+    assert np.max(np.abs(got - want)) < 1e-12
+    covered = sorted(i for it in plan for i in it.gates)
+    if not merge:
+        assert covered == list(range(len(gates)))
+
+
+@pytest.mark.parametrize("kmax", [2, 3, 4, 5])
+def test_fused_plan_is_the_same_unitary(kmax):
+    n = 8
+    gates = CI.random_mix(n, 6, seed=kmax)
+    plan = P.plan_fused(gates, n, kmax)
+    assert all(len(it.qubits) <= kmax for it in plan)
+    psi0 = rand_state(n, 2)
+    assert np.max(np.abs(emulate_plan(plan, n, psi0) - simulate(gates, n, psi0))) < 1e-12
+    assert len(plan) < len(gates)
+
+
+def test_embed_matches_kron():
+    rng = np.random.default_rng(3)
+    U = np.linalg.qr(rng.normal(size=(4, 4)) + 1j * rng.normal(size=(4, 4)))[0]
+    block = [5, 1, 3]  # index bit j <-> block[j]
+    E = P.embed(U, [3, 5], block)
+    # compare on a 6-qubit state via full unitaries
+    n = 6
+    full = full_unitary(E, list(reversed(block)), n)
+    assert np.allclose(full, full_unitary(U, [3, 5], n))
+
+
+# --- C ABI surface -----------------------------------------------------------
+
+def declared_symbols():
+    text = open(os.path.join(REPO, "include", "qvmhip.h")).read()
+    return sorted(set(re.findall(r"QVMHIP_API\s+[\w\s\*]+?\b(qvmhip_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    syms = declared_symbols()
+    assert len(syms) >= 35
+    L = N.lib()
+    for s in syms:
+        assert hasattr(L, s), s
+        assert s in N.SIGNATURES, f"{s} has no ctypes signature"
+    assert N.lib().qvmhip_version().decode().startswith("qvmhip")
+
+
+def test_error_paths_without_a_device():
+    L = N.lib()
+    h = C.c_void_p()
+    cfg = N.Config(C.sizeof(N.Config), -1, 1 << 20, 0)  # 1 MiB cap
+    assert L.qvmhip_create(24, N.F64, C.byref(cfg), C.byref(h)) == N.ENOMEM
+    assert "cap" in N.last_error()
+    assert L.qvmhip_create(0, N.F64, None, C.byref(h)) == N.EINVAL
+    assert L.qvmhip_create(10, 7, None, C.byref(h)) == N.EINVAL
+    bad = N.Config(3, -1, 0, 0)
+    assert L.qvmhip_create(10, N.F64, C.byref(bad), C.byref(h)) == N.EINVAL
+    nbytes = C.c_uint64()
+    assert L.qvmhip_state_bytes(30, N.F64, C.byref(nbytes)) == N.OK
+    assert nbytes.value >= 16 << 30
+    assert L.qvmhip_apply_matrix(None, None, 1, None) == N.EINVAL
+    assert L.qvmhip_measure(None, 0, 0.5, 0, None, None) == N.EINVAL
