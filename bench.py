@@ -1,0 +1,262 @@
+#!/usr/bin/env python3
+"""Benchmark of the pure-state path on one MI355X.
+
+Workload (one step): a 30-qubit FP64 statevector (16 GiB, resident in HBM)
+  (a) layered circuit, 40 layers of RX(a) q, RZ(b) q on every qubit plus
+      brickwork CNOTs (2400 1-qubit gates + 580 CNOTs), from |0...0>;
+  (b) QFT-30 (30 H, 435 CPHASE, 15 SWAP) from |0...0>;
+  then 10^6 shots sampled from the FP64 prefix sum of (b)'s state.
+Gates run on the plain path: one engine launch per gate, the arithmetic of
+the reference's serial kernels.
+
+value = amp-updates/s of the step: sum over gates of changed(g) (2^n for
+dense/diagonal 1q gates, 2^(n-1) for CNOT/SWAP, 2^(n-2) for CPHASE) divided
+by the step's wall time.  %HBM = 2 * changed * 16 B / time / 8 TB/s.
+
+Also reported (measured once each, outside the timed loop):
+  paths.tiled_f64  -- the same circuits as LDS tile passes (bit-identical),
+  paths.fused_f64  -- dense 4-qubit blocks on MFMA,
+  paths.fused_f32 / tiled_f32 -- FP32 states, with infidelity, max |d psi|
+                      and total-variation distance against the FP64 result.
+  cpu_baseline     -- the oracle (a C restatement of the reference CPU path,
+                      compiled mode, threaded) on a prefix of circuit (a).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK = 8.0e12  # B/s, MI355X
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=30)
+    ap.add_argument("--layers", type=int, default=40)
+    ap.add_argument("--shots", type=int, default=1_000_000)
+    ap.add_argument("--no-paths", action="store_true", help="skip the tiled/fused/FP32 comparison")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(sharding is a design, see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import numpy as np
+
+    from qvm_amd import circuits as C
+    from qvm_amd import gates as G
+    from qvm_amd import planner as P
+    from qvm_amd.state import HipPureState
+
+    n = args.n
+    ga = C.layered(n, args.layers, seed=1)
+    gb = C.qft(n)
+    upd_a, upd_b = C.amp_updates(ga, n), C.amp_updates(gb, n)
+    upd = upd_a + upd_b
+    u = C.uniforms(20250, args.shots)
+    nat_a = [(U, list(G.nat_tuple(*q))) for U, q in ga]
+    nat_b = [(U, list(G.nat_tuple(*q))) for U, q in gb]
+
+    st = HipPureState(n, "f64")
+
+    def run_plain(state, gates):
+        state.reset_zero()
+        for U, q in gates:
+            state.apply_matrix(U, q)
+
+    def step():
+        run_plain(st, nat_a)
+        run_plain(st, nat_b)
+        return st.sample(u)
+
+    log(f"bench: n={n} layered gates={len(ga)} qft gates={len(gb)} amp-updates/step={upd:.4e}")
+    for w in range(args.warmup):
+        step()
+        log(f"warmup {w} done")
+    st.sync()
+    st.profile(True)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        shots = step()
+        log(f"step {k} done")
+    st.sync()
+    t1 = time.perf_counter()
+    prof = st.profile_summary()
+    st.profile(False)
+    step_s = (t1 - t0) / args.steps
+    value = upd / step_s
+    bytes_alg = 2 * upd * 16
+
+    # parity of the timed result: QFT|0> is the uniform superposition
+    amps = st.read_amplitudes(0, 4096)
+    qft_err = float(np.max(np.abs(amps - 2.0 ** (-n / 2))))
+    norm_b = st.norm()
+    hist_ok = int(shots.max()) < (1 << n)
+
+    # dominant kernel: family with the largest total time
+    dom, (dcount, dms) = max(prof.items(), key=lambda kv: kv[1][1])
+    per_launch_ms = dms / dcount
+    unit_bytes = 2 * (1 << n) * 16  # dense 1q / diagonal sweep: read + write every amplitude
+    roofline = {
+        "kernel": {"dense1": "qvmhip::k_dense<double, 1, 4>"}.get(dom, dom),
+        "bound": "hbm",
+        "achieved": round(unit_bytes / (per_launch_ms * 1e-3) / 1e9, 1),
+        "peak": 8000.0,
+        "unit": "GB/s",
+        "frac": round(unit_bytes / (per_launch_ms * 1e-3) / HBM_PEAK, 4),
+        "traffic": None,
+        "launches": dcount,
+        "avg_launch_ms": round(per_launch_ms, 4),
+        "algorithmic_bytes_per_launch": unit_bytes,
+    }
+    pmc = os.path.join(REPO, "profiles", "pmc_dense1_n30.json")
+    if os.path.exists(pmc) and n == 30:
+        with open(pmc) as f:
+            roofline["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+    kernels = {k: {"launches": c, "total_ms": round(t, 2), "avg_ms": round(t / c, 4)} for k, (c, t) in prof.items()}
+
+    out = {
+        "metric": "amp-updates/s + %HBM peak, 30q FP64 plain vs fused/FP32 at infidelity≤1e-6",
+        "value": value,
+        "unit": "amp-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step_s * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded (splitmix64) layered RX/RZ+CNOT and QFT circuits, seeded uniform draws",
+        "config": {
+            "workload": f"{n}-qubit FP64 statevector: layered RX/RZ+CNOT depth {args.layers} + QFT-{n}, "
+                        f"then {args.shots} shots from the FP64 prefix sum; plain gate-by-gate path",
+            "n_qubits": n, "layers": args.layers, "shots": args.shots,
+            "gates": len(ga) + len(gb), "amp_updates_per_step": upd,
+        },
+        "hbm_frac": round(bytes_alg / step_s / HBM_PEAK, 4),
+        "roofline": roofline,
+        "kernels": kernels,
+        "parity": {"qft_uniform_max_abs_err": qft_err, "norm": norm_b, "shots_in_range": hist_ok},
+    }
+
+    if not args.no_paths:
+        out["paths"] = run_paths(args, n, ga, gb, nat_a, upd_a, upd_b, u, st, HipPureState, P, np)
+
+    st.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(n, ga, args.cpu_seconds, args.cpu_threads)
+    print(json.dumps(out))
+    return 0
+
+
+def run_paths(args, n, ga, gb, nat_a, upd_a, upd_b, u, st, HipPureState, P, np):
+    """Tiled / fused / FP32 forms of the same circuits against the plain FP64 state."""
+    res = {}
+    # plain FP64 reference results for (a) and (b), kept on device
+    ref_a = HipPureState(n, "f64")
+    for U, q in nat_a:
+        ref_a.apply_matrix(U, q)
+    ref_a.sync()
+    st.reset_zero()  # st now holds the plain QFT result? rebuild it explicitly
+    for U, q in gb:
+        st.apply_matrix(U, list(reversed(q)))
+    st.sync()
+    ref_b = st
+    p64 = ref_a.sample(u)
+
+    def timed_program(state, plan):
+        prog = P.build_program(state, plan, use_graph=True)
+        state.reset_zero()
+        prog.run()  # warm
+        state.sync()
+        state.reset_zero()
+        state.sync()
+        t0 = time.perf_counter()
+        prog.run()
+        state.sync()
+        dt = time.perf_counter() - t0
+        prog.close()
+        return dt
+
+    def record(name, state, plan_a, plan_b):
+        ta = timed_program(state, plan_a)
+        da = state.max_abs_diff(ref_a)
+        ip = state.inner_product(ref_a)
+        pa = state.sample(u)
+        tb = timed_program(state, plan_b)
+        db = state.max_abs_diff(ref_b)
+        tot = ta + tb
+        r = {"amp_updates_per_s": (upd_a + upd_b) / tot, "s_layered": ta, "s_qft": tb,
+             "sweeps_layered": len(plan_a), "sweeps_qft": len(plan_b),
+             "max_abs_diff_vs_plain_f64": max(da, db),
+             "infidelity_vs_plain_f64": max(0.0, 1.0 - abs(ip) ** 2),
+             "shots_differing_vs_f64": int(np.count_nonzero(pa != p64))}
+        if state.dtype == "f32":
+            vals, cnt = np.unique(np.concatenate([pa, p64]), return_counts=False), None
+            h32 = np.unique(pa, return_counts=True)
+            h64 = np.unique(p64, return_counts=True)
+            d32 = dict(zip(h32[0].tolist(), h32[1].tolist()))
+            d64 = dict(zip(h64[0].tolist(), h64[1].tolist()))
+            keys = set(d32) | set(d64)
+            r["tv_distance_sampled"] = 0.5 * sum(abs(d32.get(k, 0) - d64.get(k, 0)) for k in keys) / len(u)
+            del vals, cnt
+        log(f"path {name}: {r['amp_updates_per_s']:.3e} amp-updates/s, diff {r['max_abs_diff_vs_plain_f64']:.2e}")
+        return r
+
+    s2 = HipPureState(n, "f64")
+    m64 = P.default_tile_m("f64")
+    res["tiled_f64"] = record("tiled_f64", s2, P.plan_tiled(ga, n, m64, 4), P.plan_tiled(gb, n, m64, 4))
+    res["tiled_f64"]["bit_identical_to_plain"] = res["tiled_f64"]["max_abs_diff_vs_plain_f64"] == 0.0
+    res["fused_f64"] = record("fused_f64", s2, P.plan_fused(ga, n, 4), P.plan_fused(gb, n, 4))
+    s2.close()
+    s3 = HipPureState(n, "f32")
+    m32 = P.default_tile_m("f32")
+    res["tiled_f32"] = record("tiled_f32", s3, P.plan_tiled(ga, n, m32, 5, merge_1q=True),
+                              P.plan_tiled(gb, n, m32, 5, merge_1q=True))
+    res["fused_f32"] = record("fused_f32", s3, P.plan_fused(ga, n, 5), P.plan_fused(gb, n, 5))
+    s3.close()
+    ref_a.close()
+    return res
+
+
+def cpu_baseline(n, ga, seconds, threads):
+    """The oracle (compiled-mode restatement of the reference CPU path) on a
+    prefix of the layered circuit, all gates at the full n."""
+    from oracle import oracle as O
+    from qvm_amd import gates as G
+    O.set_parallelization(19, threads)
+    psi = O.zero_state(n)
+    done, upd = 0, 0
+    t0 = time.perf_counter()
+    for U, q in ga:
+        O.apply_compiled(psi, U, G.nat_tuple(*q))
+        done += 1
+        upd += G.changed_amplitudes(U, n)
+        if time.perf_counter() - t0 > seconds and done >= 2:
+            break
+    dt = time.perf_counter() - t0
+    del psi
+    return {"value": upd / dt, "unit": "amp-updates/s", "cores": threads, "kind": "port",
+            "sample": f"first {done} gates of the {n}-qubit layered circuit, FP64, compiled mode "
+                      f"(pdotimes over complement groups), {threads} threads, {dt:.1f} s"}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -204,7 +204,30 @@ QVMHIP_API int qvmhip_program_finalize(qvmhip_program *p, int use_graph);
 QVMHIP_API int qvmhip_program_run(qvmhip_program *p);
 QVMHIP_API int qvmhip_program_destroy(qvmhip_program *p);
 
+/* max_i |a_i - b_i| over both components (FP64 accumulate); states may
+ * differ in dtype.  The parity check of a plan against the plain path. */
+QVMHIP_API int qvmhip_max_abs_diff(qvmhip_state *a, qvmhip_state *b, double *out);
+
 /* --- timing (engine-recorded kernel timings for the fusion cost model) --- */
+/* Kernel families reported by qvmhip_profile_summary. */
+enum {
+    QVMHIP_K_DENSE1 = 1,    /* dense 1-qubit (reduced) operator          */
+    QVMHIP_K_DENSE2 = 2,
+    QVMHIP_K_DENSE3 = 3,
+    QVMHIP_K_DENSE_BIG = 4, /* dense 4..6 qubits, LDS-staged matrix       */
+    QVMHIP_K_DIAG_FULL = 5,
+    QVMHIP_K_DIAG_SPARSE = 6,
+    QVMHIP_K_PERM = 7,
+    QVMHIP_K_TILE = 8,
+    QVMHIP_K_FUSED = 9,
+    QVMHIP_K_NKINDS = 10
+};
+/* on != 0: start recording an event pair around every kernel launch on the
+ * handle (previous records are dropped); on == 0: stop recording. */
+QVMHIP_API int qvmhip_profile_enable(qvmhip_state *s, int on);
+/* Waits for the handle, then reports the number of recorded launches of
+ * `kind` and their summed duration in milliseconds. */
+QVMHIP_API int qvmhip_profile_summary(qvmhip_state *s, int kind, uint64_t *count, double *total_ms);
 /* Time `reps` launches of the last program (or a single gate when p is
  * NULL) with HIP events on the handle's stream; *ms = mean per launch. */
 QVMHIP_API int qvmhip_time_program(qvmhip_program *p, int reps, double *ms);

@@ -101,7 +101,16 @@ SIGNATURES = {
     "qvmhip_program_run": [_vp],
     "qvmhip_program_destroy": [_vp],
     "qvmhip_time_program": [_vp, C.c_int, _dp],
+    "qvmhip_max_abs_diff": [_vp, _vp, _dp],
+    "qvmhip_profile_enable": [_vp, C.c_int],
+    "qvmhip_profile_summary": [_vp, C.c_int, _u64p, _dp],
 }
+
+# kernel families of qvmhip_profile_summary
+K_DENSE1, K_DENSE2, K_DENSE3, K_DENSE_BIG, K_DIAG_FULL, K_DIAG_SPARSE, K_PERM, K_TILE, K_FUSED = range(1, 10)
+K_NAMES = {K_DENSE1: "dense1", K_DENSE2: "dense2", K_DENSE3: "dense3", K_DENSE_BIG: "dense_big",
+           K_DIAG_FULL: "diag_full", K_DIAG_SPARSE: "diag_sparse", K_PERM: "perm", K_TILE: "tile",
+           K_FUSED: "fused"}
 _RESTYPE = {"qvmhip_last_error": C.c_char_p, "qvmhip_version": C.c_char_p}
 
 _lib = None

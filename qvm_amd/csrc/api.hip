@@ -67,6 +67,11 @@ int scalar_to_host(qvmhip_state *s, double *dev, double *out) {
 
 void free_state(qvmhip_state *s) {
     if (!s) return;
+    for (auto &r : s->prof) {
+        if (r.a) hipEventDestroy(r.a);
+        if (r.b) hipEventDestroy(r.b);
+    }
+    for (auto e : s->event_pool) hipEventDestroy(e);
     if (s->amps) hipFree(s->amps);
     if (s->red) hipFree(s->red);
     if (s->mat) hipFree(s->mat);
@@ -320,12 +325,48 @@ int qvmhip_expectation_zz(qvmhip_state *s, const uint32_t *edges, uint32_t nedge
 
 int qvmhip_inner_product(qvmhip_state *a, qvmhip_state *b, double *re, double *im) {
     if (!a || !b || !re || !im) return fail(QVMHIP_EINVAL, "null argument");
-    if (a->n != b->n || a->dtype != b->dtype) return fail(QVMHIP_EINVAL, "states differ in size or dtype");
+    if (a->n != b->n) return fail(QVMHIP_EINVAL, "states differ in size");
     if (a != b) {
         // b's queued work must finish before a's stream reads it
         QH_CHECK(hipStreamSynchronize(b->stream));
     }
     return inner_product(a, b, re, im);
+}
+
+int qvmhip_max_abs_diff(qvmhip_state *a, qvmhip_state *b, double *out) {
+    if (!a || !b || !out) return fail(QVMHIP_EINVAL, "null argument");
+    if (a->n != b->n) return fail(QVMHIP_EINVAL, "states differ in size");
+    if (a != b) QH_CHECK(hipStreamSynchronize(b->stream));
+    return max_abs_diff(a, b, out);
+}
+
+int qvmhip_profile_enable(qvmhip_state *s, int on) {
+    if (!s) return fail(QVMHIP_EINVAL, "null state");
+    QH_CHECK(hipStreamSynchronize(s->stream));
+    for (auto &r : s->prof) {
+        if (r.a) s->event_pool.push_back(r.a);
+        if (r.b) s->event_pool.push_back(r.b);
+    }
+    s->prof.clear();
+    s->profiling = on != 0;
+    return QVMHIP_OK;
+}
+
+int qvmhip_profile_summary(qvmhip_state *s, int kind, uint64_t *count, double *total_ms) {
+    if (!s || !count || !total_ms) return fail(QVMHIP_EINVAL, "null argument");
+    QH_CHECK(hipStreamSynchronize(s->stream));
+    uint64_t c = 0;
+    double t = 0.0;
+    for (auto &r : s->prof) {
+        if (r.kind != kind || !r.a || !r.b) continue;
+        float ms = 0.f;
+        QH_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
+        t += ms;
+        ++c;
+    }
+    *count = c;
+    *total_ms = t;
+    return QVMHIP_OK;
 }
 
 int qvmhip_force_measure(qvmhip_state *s, uint32_t q, int bit, double p1) {

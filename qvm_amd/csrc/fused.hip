@@ -27,17 +27,23 @@
 namespace qvmhip {
 
 template <typename T> struct Mfma;
+// Operand layouts (both instructions): lane = i + 16*k holds A[i][k] and
+// lane = j + 16*k holds B[k][j].  Result layouts differ: register e of lane
+// (j + 16*b) holds row 4*b + e for the f32 instruction (one group of four
+// rows per lane block) and row b + 4*e for the f64 one (four groups of one).
 template <> struct Mfma<double> {
     typedef double v4 __attribute__((ext_vector_type(4)));
     static __device__ inline v4 mma(double a, double b, v4 c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ inline int row(int blk, int e) { return blk + 4 * e; }
 };
 template <> struct Mfma<float> {
     typedef float v4 __attribute__((ext_vector_type(4)));
     static __device__ inline v4 mma(float a, float b, v4 c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ inline int row(int blk, int e) { return 4 * blk + e; }
 };
 
 struct FusedOff {
@@ -103,12 +109,13 @@ __global__ __launch_bounds__(256) void k_fused(vec2_t<T> *__restrict__ psi, cons
                 acc[mt] = Mfma<T>::mma(a, b, acc[mt]);
             }
         }
-        // D fragment: lane holds rows 4*kk + e (e = 0..3) of column lj.
+        // D fragment: register e of this lane is row Mfma<T>::row(kk, e) of
+        // column lj.
         T *ycol = &sx[(wave * 16 + lj) * LX];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) ycol[mt * 16 + 4 * kk + e] = acc[mt][e];
+            for (int e = 0; e < 4; ++e) ycol[mt * 16 + Mfma<T>::row(kk, e)] = acc[mt][e];
         }
         __syncthreads();
         for (int e = threadIdx.x; e < NCB * D; e += 256) {
@@ -147,8 +154,10 @@ int run_fused(qvmhip_state *s, const void *mat_dev, const uint32_t *qubits) {
     for (int b = 0; b < K; ++b) has_q0 |= qubits[b] == 0;
     const uint64_t ntiles = (g.ngroups + kFusedCols - 1) / kFusedCols;
     const uint32_t grid = (uint32_t)std::min<uint64_t>(ntiles, 256 * 8);
+    prof_begin(s, QVMHIP_K_FUSED);
     hipLaunchKernelGGL((k_fused<T, K>), dim3(grid), dim3(256), 0, s->stream, (vec2_t<T> *)s->amps,
                        (const double *)mat_dev, g, off, has_q0 ? 0 : 1);
+    prof_end(s);
     QH_LAUNCH_CHECK();
     return QVMHIP_OK;
 }

@@ -388,8 +388,10 @@ int run_dense_small(qvmhip_state *s, const GateSplit &gs) {
         a.ur[i] = (T)gs.V[i].real();
         a.ui[i] = (T)gs.V[i].imag();
     }
+    prof_begin(s, KT);
     hipLaunchKernelGGL((k_dense<T, KT, GPT>), dim3(grid_for(a.g.ngroups, (uint64_t)kBlock * GPT)),
                        dim3(kBlock), 0, s->stream, (vec2_t<T> *)s->amps, a);
+    prof_end(s);
     QH_LAUNCH_CHECK();
     return QVMHIP_OK;
 }
@@ -414,8 +416,10 @@ int run_dense_big(qvmhip_state *s, const GateSplit &gs) {
     const uint64_t gpb = kBlock / D;
     const uint64_t nbatch = (g.ngroups + gpb - 1) / gpb;
     const uint32_t grid = (uint32_t)std::min<uint64_t>(nbatch, 256 * 16);
+    prof_begin(s, QVMHIP_K_DENSE_BIG);
     hipLaunchKernelGGL((k_dense_big<T, KT>), dim3(grid), dim3(kBlock), 0, s->stream,
                        (vec2_t<T> *)s->amps, (const T *)s->mat, g, off);
+    prof_end(s);
     QH_LAUNCH_CHECK();
     return QVMHIP_OK;
 }
@@ -438,8 +442,10 @@ int run_diag(qvmhip_state *s, const GateSplit &gs) {
             a.di[j] = (T)gs.V[(size_t)j * Dr + j].imag();
         }
         constexpr int PPT = 4;
+        prof_begin(s, QVMHIP_K_DIAG_FULL);
         hipLaunchKernelGGL((k_diag_full<T, PPT>), dim3(grid_for(a.len, (uint64_t)kBlock * PPT)),
                            dim3(kBlock), 0, s->stream, (vec2_t<T> *)s->amps, a);
+        prof_end(s);
         QH_LAUNCH_CHECK();
         return QVMHIP_OK;
     }
@@ -453,8 +459,10 @@ int run_diag(qvmhip_state *s, const GateSplit &gs) {
         a.di[e] = (T)gs.V[(size_t)ent[e] * Dr + ent[e]].imag();
     }
     constexpr int GPT = 2;
+    prof_begin(s, QVMHIP_K_DIAG_SPARSE);
     hipLaunchKernelGGL((k_diag_sparse<T, GPT>), dim3(grid_for(a.g.ngroups, (uint64_t)kBlock * GPT)),
                        dim3(kBlock), 0, s->stream, (vec2_t<T> *)s->amps, a);
+    prof_end(s);
     QH_LAUNCH_CHECK();
     return QVMHIP_OK;
 }
@@ -473,8 +481,10 @@ int run_perm(qvmhip_state *s, const GateSplit &gs) {
     }
     if (a.nent == 0) return QVMHIP_OK;
     constexpr int GPT = 2;
+    prof_begin(s, QVMHIP_K_PERM);
     hipLaunchKernelGGL((k_perm<T, GPT>), dim3(grid_for(a.g.ngroups, (uint64_t)kBlock * GPT)),
                        dim3(kBlock), 0, s->stream, (vec2_t<T> *)s->amps, a);
+    prof_end(s);
     QH_LAUNCH_CHECK();
     return QVMHIP_OK;
 }

@@ -167,8 +167,11 @@ int qvmhip_program_finalize(qvmhip_program *p, int use_graph) {
     p->use_graph = use_graph != 0;
     if (p->use_graph) {
         QH_CHECK(hipStreamSynchronize(s->stream));
+        const bool was_profiling = s->profiling;
+        s->profiling = false;  // no event records inside the captured graph
         QH_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
         int rc = replay(p);
+        s->profiling = was_profiling;
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(s->stream, &g);
         if (rc) {

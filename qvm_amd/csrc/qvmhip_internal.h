@@ -10,6 +10,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/qvmhip.h"
 
@@ -79,10 +80,40 @@ struct qvmhip_state {
     void *stage = nullptr;          // pinned host staging for read/write
     size_t stage_bytes = 0;
     uint32_t flags = 0;
-    uint64_t counters[4] = {0, 0, 0, 0};
+    // live kernel timing (qvmhip_profile_*): one event pair per launch
+    struct ProfRec {
+        int kind;
+        hipEvent_t a, b;
+    };
+    bool profiling = false;
+    std::vector<ProfRec> prof;
+    std::vector<hipEvent_t> event_pool;
 };
 
 namespace qvmhip {
+// Live per-launch timing.  prof_begin/prof_end bracket one kernel launch
+// with HIP events on the state's stream when profiling is on.
+inline hipEvent_t prof_event(qvmhip_state *s) {
+    if (!s->event_pool.empty()) {
+        hipEvent_t e = s->event_pool.back();
+        s->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+inline void prof_begin(qvmhip_state *s, int kind) {
+    if (!s->profiling) return;
+    qvmhip_state::ProfRec r{kind, prof_event(s), prof_event(s)};
+    if (r.a) hipEventRecord(r.a, s->stream);
+    s->prof.push_back(r);
+}
+inline void prof_end(qvmhip_state *s) {
+    if (!s->profiling || s->prof.empty()) return;
+    if (s->prof.back().b) hipEventRecord(s->prof.back().b, s->stream);
+}
+
 // Launch helpers implemented in the kernel files.
 int launch_gate(qvmhip_state *s, const double *U, uint32_t k, const uint32_t *qubits, int force_dense);
 int launch_diagonal(qvmhip_state *s, const double *d, uint32_t k, const uint32_t *qubits);
@@ -99,6 +130,7 @@ int launch_tile_pass(qvmhip_state *s, uint32_t m, const uint32_t *tile_qubits, u
 int launch_fused_dense(qvmhip_state *s, const void *mat_dev, uint32_t k, const uint32_t *qubits);
 int expectation_zz(qvmhip_state *s, const uint32_t *edges, uint32_t nedges, double *out);
 int inner_product(qvmhip_state *a, qvmhip_state *b, double *re, double *im);
+int max_abs_diff(qvmhip_state *a, qvmhip_state *b, double *out);
 int probabilities(qvmhip_state *s, uint64_t start, uint64_t count, double *host_out);
 int convert_f32_to_f64(qvmhip_state *s, uint64_t start, uint64_t count, double *dev_out);
 int convert_f64_to_f32(qvmhip_state *s, uint64_t start, uint64_t count, const double *dev_in);

@@ -111,9 +111,9 @@ __global__ __launch_bounds__(kBlock) void k_zz_partial(const vec2_t<T> *__restri
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_dot_partial(const vec2_t<T> *__restrict__ a,
-                                                        const vec2_t<T> *__restrict__ b,
+template <typename TA, typename TB>
+__global__ __launch_bounds__(kBlock) void k_dot_partial(const vec2_t<TA> *__restrict__ a,
+                                                        const vec2_t<TB> *__restrict__ b,
                                                         uint64_t count, double *__restrict__ partial) {
     __shared__ double sm[kBlock / kWave];
     const uint64_t stride = (uint64_t)gridDim.x * kBlock;
@@ -129,6 +129,41 @@ __global__ __launch_bounds__(kBlock) void k_dot_partial(const vec2_t<T> *__restr
     if (threadIdx.x == 0) {
         partial[2 * blockIdx.x] = tr;
         partial[2 * blockIdx.x + 1] = ti;
+    }
+}
+
+// max |a_i - b_i| over re and im parts, in FP64.
+template <typename TA, typename TB>
+__global__ __launch_bounds__(kBlock) void k_maxdiff_partial(const vec2_t<TA> *__restrict__ a,
+                                                            const vec2_t<TB> *__restrict__ b,
+                                                            uint64_t count, double *__restrict__ partial) {
+    __shared__ double sm[kBlock / kWave];
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    double m = 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += stride) {
+        const double dr = fabs((double)a[i].x - (double)b[i].x);
+        const double di = fabs((double)a[i].y - (double)b[i].y);
+        m = fmax(m, fmax(dr, di));
+        if (dr != dr || di != di) m = INFINITY;  // NaN counts as a mismatch
+    }
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, kWave));
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    if (lane == 0) sm[wid] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) t = fmax(t, sm[w]);
+        partial[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_final_max(const double *__restrict__ partial, int n,
+                                                      double *__restrict__ out) {
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int i = 0; i < n; ++i) m = fmax(m, partial[i]);
+        out[0] = m;
     }
 }
 
@@ -524,14 +559,43 @@ int expectation_zz(qvmhip_state *s, const uint32_t *edges, uint32_t nedges, doub
     return QVMHIP_OK;
 }
 
+template <template <typename, typename> class Launch>
+static void dispatch2(qvmhip_state *a, qvmhip_state *b, uint32_t grid) {
+    if (a->dtype == QVMHIP_F64 && b->dtype == QVMHIP_F64) Launch<double, double>::go(a, b, grid);
+    else if (a->dtype == QVMHIP_F64) Launch<double, float>::go(a, b, grid);
+    else if (b->dtype == QVMHIP_F64) Launch<float, double>::go(a, b, grid);
+    else Launch<float, float>::go(a, b, grid);
+}
+
+template <typename TA, typename TB> struct DotLaunch {
+    static void go(qvmhip_state *a, qvmhip_state *b, uint32_t grid) {
+        hipLaunchKernelGGL((k_dot_partial<TA, TB>), dim3(grid), dim3(kBlock), 0, a->stream,
+                           (const vec2_t<TA> *)a->amps, (const vec2_t<TB> *)b->amps, a->len, a->red);
+    }
+};
+
+template <typename TA, typename TB> struct DiffLaunch {
+    static void go(qvmhip_state *a, qvmhip_state *b, uint32_t grid) {
+        hipLaunchKernelGGL((k_maxdiff_partial<TA, TB>), dim3(grid), dim3(kBlock), 0, a->stream,
+                           (const vec2_t<TA> *)a->amps, (const vec2_t<TB> *)b->amps, a->len, a->red);
+    }
+};
+
+int max_abs_diff(qvmhip_state *a, qvmhip_state *b, double *out) {
+    const uint32_t grid = red_grid(a->len);
+    dispatch2<DiffLaunch>(a, b, grid);
+    QH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_final_max, dim3(1), dim3(64), 0, a->stream, a->red, (int)grid, a->red + kRedBlocks * 2);
+    QH_LAUNCH_CHECK();
+    QH_CHECK(hipMemcpyAsync(a->host, a->red + kRedBlocks * 2, sizeof(double), hipMemcpyDeviceToHost, a->stream));
+    QH_CHECK(hipStreamSynchronize(a->stream));
+    *out = a->host[0];
+    return QVMHIP_OK;
+}
+
 int inner_product(qvmhip_state *a, qvmhip_state *b, double *re, double *im) {
     const uint32_t grid = red_grid(a->len);
-    if (a->dtype == QVMHIP_F64)
-        hipLaunchKernelGGL(k_dot_partial<double>, dim3(grid), dim3(kBlock), 0, a->stream,
-                           (const double2 *)a->amps, (const double2 *)b->amps, a->len, a->red);
-    else
-        hipLaunchKernelGGL(k_dot_partial<float>, dim3(grid), dim3(kBlock), 0, a->stream,
-                           (const float2 *)a->amps, (const float2 *)b->amps, a->len, a->red);
+    dispatch2<DotLaunch>(a, b, grid);
     QH_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(kBlock), 0, a->stream, a->red, (int)grid, 2,
                        a->red + kRedBlocks * 2);

@@ -76,8 +76,7 @@ __global__ __launch_bounds__(NT) void k_tile(vec2_t<T> *__restrict__ psi, const 
             const T u00r = (T)op.m[0], u00i = (T)op.m[1], u01r = (T)op.m[2], u01i = (T)op.m[3];
             const T u10r = (T)op.m[4], u10i = (T)op.m[5], u11r = (T)op.m[6], u11i = (T)op.m[7];
 #pragma unroll
-            for (int r = 0; r < R / 2; ++r) {
-                const uint32_t p = t + r * NT;
+            for (uint32_t p = t; p < S / 2; p += NT) {
                 const uint32_t ia = ins0(p, q0), ib = ia | (1u << q0);
                 if ((ia & cm) != cm) continue;
                 const V xa = sa[ia], xb = sa[ib];
@@ -97,8 +96,7 @@ __global__ __launch_bounds__(NT) void k_tile(vec2_t<T> *__restrict__ psi, const 
         } else if (kind == QVMHIP_TILE_DENSE2) {
             const uint32_t lo = q0 < q1 ? q0 : q1, hi = q0 < q1 ? q1 : q0;
 #pragma unroll
-            for (int r = 0; r < R / 4; ++r) {
-                const uint32_t p = t + r * NT;
+            for (uint32_t p = t; p < S / 4; p += NT) {
                 const uint32_t i0 = ins0(ins0(p, lo), hi);
                 if ((i0 & cm) != cm) continue;
                 const uint32_t idx[4] = {i0, i0 | (1u << q0), i0 | (1u << q1), i0 | (1u << q0) | (1u << q1)};
@@ -125,8 +123,7 @@ __global__ __launch_bounds__(NT) void k_tile(vec2_t<T> *__restrict__ psi, const 
         } else if (kind == QVMHIP_TILE_DIAG1 || kind == QVMHIP_TILE_DIAG2) {
             const bool two = kind == QVMHIP_TILE_DIAG2;
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const uint32_t l = t + r * NT;
+            for (uint32_t l = t; l < S; l += NT) {
                 if ((l & cm) != cm) continue;
                 uint32_t j = (l >> q0) & 1u;
                 if (two) j |= ((l >> q1) & 1u) << 1;
@@ -139,8 +136,7 @@ __global__ __launch_bounds__(NT) void k_tile(vec2_t<T> *__restrict__ psi, const 
             // target q0, control q1: swap (c=1,t=0) <-> (c=1,t=1)
             const uint32_t lo = q0 < q1 ? q0 : q1, hi = q0 < q1 ? q1 : q0;
 #pragma unroll
-            for (int r = 0; r < R / 4; ++r) {
-                const uint32_t p = t + r * NT;
+            for (uint32_t p = t; p < S / 4; p += NT) {
                 const uint32_t i0 = ins0(ins0(p, lo), hi) | (1u << q1);
                 if ((i0 & cm) != cm) continue;
                 const uint32_t i1 = i0 | (1u << q0);
@@ -151,8 +147,7 @@ __global__ __launch_bounds__(NT) void k_tile(vec2_t<T> *__restrict__ psi, const 
         } else if (kind == QVMHIP_TILE_SWAP) {
             const uint32_t lo = q0 < q1 ? q0 : q1, hi = q0 < q1 ? q1 : q0;
 #pragma unroll
-            for (int r = 0; r < R / 4; ++r) {
-                const uint32_t p = t + r * NT;
+            for (uint32_t p = t; p < S / 4; p += NT) {
                 const uint32_t i0 = ins0(ins0(p, lo), hi);
                 if ((i0 & cm) != cm) continue;
                 const uint32_t ia = i0 | (1u << q0), ib = i0 | (1u << q1);
@@ -164,8 +159,7 @@ __global__ __launch_bounds__(NT) void k_tile(vec2_t<T> *__restrict__ psi, const 
             const uint32_t lo = q0 < q1 ? q0 : q1, hi = q0 < q1 ? q1 : q0;
             const T mr = (T)op.m[0], mi = (T)op.m[1];
 #pragma unroll
-            for (int r = 0; r < R / 4; ++r) {
-                const uint32_t p = t + r * NT;
+            for (uint32_t p = t; p < S / 4; p += NT) {
                 const uint32_t i0 = ins0(ins0(p, lo), hi) | (1u << q0) | (1u << q1);
                 if ((i0 & cm) != cm) continue;
                 const V x = sa[i0];
@@ -212,8 +206,10 @@ int run_tile(qvmhip_state *s, const uint32_t *tq, uint32_t m, const qvmhip_tile_
             if ((l >> b) & 1u) o |= uint64_t(1) << tq[b];
         a.roff[r] = o;
     }
+    prof_begin(s, QVMHIP_K_TILE);
     hipLaunchKernelGGL((k_tile<T, M, NT>), dim3((uint32_t)a.g.ngroups), dim3(NT), 0, s->stream,
                        (vec2_t<T> *)s->amps, a, ops_dev, nops);
+    prof_end(s);
     QH_LAUNCH_CHECK();
     return QVMHIP_OK;
 }

@@ -70,13 +70,40 @@ class _Ready:
         self.remaining -= 1
 
 
+def group_in_order(gates_qubits: List[Tuple[int, ...]], width: int, seed_qubits=(),
+                   standalone=lambda i: False) -> List[Tuple[str, List[int], List[int]]]:
+    """Groups of consecutive gates (original order kept, so the grouped
+    execution performs exactly the plain path's floating-point operations)."""
+    out = []
+    T: set = set(seed_qubits)
+    members: List[int] = []
+    for i, qs in enumerate(gates_qubits):
+        if standalone(i):
+            if members:
+                out.append(("group", sorted(T), members))
+            out.append(("single", list(qs), [i]))
+            T, members = set(seed_qubits), []
+            continue
+        if len(T | set(qs)) > width:
+            out.append(("group", sorted(T), members))
+            T, members = set(seed_qubits), []
+        T |= set(qs)
+        members.append(i)
+    if members:
+        out.append(("group", sorted(T), members))
+    return out
+
+
 def group_gates(gates_qubits: List[Tuple[int, ...]], nq: int, width: int, seed_qubits=(),
                 standalone=lambda i: False) -> List[Tuple[str, List[int], List[int]]]:
     """Greedy dependency-aware grouping.
 
     Returns a list of (kind, qubit set, gate indices in application order);
     kind is "group" or "single" (a gate that ``standalone`` excludes from
-    groups, applied on its own).
+    groups, applied on its own).  Gates on disjoint qubits may move past each
+    other: the unitary is unchanged, the rounding of the result is not
+    (floating-point gate applications on disjoint qubits do not commute
+    bit-for-bit).
     """
     R = _Ready(gates_qubits, nq)
     out = []
@@ -261,8 +288,13 @@ def _merge_runs_1q(gates: List[Gate]) -> List[Gate]:
     return merged
 
 
-def plan_tiled(gates: List[Gate], nq: int, m: int, low: int = 4, merge_1q: bool = False) -> List[PlanItem]:
-    """LDS tile passes of up to m qubits that always include qubits 0..low-1."""
+def plan_tiled(gates: List[Gate], nq: int, m: int, low: int = 4, merge_1q: bool = False,
+               reorder: bool = True) -> List[PlanItem]:
+    """LDS tile passes of up to m qubits that always include qubits 0..low-1.
+
+    reorder=False keeps the original gate order, so the FP64 result is
+    bit-identical to the plain path; reorder=True lets commuting gates move
+    into earlier passes (far fewer passes, results equal to rounding)."""
     if merge_1q:
         gates = _merge_runs_1q(gates)
     if nq < 8:
@@ -270,7 +302,10 @@ def plan_tiled(gates: List[Gate], nq: int, m: int, low: int = 4, merge_1q: bool 
     m = min(m, nq)
     low = min(low, m)
     qs = [tuple(q) for _, q in gates]
-    groups = group_gates(qs, nq, m, seed_qubits=range(low), standalone=lambda i: len(qs[i]) > 2)
+    if reorder:
+        groups = group_gates(qs, nq, m, seed_qubits=range(low), standalone=lambda i: len(qs[i]) > 2)
+    else:
+        groups = group_in_order(qs, m, seed_qubits=range(low), standalone=lambda i: len(qs[i]) > 2)
     plan = []
     for kind, T, members in groups:
         if kind == "single":
@@ -336,13 +371,14 @@ def default_low(dtype: str) -> int:
 
 
 def compile_gates(state, gates: List[Gate], mode: str = "tiled", use_graph: bool = True, m: int = 0,
-                  kmax: int = 4, merge_1q: bool = False):
+                  kmax: int = 4, merge_1q: bool = False, reorder: bool = True):
     from .qvm import CompiledProgram
     nq = state.num_qubits
     if mode == "plain":
         plan = plan_plain(gates, nq)
     elif mode == "tiled":
-        plan = plan_tiled(gates, nq, m or default_tile_m(state.dtype), default_low(state.dtype), merge_1q)
+        plan = plan_tiled(gates, nq, m or default_tile_m(state.dtype), default_low(state.dtype), merge_1q,
+                          reorder)
     elif mode == "fused":
         plan = plan_fused(gates, nq, kmax)
     else:

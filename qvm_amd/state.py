@@ -189,6 +189,26 @@ class HipPureState:
     def amplitudes(self) -> np.ndarray:
         return self.read_amplitudes()
 
+    def max_abs_diff(self, other: "HipPureState") -> float:
+        out = C.c_double()
+        N.check(N.lib().qvmhip_max_abs_diff(self._h, other._h, C.byref(out)))
+        return out.value
+
+    # -- live kernel timing ----------------------------------------------
+    def profile(self, on: bool = True) -> None:
+        N.check(N.lib().qvmhip_profile_enable(self._h, 1 if on else 0))
+
+    def profile_summary(self) -> dict:
+        """{family: (launches, total ms)} of the launches recorded since
+        profile(True)."""
+        out = {}
+        for k, name in N.K_NAMES.items():
+            c, t = C.c_uint64(), C.c_double()
+            N.check(N.lib().qvmhip_profile_summary(self._h, k, C.byref(c), C.byref(t)))
+            if c.value:
+                out[name] = (c.value, t.value)
+        return out
+
     def copy_from(self, other: "HipPureState") -> None:
         """copy-wavefunction (src/wavefunction.lisp:93-109)."""
         N.check(N.lib().qvmhip_copy(self._h, other._h))

@@ -246,7 +246,8 @@ def test_sample_cdf_rule(S, oracle):
         bad = np.nonzero(got != want)[0]
         for i in bad:
             assert near_boundary(cdf, u[i], int(want[i]), 1e-13), (n, u[i], got[i], want[i])
-        assert len(bad) <= max(2, 64 // 8)
+        # draws 3000.. sit exactly on CDF values: only those may disagree
+        assert np.count_nonzero(bad < 3000) <= 1
         got2 = st.sample_measure_all(u)
         want2 = np.array([oracle.sample_bisect(psi, p) for p in u[:500]], dtype=np.uint64)
         bad2 = np.nonzero(got2[:500] != want2)[0]
@@ -311,19 +312,41 @@ def test_determinism(S):
 
 # --- tile passes, fused blocks, FP32, programs ------------------------------
 
-@pytest.mark.parametrize("m", [8, 10, 12])
-def test_tile_passes_bit_exact_vs_plain(S, oracle, m):
+@pytest.mark.parametrize("reorder", [False, True])
+@pytest.mark.parametrize("m", [8, 9, 10, 11, 12])
+def test_tile_passes_vs_plain(S, oracle, m, reorder):
+    """In-order tile passes reproduce the plain path bit for bit; reordered
+    passes (commuting gates moved) agree to rounding."""
     n = 16
     gates = CI.layered(n, 6, seed=5) + CI.qft(n)[:40] + [(G.gate_matrix("RZZ", [0.7]), [3, 14]),
-                                                         (G.controlled(rand_unitary(1, np.random.default_rng(1))), [15, 2])]
-    plan = P.plan_tiled(gates, n, m=m, low=4)
-    say(f"tile m={m} passes={sum(1 for it in plan if it.kind == 'tile')} items={len(plan)}")
+                                                         (G.controlled(rand_unitary(1, np.random.default_rng(1))), [15, 2]),
+                                                         (G.gate_matrix("CCNOT"), [1, 9, 13])]
+    plan = P.plan_tiled(gates, n, m=m, low=4, reorder=reorder)
+    say(f"tile m={m} reorder={reorder} passes={sum(1 for it in plan if it.kind == 'tile')} items={len(plan)}")
     st = S(n, "f64")
     P.execute_plan(st, plan)
     ref = oracle.zero_state(n)
     for U, q in gates:
         oracle.apply_matrix(ref, U, G.nat_tuple(*q))
-    np.testing.assert_array_equal(st.amplitudes(), ref)
+    got = st.amplitudes()
+    if reorder:
+        assert np.max(np.abs(got - ref)) < 1e-14
+    else:
+        np.testing.assert_array_equal(got, ref)
+    st.close()
+
+
+def test_tile_pass_f32(S, oracle):
+    n = 14
+    gates = CI.layered(n, 4, seed=6)
+    plan = P.plan_tiled(gates, n, m=13, low=5, merge_1q=True)
+    st = S(n, "f32")
+    P.execute_plan(st, plan)
+    ref = oracle.zero_state(n)
+    for U, q in gates:
+        oracle.apply_matrix(ref, U, G.nat_tuple(*q))
+    got = st.amplitudes()
+    assert np.max(np.abs(got - ref)) < 1e-6
     st.close()
 
 
@@ -369,10 +392,14 @@ def test_f32_layered_infidelity(S, oracle):
         oracle.apply_matrix(ref, U, G.nat_tuple(*q))
         st.apply_matrix(U, G.nat_tuple(*q))
     got = st.amplitudes()
-    infid = 1 - abs(np.vdot(ref, got)) ** 2
-    say(f"f32 infidelity {infid:.3e} max err {np.max(np.abs(got - ref)):.3e}")
-    assert infid < 1e-6
-    assert abs(st.norm() - 1) < 1e-5
+    # fidelity of the normalised states: FP32 gate matrices are unitary only
+    # to ~6e-8, so the FP32 norm drifts; it is renormalised in FP64.
+    infid = 1 - abs(np.vdot(ref, got)) ** 2 / np.vdot(got, got).real
+    say(f"f32 infidelity {infid:.3e} max err {np.max(np.abs(got - ref)):.3e} norm {st.norm():.9f}")
+    assert infid < 1e-9
+    assert abs(st.norm() - 1) < 1e-4
+    st.normalize()
+    assert abs(st.norm() - 1) < 1e-6
     st.close()
 
 
