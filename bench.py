@@ -51,11 +51,17 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--workload", default="circuits", choices=["circuits", "precision"],
+                    help="circuits: the headline line (default); precision: FP32 vs FP64 report")
+    ap.add_argument("--big-n", type=int, default=34, help="FP32-only size of the precision report")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(sharding is a design, see DESIGN.md); run with --gpus 1", file=sys.stderr)
         return 2
+
+    if args.workload == "precision":
+        return precision(args)
 
     import numpy as np
 
@@ -114,7 +120,7 @@ def main() -> int:
     per_launch_ms = dms / dcount
     unit_bytes = 2 * (1 << n) * 16  # dense 1q / diagonal sweep: read + write every amplitude
     roofline = {
-        "kernel": {"dense1": "qvmhip::k_dense<double, 1, 4>"}.get(dom, dom),
+        "kernel": {"dense1": "qvmhip::k_dense<double, 1, 1, true|false>"}.get(dom, dom),
         "bound": "hbm",
         "achieved": round(unit_bytes / (per_launch_ms * 1e-3) / 1e9, 1),
         "peak": 8000.0,
@@ -211,14 +217,12 @@ def run_paths(args, n, ga, gb, nat_a, upd_a, upd_b, u, st, HipPureState, P, np):
              "max_abs_diff_vs_plain_f64": max(da, db),
              "infidelity_vs_plain_f64": max(0.0, 1.0 - abs(ip) ** 2),
              "shots_differing_vs_f64": int(np.count_nonzero(pa != p64))}
+        # TV distance of the sampled distribution |psi|^2 against FP64
+        r["tv_distance_vs_plain_f64"] = state.tv_distance(ref_a)
         if state.dtype == "f32":
             r["norm_before_renormalisation"] = nrm
-            h32 = np.unique(pa, return_counts=True)
-            h64 = np.unique(p64, return_counts=True)
-            d32 = dict(zip(h32[0].tolist(), h32[1].tolist()))
-            d64 = dict(zip(h64[0].tolist(), h64[1].tolist()))
-            keys = set(d32) | set(d64)
-            r["tv_distance_sampled"] = 0.5 * sum(abs(d32.get(k, 0) - d64.get(k, 0)) for k in keys) / len(u)
+            # same uniforms: shots land on the same or a neighbouring index
+            r["shots_median_index_shift"] = float(np.median(np.abs(pa.astype(np.int64) - p64.astype(np.int64))))
         log(f"path {name}: {r['amp_updates_per_s']:.3e} amp-updates/s, diff {r['max_abs_diff_vs_plain_f64']:.2e}")
         return r
 
@@ -238,6 +242,78 @@ def run_paths(args, n, ga, gb, nat_a, upd_a, upd_b, u, st, HipPureState, P, np):
     s3.close()
     ref_a.close()
     return res
+
+
+def precision(args) -> int:
+    """FP32 against FP64 on Haar-U3 + CZ-matching circuits of depth 20: n=30
+    in both dtypes (infidelity, max |d psi|, TV distance of the outcome
+    distributions), and the FP32-only size (default n=34, 128 GiB) timed on
+    the plain and tiled paths."""
+    from qvm_amd import circuits as C
+    from qvm_amd import gates as G
+    from qvm_amd import planner as P
+    from qvm_amd.state import HipPureState
+
+    def run(state, gates, mode):
+        state.reset_zero()
+        state.sync()
+        t0 = time.perf_counter()
+        if mode == "plain":
+            for U, q in gates:
+                state.apply_matrix(U, G.nat_tuple(*q))
+        else:
+            m = P.default_tile_m(state.dtype)
+            prog = P.build_program(state, P.plan_tiled(gates, state.num_qubits, m, P.default_low(state.dtype)),
+                                   use_graph=True)
+            state.reset_zero()
+            state.sync()
+            t0 = time.perf_counter()
+            prog.run()
+        state.sync()
+        dt = time.perf_counter() - t0
+        if mode != "plain":
+            prog.close()
+        return dt
+
+    n = args.n
+    gates = C.haar_cz(n, 20, seed=3)
+    upd = C.amp_updates(gates, n)
+    out = {"metric": "FP32 vs FP64 precision, Haar-U3 + CZ matching, depth 20", "n": n,
+           "gates": len(gates), "amp_updates": upd, "results": {}}
+    ref = HipPureState(n, "f64")
+    t64 = run(ref, gates, "plain")
+    log(f"precision: f64 plain {t64:.2f} s")
+    out["results"]["f64_plain"] = {"s": t64, "amp_updates_per_s": upd / t64, "norm": ref.norm()}
+    for mode in ("plain", "tiled"):
+        s32 = HipPureState(n, "f32")
+        dt = run(s32, gates, mode)
+        nrm = s32.norm()
+        ip = s32.inner_product(ref) / nrm
+        s32.normalize(nrm)
+        r = {"s": dt, "amp_updates_per_s": upd / dt, "norm_before_renormalisation": nrm,
+             "infidelity_vs_f64": max(0.0, 1.0 - abs(ip) ** 2),
+             "max_abs_amp_err_vs_f64": s32.max_abs_diff(ref),
+             "tv_distance_vs_f64": s32.tv_distance(ref)}
+        log(f"precision: f32 {mode} {dt:.2f} s infidelity {r['infidelity_vs_f64']:.2e}")
+        out["results"][f"f32_{mode}"] = r
+        s32.close()
+    t64t = run(ref, gates, "tiled")
+    out["results"]["f64_tiled"] = {"s": t64t, "amp_updates_per_s": upd / t64t}
+    ref.close()
+    nb = args.big_n
+    if nb:
+        gb = C.haar_cz(nb, 20, seed=3)
+        updb = C.amp_updates(gb, nb)
+        big = HipPureState(nb, "f32")
+        for mode in ("tiled", "plain"):
+            dt = run(big, gb, mode)
+            nrm = big.norm()
+            out["results"][f"f32_n{nb}_{mode}"] = {"s": dt, "amp_updates_per_s": updb / dt, "gates": len(gb),
+                                                   "norm": nrm, "state_GiB": (1 << nb) * 8 / 2 ** 30}
+            log(f"precision: f32 n={nb} {mode} {dt:.2f} s norm {nrm:.9f}")
+        big.close()
+    print(json.dumps(out))
+    return 0
 
 
 def cpu_baseline(n, ga, seconds, threads):

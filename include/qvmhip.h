@@ -207,6 +207,9 @@ QVMHIP_API int qvmhip_program_destroy(qvmhip_program *p);
 /* max_i |a_i - b_i| over both components (FP64 accumulate); states may
  * differ in dtype.  The parity check of a plan against the plain path. */
 QVMHIP_API int qvmhip_max_abs_diff(qvmhip_state *a, qvmhip_state *b, double *out);
+/* Total-variation distance of the two outcome distributions,
+ * 1/2 sum_i | |a_i|^2 - |b_i|^2 | (FP64 accumulate); dtypes may differ. */
+QVMHIP_API int qvmhip_tv_distance(qvmhip_state *a, qvmhip_state *b, double *out);
 
 /* --- timing (engine-recorded kernel timings for the fusion cost model) --- */
 /* Kernel families reported by qvmhip_profile_summary. */

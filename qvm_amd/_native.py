@@ -102,6 +102,7 @@ SIGNATURES = {
     "qvmhip_program_destroy": [_vp],
     "qvmhip_time_program": [_vp, C.c_int, _dp],
     "qvmhip_max_abs_diff": [_vp, _vp, _dp],
+    "qvmhip_tv_distance": [_vp, _vp, _dp],
     "qvmhip_profile_enable": [_vp, C.c_int],
     "qvmhip_profile_summary": [_vp, C.c_int, _u64p, _dp],
 }

@@ -340,6 +340,13 @@ int qvmhip_max_abs_diff(qvmhip_state *a, qvmhip_state *b, double *out) {
     return max_abs_diff(a, b, out);
 }
 
+int qvmhip_tv_distance(qvmhip_state *a, qvmhip_state *b, double *out) {
+    if (!a || !b || !out) return fail(QVMHIP_EINVAL, "null argument");
+    if (a->n != b->n) return fail(QVMHIP_EINVAL, "states differ in size");
+    if (a != b) QH_CHECK(hipStreamSynchronize(b->stream));
+    return tv_distance(a, b, out);
+}
+
 int qvmhip_profile_enable(qvmhip_state *s, int on) {
     if (!s) return fail(QVMHIP_EINVAL, "null state");
     QH_CHECK(hipStreamSynchronize(s->stream));

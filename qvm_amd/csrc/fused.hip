@@ -11,13 +11,14 @@
 // form), so rounding stays that of an ordinary complex matvec with FMA.
 //
 // Workgroup: 4 wavefronts, 64 complement groups (columns) per tile; wave w
-// owns columns 16w..16w+15 and all 2D/16 row tiles.  U is staged once per
-// workgroup in LDS (complex, rows padded by one element so the 16 lanes of a
-// fragment hit distinct banks); each X tile is gathered into LDS with
-// coalesced loads, multiplied with v_mfma_f64_16x16x4f64 /
-// v_mfma_f32_16x16x4f32, written back into the wave's own LDS columns and
-// stored with the same coalesced pattern.  The workgroup grid-strides over
-// tiles, so U is read from HBM once per workgroup, not once per tile.
+// owns columns 16w..16w+15 and all 2D/16 row tiles.  The A fragments of U
+// are loaded once per workgroup into registers (k <= 5) or LDS (k = 6, rows
+// padded by one element so the 16 lanes of a fragment hit distinct banks).
+// Each X tile is gathered with coalesced loads one tile ahead (software
+// pipelined, so HBM latency overlaps the MFMAs), staged in LDS, multiplied
+// with v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32, written back into the
+// wave's own LDS columns and stored with the same coalesced pattern.  The
+// workgroup grid-strides over tiles.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -64,53 +65,91 @@ __global__ __launch_bounds__(256) void k_fused(vec2_t<T> *__restrict__ psi, cons
     constexpr int MT = R2 / 16;     // row tiles
     constexpr int KS = R2 / 4;      // k-steps
     constexpr int NCB = kFusedCols;
-    __shared__ T su[D * LU];
+    constexpr int EPT = NCB * D / 256;   // amplitudes per thread per tile
+    // A fragments live in registers up to k = 5 (MT*KS values per lane);
+    // the 64 x 64 matrix of k = 6 is read from LDS.
+    constexpr bool AREG = K <= 5;
+    __shared__ T su[AREG ? 1 : D * LU];
     __shared__ T sx[NCB * LX];
 
-    for (int e = threadIdx.x; e < D * D; e += 256) {
-        const int i = e / D, j = e % D;
-        su[i * LU + 2 * j] = (T)U[2 * e];
-        su[i * LU + 2 * j + 1] = (T)U[2 * e + 1];
-    }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lj = lane & 15, kk = lane >> 4;
-    // A fragment of lane: row r = mt*16 + lj, col c = 4*ks + kk
+    // A fragment of lane: row r = mt*16 + lj, col c = 4*ks + kk; the real
+    // embedding picks Re or Im of U[r/2][c/2] and a sign from the parities.
     const int rpar = lj & 1, cpar = kk & 1;
-    const bool use_im = rpar != cpar;
+    const int use_im = rpar != cpar ? 1 : 0;
     const T sgn = (rpar == 0 && cpar == 1) ? T(-1) : T(1);
+    T afrag[AREG ? MT * KS : 1];
+    if constexpr (AREG) {
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                const int ri = (mt * 16 + lj) >> 1, cj = (4 * ks + kk) >> 1;
+                afrag[mt * KS + ks] = sgn * (T)U[2 * (ri * D + cj) + use_im];
+            }
+    } else {
+        for (int e = threadIdx.x; e < D * D; e += 256) {
+            const int i = e / D, j = e % D;
+            su[i * LU + 2 * j] = (T)U[2 * e];
+            su[i * LU + 2 * j + 1] = (T)U[2 * e + 1];
+        }
+    }
+
+    // element e of a tile -> (column gi, amplitude a)
+    auto col_of = [&](int e) { return group_fastest ? (e % NCB) : (e / D); };
+    auto amp_of = [&](int e) { return group_fastest ? (e / NCB) : (e % D); };
+    auto load_tile = [&](uint64_t tile, V *x) {
+#pragma unroll
+        for (int j = 0; j < EPT; ++j) {
+            const int e = threadIdx.x + 256 * j;
+            const uint64_t grp = tile * NCB + col_of(e);
+            if (grp < g.ngroups) {
+                x[j] = psi[group_base(g, grp) + off.o[amp_of(e)]];
+            } else {
+                x[j].x = T(0);
+                x[j].y = T(0);
+            }
+        }
+    };
 
     const uint64_t ntiles = (g.ngroups + NCB - 1) / NCB;
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        __syncthreads();
-        for (int e = threadIdx.x; e < NCB * D; e += 256) {
-            const int gi = group_fastest ? (e % NCB) : (e / D);
-            const int a = group_fastest ? (e / NCB) : (e % D);
-            const uint64_t grp = tile * NCB + gi;
-            V x;
-            x.x = T(0);
-            x.y = T(0);
-            if (grp < g.ngroups) x = psi[group_base(g, grp) + off.o[a]];
-            sx[gi * LX + 2 * a] = x.x;
-            sx[gi * LX + 2 * a + 1] = x.y;
+    V pre[EPT];
+    uint64_t tile = blockIdx.x;
+    if (tile < ntiles) load_tile(tile, pre);
+    for (; tile < ntiles; tile += gridDim.x) {
+        __syncthreads();  // previous tile fully stored
+#pragma unroll
+        for (int j = 0; j < EPT; ++j) {
+            const int e = threadIdx.x + 256 * j;
+            sx[col_of(e) * LX + 2 * amp_of(e)] = pre[j].x;
+            sx[col_of(e) * LX + 2 * amp_of(e) + 1] = pre[j].y;
         }
         __syncthreads();
+        const uint64_t next = tile + gridDim.x;
+        if (next < ntiles) load_tile(next, pre);  // in flight during the MFMAs
         v4 acc[MT];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) acc[mt] = v4{0, 0, 0, 0};
         const T *xcol = &sx[(wave * 16 + lj) * LX];
-#pragma unroll 4
+#pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
             const T b = xcol[4 * ks + kk];
-            const int cj = (4 * ks + kk) >> 1;
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
-                const int ri = (mt * 16 + lj) >> 1;
-                const T a = sgn * su[ri * LU + 2 * cj + (use_im ? 1 : 0)];
+                T a;
+                if constexpr (AREG) {
+                    a = afrag[mt * KS + ks];
+                } else {
+                    const int ri = (mt * 16 + lj) >> 1, cj = (4 * ks + kk) >> 1;
+                    a = sgn * su[ri * LU + 2 * cj + use_im];
+                }
                 acc[mt] = Mfma<T>::mma(a, b, acc[mt]);
             }
         }
         // D fragment: register e of this lane is row Mfma<T>::row(kk, e) of
-        // column lj.
+        // column lj.  Each wave only overwrites its own 16 columns, which no
+        // other wave reads.
         T *ycol = &sx[(wave * 16 + lj) * LX];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
@@ -118,15 +157,15 @@ __global__ __launch_bounds__(256) void k_fused(vec2_t<T> *__restrict__ psi, cons
             for (int e = 0; e < 4; ++e) ycol[mt * 16 + Mfma<T>::row(kk, e)] = acc[mt][e];
         }
         __syncthreads();
-        for (int e = threadIdx.x; e < NCB * D; e += 256) {
-            const int gi = group_fastest ? (e % NCB) : (e / D);
-            const int a = group_fastest ? (e / NCB) : (e % D);
-            const uint64_t grp = tile * NCB + gi;
+#pragma unroll
+        for (int j = 0; j < EPT; ++j) {
+            const int e = threadIdx.x + 256 * j;
+            const uint64_t grp = tile * NCB + col_of(e);
             if (grp >= g.ngroups) continue;
             V y;
-            y.x = sx[gi * LX + 2 * a];
-            y.y = sx[gi * LX + 2 * a + 1];
-            psi[group_base(g, grp) + off.o[a]] = y;
+            y.x = sx[col_of(e) * LX + 2 * amp_of(e)];
+            y.y = sx[col_of(e) * LX + 2 * amp_of(e) + 1];
+            psi[group_base(g, grp) + off.o[amp_of(e)]] = y;
         }
     }
 }

@@ -41,6 +41,8 @@ struct OffTable {
     uint64_t o[64];
 };
 
+template <typename T> using V2Ptr = vec2_t<T> *;
+
 template <typename T>
 struct DiagFullArgs {
     uint64_t len;
@@ -67,7 +69,7 @@ struct SparseArgs {
 // per thread kept in flight.  Consecutive threads take consecutive groups, so
 // a wavefront reads contiguous runs (pair-strided indexing, coalesced for
 // every target).
-template <typename T, int KT, int GPT>
+template <typename T, int KT, int GPT, bool NTM>
 __global__ __launch_bounds__(kBlock) void k_dense(vec2_t<T> *__restrict__ psi,
                                                   const DenseArgs<T, KT> a) {
     constexpr int D = 1 << KT;
@@ -81,7 +83,7 @@ __global__ __launch_bounds__(kBlock) void k_dense(vec2_t<T> *__restrict__ psi,
         base[g] = group_base(a.g, i < a.g.ngroups ? i : 0);
         if (i < a.g.ngroups) {
 #pragma unroll
-            for (int j = 0; j < D; ++j) x[g][j] = psi[base[g] + a.off[j]];
+            for (int j = 0; j < D; ++j) x[g][j] = ldg<NTM>(&psi[base[g] + a.off[j]]);
         }
     }
 #pragma unroll
@@ -102,7 +104,7 @@ __global__ __launch_bounds__(kBlock) void k_dense(vec2_t<T> *__restrict__ psi,
             V y;
             y.x = yr;
             y.y = yi;
-            psi[base[g] + a.off[r]] = y;
+            stg<NTM>(&psi[base[g] + a.off[r]], y);
         }
     }
 }
@@ -158,7 +160,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_big(vec2_t<T> *__restrict__ ps
 
 // Diagonal operator with every entry != 1 and no controls: contiguous sweep,
 // amplitude i is multiplied by d[bits of i at the targets].
-template <typename T, int PPT>
+template <typename T, int PPT, bool NTM>
 __global__ __launch_bounds__(kBlock) void k_diag_full(vec2_t<T> *__restrict__ psi,
                                                       const DiagFullArgs<T> a) {
     using V = vec2_t<T>;
@@ -167,7 +169,7 @@ __global__ __launch_bounds__(kBlock) void k_diag_full(vec2_t<T> *__restrict__ ps
 #pragma unroll
     for (int p = 0; p < PPT; ++p) {
         const uint64_t i = first + (uint64_t)p * kBlock;
-        if (i < a.len) x[p] = psi[i];
+        if (i < a.len) x[p] = ldg<NTM>(&psi[i]);
     }
 #pragma unroll
     for (int p = 0; p < PPT; ++p) {
@@ -179,13 +181,15 @@ __global__ __launch_bounds__(kBlock) void k_diag_full(vec2_t<T> *__restrict__ ps
         V y;
         y.x = x[p].x * mr - x[p].y * mi;
         y.y = x[p].x * mi + x[p].y * mr;
-        psi[i] = y;
+        stg<NTM>(&psi[i], y);
     }
 }
 
 // Sparse diagonal: only the nent listed entries of each group change
 // (controlled phases: CZ, CPHASE, PHASE, CCPHASE ...).
-template <typename T, int GPT>
+// ONE: a single changed entry per group (CZ, CPHASE, PHASE, CCPHASE ...),
+// read from the argument block with static indices.
+template <typename T, int GPT, bool NTM, bool ONE>
 __global__ __launch_bounds__(kBlock) void k_diag_sparse(vec2_t<T> *__restrict__ psi,
                                                         const SparseArgs<T> a) {
     using V = vec2_t<T>;
@@ -195,19 +199,28 @@ __global__ __launch_bounds__(kBlock) void k_diag_sparse(vec2_t<T> *__restrict__ 
         const uint64_t i = first + (uint64_t)g * kBlock;
         if (i >= a.g.ngroups) continue;
         const uint64_t base = group_base(a.g, i);
+        if constexpr (ONE) {
+            const V x = ldg<NTM>(&psi[base + a.off[0]]);
+            V y;
+            y.x = x.x * a.dr[0] - x.y * a.di[0];
+            y.y = x.x * a.di[0] + x.y * a.dr[0];
+            stg<NTM>(&psi[base + a.off[0]], y);
+            continue;
+        }
         for (uint32_t e = 0; e < a.nent; ++e) {
-            const V x = psi[base + a.off[e]];
+            const V x = ldg<NTM>(&psi[base + a.off[e]]);
             V y;
             y.x = x.x * a.dr[e] - x.y * a.di[e];
             y.y = x.x * a.di[e] + x.y * a.dr[e];
-            psi[base + a.off[e]] = y;
+            stg<NTM>(&psi[base + a.off[e]], y);
         }
     }
 }
 
 // Permutation: psi'[dst_e] = psi[src_e] for the moved entries of each group
 // (CNOT, SWAP, CCNOT, CSWAP ...): pure data movement, no arithmetic.
-template <typename T, int GPT>
+// TWO: exactly one transposition per group (CNOT, SWAP, CCNOT, CSWAP).
+template <typename T, int GPT, bool NTM, bool TWO>
 __global__ __launch_bounds__(kBlock) void k_perm(vec2_t<T> *__restrict__ psi,
                                                  const SparseArgs<T> a) {
     using V = vec2_t<T>;
@@ -217,11 +230,11 @@ __global__ __launch_bounds__(kBlock) void k_perm(vec2_t<T> *__restrict__ psi,
         const uint64_t i = first + (uint64_t)g * kBlock;
         if (i >= a.g.ngroups) continue;
         const uint64_t base = group_base(a.g, i);
-        if (a.nent == 2) {  // one transposition (CNOT, SWAP, CCNOT, CSWAP)
-            const V x0 = psi[base + a.src[0]];
-            const V x1 = psi[base + a.src[1]];
-            psi[base + a.off[0]] = x0;
-            psi[base + a.off[1]] = x1;
+        if constexpr (TWO) {
+            const V x0 = ldg<NTM>(&psi[base + a.src[0]]);
+            const V x1 = ldg<NTM>(&psi[base + a.src[1]]);
+            stg<NTM>(&psi[base + a.off[0]], x0);
+            stg<NTM>(&psi[base + a.off[1]], x1);
         } else {
             V tmp[64];
             for (uint32_t e = 0; e < a.nent; ++e) tmp[e] = psi[base + a.src[e]];
@@ -376,10 +389,18 @@ uint32_t grid_for(uint64_t work, uint64_t per_block) {
     return (uint32_t)std::max<uint64_t>(b, 1);
 }
 
+// Non-temporal access pays off when every contiguous run of the gate's
+// access pattern spans >= 128 B: the lowest target/control qubit q gives
+// runs of 2^q amplitudes.
+bool long_runs(const GroupSpec &g, size_t amp_bytes) {
+    const uint32_t qmin = g.nsorted ? g.sorted[0] : 63;
+    return qmin >= 16 || ((uint64_t(1) << qmin) * amp_bytes >= 128);
+}
+
 template <typename T, int KT>
 int run_dense_small(qvmhip_state *s, const GateSplit &gs) {
     constexpr int D = 1 << KT;
-    constexpr int GPT = KT == 1 ? 4 : (KT == 2 ? 2 : 1);
+    constexpr int GPT = 1;
     DenseArgs<T, KT> a;
     std::memset(&a, 0, sizeof(a));
     make_group_spec(a.g, gs, s->n);
@@ -388,9 +409,12 @@ int run_dense_small(qvmhip_state *s, const GateSplit &gs) {
         a.ur[i] = (T)gs.V[i].real();
         a.ui[i] = (T)gs.V[i].imag();
     }
+    const dim3 grid(grid_for(a.g.ngroups, (uint64_t)kBlock * GPT));
     prof_begin(s, KT);
-    hipLaunchKernelGGL((k_dense<T, KT, GPT>), dim3(grid_for(a.g.ngroups, (uint64_t)kBlock * GPT)),
-                       dim3(kBlock), 0, s->stream, (vec2_t<T> *)s->amps, a);
+    if (long_runs(a.g, s->amp_bytes))
+        hipLaunchKernelGGL((k_dense<T, KT, GPT, true>), grid, dim3(kBlock), 0, s->stream, (vec2_t<T> *)s->amps, a);
+    else
+        hipLaunchKernelGGL((k_dense<T, KT, GPT, false>), grid, dim3(kBlock), 0, s->stream, (vec2_t<T> *)s->amps, a);
     prof_end(s);
     QH_LAUNCH_CHECK();
     return QVMHIP_OK;
@@ -443,7 +467,7 @@ int run_diag(qvmhip_state *s, const GateSplit &gs) {
         }
         constexpr int PPT = 4;
         prof_begin(s, QVMHIP_K_DIAG_FULL);
-        hipLaunchKernelGGL((k_diag_full<T, PPT>), dim3(grid_for(a.len, (uint64_t)kBlock * PPT)),
+        hipLaunchKernelGGL((k_diag_full<T, PPT, true>), dim3(grid_for(a.len, (uint64_t)kBlock * PPT)),
                            dim3(kBlock), 0, s->stream, (vec2_t<T> *)s->amps, a);
         prof_end(s);
         QH_LAUNCH_CHECK();
@@ -458,10 +482,18 @@ int run_diag(qvmhip_state *s, const GateSplit &gs) {
         a.dr[e] = (T)gs.V[(size_t)ent[e] * Dr + ent[e]].real();
         a.di[e] = (T)gs.V[(size_t)ent[e] * Dr + ent[e]].imag();
     }
-    constexpr int GPT = 2;
+    constexpr int GPT = 1;
+    const dim3 grid(grid_for(a.g.ngroups, (uint64_t)kBlock * GPT));
     prof_begin(s, QVMHIP_K_DIAG_SPARSE);
-    hipLaunchKernelGGL((k_diag_sparse<T, GPT>), dim3(grid_for(a.g.ngroups, (uint64_t)kBlock * GPT)),
-                       dim3(kBlock), 0, s->stream, (vec2_t<T> *)s->amps, a);
+    const bool lr = long_runs(a.g, s->amp_bytes);
+    V2Ptr<T> psi = (vec2_t<T> *)s->amps;
+    if (a.nent == 1) {
+        if (lr) hipLaunchKernelGGL((k_diag_sparse<T, GPT, true, true>), grid, dim3(kBlock), 0, s->stream, psi, a);
+        else hipLaunchKernelGGL((k_diag_sparse<T, GPT, false, true>), grid, dim3(kBlock), 0, s->stream, psi, a);
+    } else {
+        if (lr) hipLaunchKernelGGL((k_diag_sparse<T, GPT, true, false>), grid, dim3(kBlock), 0, s->stream, psi, a);
+        else hipLaunchKernelGGL((k_diag_sparse<T, GPT, false, false>), grid, dim3(kBlock), 0, s->stream, psi, a);
+    }
     prof_end(s);
     QH_LAUNCH_CHECK();
     return QVMHIP_OK;
@@ -480,10 +512,18 @@ int run_perm(qvmhip_state *s, const GateSplit &gs) {
         ++a.nent;
     }
     if (a.nent == 0) return QVMHIP_OK;
-    constexpr int GPT = 2;
+    constexpr int GPT = 1;
+    const dim3 grid(grid_for(a.g.ngroups, (uint64_t)kBlock * GPT));
     prof_begin(s, QVMHIP_K_PERM);
-    hipLaunchKernelGGL((k_perm<T, GPT>), dim3(grid_for(a.g.ngroups, (uint64_t)kBlock * GPT)),
-                       dim3(kBlock), 0, s->stream, (vec2_t<T> *)s->amps, a);
+    const bool lr = long_runs(a.g, s->amp_bytes);
+    V2Ptr<T> psi = (vec2_t<T> *)s->amps;
+    if (a.nent == 2) {
+        if (lr) hipLaunchKernelGGL((k_perm<T, GPT, true, true>), grid, dim3(kBlock), 0, s->stream, psi, a);
+        else hipLaunchKernelGGL((k_perm<T, GPT, false, true>), grid, dim3(kBlock), 0, s->stream, psi, a);
+    } else {
+        if (lr) hipLaunchKernelGGL((k_perm<T, GPT, true, false>), grid, dim3(kBlock), 0, s->stream, psi, a);
+        else hipLaunchKernelGGL((k_perm<T, GPT, false, false>), grid, dim3(kBlock), 0, s->stream, psi, a);
+    }
     prof_end(s);
     QH_LAUNCH_CHECK();
     return QVMHIP_OK;

@@ -48,6 +48,41 @@ __device__ inline uint64_t group_base(const GroupSpec &g, uint64_t i) {
     return i | g.ctrl_mask;
 }
 
+// Streaming loads/stores.  NT = true marks them non-temporal (the state is
+// swept once per gate and never re-read from cache); measured on MI355X this
+// is faster when each contiguous run of the access pattern is >= 128 B, and
+// slower for shorter runs.
+template <typename V> struct NativeVec;
+template <> struct NativeVec<double2> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct NativeVec<float2> { typedef float type __attribute__((ext_vector_type(2))); };
+
+template <bool NT, typename V>
+__device__ inline V ldg(const V *p) {
+    if constexpr (NT) {
+        typedef typename NativeVec<V>::type N;
+        const N v = __builtin_nontemporal_load((const N *)p);
+        V r;
+        r.x = v.x;
+        r.y = v.y;
+        return r;
+    } else {
+        return *p;
+    }
+}
+
+template <bool NT, typename V>
+__device__ inline void stg(V *p, const V &x) {
+    if constexpr (NT) {
+        typedef typename NativeVec<V>::type N;
+        N v;
+        v.x = x.x;
+        v.y = x.y;
+        __builtin_nontemporal_store(v, (N *)p);
+    } else {
+        *p = x;
+    }
+}
+
 // Error reporting: thread-local message + status code.
 void set_error(const std::string &msg);
 int fail(int code, const std::string &msg);
@@ -131,6 +166,7 @@ int launch_fused_dense(qvmhip_state *s, const void *mat_dev, uint32_t k, const u
 int expectation_zz(qvmhip_state *s, const uint32_t *edges, uint32_t nedges, double *out);
 int inner_product(qvmhip_state *a, qvmhip_state *b, double *re, double *im);
 int max_abs_diff(qvmhip_state *a, qvmhip_state *b, double *out);
+int tv_distance(qvmhip_state *a, qvmhip_state *b, double *out);
 int probabilities(qvmhip_state *s, uint64_t start, uint64_t count, double *host_out);
 int convert_f32_to_f64(qvmhip_state *s, uint64_t start, uint64_t count, double *dev_out);
 int convert_f64_to_f32(qvmhip_state *s, uint64_t start, uint64_t count, const double *dev_in);

@@ -158,6 +158,21 @@ __global__ __launch_bounds__(kBlock) void k_maxdiff_partial(const vec2_t<TA> *__
     }
 }
 
+// sum |(|a_i|^2 - |b_i|^2)|: twice the total-variation distance of the two
+// outcome distributions.
+template <typename TA, typename TB>
+__global__ __launch_bounds__(kBlock) void k_tv_partial(const vec2_t<TA> *__restrict__ a,
+                                                       const vec2_t<TB> *__restrict__ b,
+                                                       uint64_t count, double *__restrict__ partial) {
+    __shared__ double sm[kBlock / kWave];
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    double acc = 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += stride)
+        acc += fabs(prob<TA>(a[i]) - prob<TB>(b[i]));
+    const double t = block_sum(acc, sm);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
 __global__ __launch_bounds__(kBlock) void k_final_max(const double *__restrict__ partial, int n,
                                                       double *__restrict__ out) {
     if (threadIdx.x == 0) {
@@ -580,6 +595,26 @@ template <typename TA, typename TB> struct DiffLaunch {
                            (const vec2_t<TA> *)a->amps, (const vec2_t<TB> *)b->amps, a->len, a->red);
     }
 };
+
+template <typename TA, typename TB> struct TvLaunch {
+    static void go(qvmhip_state *a, qvmhip_state *b, uint32_t grid) {
+        hipLaunchKernelGGL((k_tv_partial<TA, TB>), dim3(grid), dim3(kBlock), 0, a->stream,
+                           (const vec2_t<TA> *)a->amps, (const vec2_t<TB> *)b->amps, a->len, a->red);
+    }
+};
+
+int tv_distance(qvmhip_state *a, qvmhip_state *b, double *out) {
+    const uint32_t grid = red_grid(a->len);
+    dispatch2<TvLaunch>(a, b, grid);
+    QH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(kBlock), 0, a->stream, a->red, (int)grid, 1,
+                       a->red + kRedBlocks * 2);
+    QH_LAUNCH_CHECK();
+    QH_CHECK(hipMemcpyAsync(a->host, a->red + kRedBlocks * 2, sizeof(double), hipMemcpyDeviceToHost, a->stream));
+    QH_CHECK(hipStreamSynchronize(a->stream));
+    *out = 0.5 * a->host[0];
+    return QVMHIP_OK;
+}
 
 int max_abs_diff(qvmhip_state *a, qvmhip_state *b, double *out) {
     const uint32_t grid = red_grid(a->len);

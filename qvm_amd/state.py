@@ -194,6 +194,12 @@ class HipPureState:
         N.check(N.lib().qvmhip_max_abs_diff(self._h, other._h, C.byref(out)))
         return out.value
 
+    def tv_distance(self, other: "HipPureState") -> float:
+        """Total-variation distance between the two outcome distributions."""
+        out = C.c_double()
+        N.check(N.lib().qvmhip_tv_distance(self._h, other._h, C.byref(out)))
+        return out.value
+
     # -- live kernel timing ----------------------------------------------
     def profile(self, on: bool = True) -> None:
         N.check(N.lib().qvmhip_profile_enable(self._h, 1 if on else 0))
