@@ -209,6 +209,7 @@ def run_paths(args, n, ga, gb, nat_a, upd_a, upd_b, u, st, HipPureState, P, np):
         if state.dtype == "f32":
             state.normalize(nrm)  # FP64-accumulated renormalisation before sampling
         pa = state.sample(u)
+        tv = state.tv_distance(ref_a)  # TV of the outcome distribution |psi|^2
         tb = timed_program(state, plan_b)
         db = state.max_abs_diff(ref_b)
         tot = ta + tb
@@ -217,8 +218,7 @@ def run_paths(args, n, ga, gb, nat_a, upd_a, upd_b, u, st, HipPureState, P, np):
              "max_abs_diff_vs_plain_f64": max(da, db),
              "infidelity_vs_plain_f64": max(0.0, 1.0 - abs(ip) ** 2),
              "shots_differing_vs_f64": int(np.count_nonzero(pa != p64))}
-        # TV distance of the sampled distribution |psi|^2 against FP64
-        r["tv_distance_vs_plain_f64"] = state.tv_distance(ref_a)
+        r["tv_distance_vs_plain_f64"] = tv
         if state.dtype == "f32":
             r["norm_before_renormalisation"] = nrm
             # same uniforms: shots land on the same or a neighbouring index

@@ -235,6 +235,27 @@ QVMHIP_API int qvmhip_profile_summary(qvmhip_state *s, int kind, uint64_t *count
  * NULL) with HIP events on the handle's stream; *ms = mean per launch. */
 QVMHIP_API int qvmhip_time_program(qvmhip_program *p, int reps, double *ms);
 
+/* --- fusion cost model (no counterpart in the reference) ----------------
+ * A small MLP (din -> hidden -> hidden -> 1, ReLU, FP32) that predicts the
+ * log launch time of a planned launch from its features; trained and run by
+ * the engine's own kernels on timings the engine records.  The planner uses it
+ * to choose fusion widths and tile passes.  Parameters are one flat vector:
+ * W1[hidden][din], b1, W2[hidden][hidden], b2, w3[hidden], b3. */
+typedef struct qvmhip_mlp qvmhip_mlp;
+QVMHIP_API int qvmhip_mlp_create(int din, int hidden, const float *init_params, qvmhip_mlp **out);
+QVMHIP_API int qvmhip_mlp_destroy(qvmhip_mlp *m);
+QVMHIP_API int qvmhip_mlp_num_params(const qvmhip_mlp *m, int *np);
+QVMHIP_API int qvmhip_mlp_get_params(qvmhip_mlp *m, float *out);
+QVMHIP_API int qvmhip_mlp_set_params(qvmhip_mlp *m, const float *in);
+/* out[b] = f(X[b, :]) for b < nb (X row-major, nb x din). */
+QVMHIP_API int qvmhip_mlp_predict(qvmhip_mlp *m, const float *X, int nb, float *out);
+/* Gradient of mean_b 0.5 (f(X_b) - T_b)^2 with respect to the parameters. */
+QVMHIP_API int qvmhip_mlp_gradient(qvmhip_mlp *m, const float *X, const float *T, int nb, float *grad,
+                                   float *loss);
+/* `epochs` full-batch Adam steps on the same loss. */
+QVMHIP_API int qvmhip_mlp_train(qvmhip_mlp *m, const float *X, const float *T, int nb, int epochs, float lr,
+                                float *final_loss);
+
 #ifdef __cplusplus
 }
 #endif

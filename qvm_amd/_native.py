@@ -59,6 +59,7 @@ _u32p = C.POINTER(C.c_uint32)
 _u64p = C.POINTER(C.c_uint64)
 _vp = C.c_void_p
 _ip = C.POINTER(C.c_int)
+_fp = C.POINTER(C.c_float)
 
 # name -> argtypes (all return int unless listed in _RESTYPE)
 SIGNATURES = {
@@ -105,6 +106,14 @@ SIGNATURES = {
     "qvmhip_tv_distance": [_vp, _vp, _dp],
     "qvmhip_profile_enable": [_vp, C.c_int],
     "qvmhip_profile_summary": [_vp, C.c_int, _u64p, _dp],
+    "qvmhip_mlp_create": [C.c_int, C.c_int, _fp, C.POINTER(_vp)],
+    "qvmhip_mlp_destroy": [_vp],
+    "qvmhip_mlp_num_params": [_vp, _ip],
+    "qvmhip_mlp_get_params": [_vp, _fp],
+    "qvmhip_mlp_set_params": [_vp, _fp],
+    "qvmhip_mlp_predict": [_vp, _fp, C.c_int, _fp],
+    "qvmhip_mlp_gradient": [_vp, _fp, _fp, C.c_int, _fp, _fp],
+    "qvmhip_mlp_train": [_vp, _fp, _fp, C.c_int, C.c_int, C.c_float, _fp],
 }
 
 # kernel families of qvmhip_profile_summary
@@ -164,6 +173,11 @@ def u32ptr(a: np.ndarray):
 
 def u64ptr(a: np.ndarray):
     return a.ctypes.data_as(_u64p)
+
+
+def fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_fp)
 
 
 def cmat(U) -> np.ndarray:

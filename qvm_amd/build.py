@@ -29,6 +29,7 @@ SOURCES = {
     "reduce.hip": ["-ffp-contract=off"],
     "fused.hip": [],
     "program.hip": [],
+    "mlp.hip": [],
 }
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

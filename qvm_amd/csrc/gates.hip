@@ -74,37 +74,40 @@ __global__ __launch_bounds__(kBlock) void k_dense(vec2_t<T> *__restrict__ psi,
                                                   const DenseArgs<T, KT> a) {
     constexpr int D = 1 << KT;
     using V = vec2_t<T>;
-    const uint64_t first = (uint64_t)blockIdx.x * (kBlock * GPT) + threadIdx.x;
-    V x[GPT][D];
-    uint64_t base[GPT];
+    const uint64_t step = (uint64_t)gridDim.x * (kBlock * GPT);
+    for (uint64_t first = (uint64_t)blockIdx.x * (kBlock * GPT) + threadIdx.x; first < a.g.ngroups;
+         first += step) {
+        V x[GPT][D];
+        uint64_t base[GPT];
 #pragma unroll
-    for (int g = 0; g < GPT; ++g) {
-        const uint64_t i = first + (uint64_t)g * kBlock;
-        base[g] = group_base(a.g, i < a.g.ngroups ? i : 0);
-        if (i < a.g.ngroups) {
+        for (int g = 0; g < GPT; ++g) {
+            const uint64_t i = first + (uint64_t)g * kBlock;
+            base[g] = group_base(a.g, i < a.g.ngroups ? i : 0);
+            if (i < a.g.ngroups) {
 #pragma unroll
-            for (int j = 0; j < D; ++j) x[g][j] = ldg<NTM>(&psi[base[g] + a.off[j]]);
-        }
-    }
-#pragma unroll
-    for (int g = 0; g < GPT; ++g) {
-        const uint64_t i = first + (uint64_t)g * kBlock;
-        if (i >= a.g.ngroups) continue;
-#pragma unroll
-        for (int r = 0; r < D; ++r) {
-            T yr = x[g][0].x * a.ur[r * D] - x[g][0].y * a.ui[r * D];
-            T yi = x[g][0].x * a.ui[r * D] + x[g][0].y * a.ur[r * D];
-#pragma unroll
-            for (int c = 1; c < D; ++c) {
-                const T pr = x[g][c].x * a.ur[r * D + c] - x[g][c].y * a.ui[r * D + c];
-                const T pi = x[g][c].x * a.ui[r * D + c] + x[g][c].y * a.ur[r * D + c];
-                yr = yr + pr;
-                yi = yi + pi;
+                for (int j = 0; j < D; ++j) x[g][j] = ldg<NTM>(&psi[base[g] + a.off[j]]);
             }
-            V y;
-            y.x = yr;
-            y.y = yi;
-            stg<NTM>(&psi[base[g] + a.off[r]], y);
+        }
+#pragma unroll
+        for (int g = 0; g < GPT; ++g) {
+            const uint64_t i = first + (uint64_t)g * kBlock;
+            if (i >= a.g.ngroups) continue;
+#pragma unroll
+            for (int r = 0; r < D; ++r) {
+                T yr = x[g][0].x * a.ur[r * D] - x[g][0].y * a.ui[r * D];
+                T yi = x[g][0].x * a.ui[r * D] + x[g][0].y * a.ur[r * D];
+#pragma unroll
+                for (int c = 1; c < D; ++c) {
+                    const T pr = x[g][c].x * a.ur[r * D + c] - x[g][c].y * a.ui[r * D + c];
+                    const T pi = x[g][c].x * a.ui[r * D + c] + x[g][c].y * a.ur[r * D + c];
+                    yr = yr + pr;
+                    yi = yi + pi;
+                }
+                V y;
+                y.x = yr;
+                y.y = yi;
+                stg<NTM>(&psi[base[g] + a.off[r]], y);
+            }
         }
     }
 }
@@ -164,24 +167,26 @@ template <typename T, int PPT, bool NTM>
 __global__ __launch_bounds__(kBlock) void k_diag_full(vec2_t<T> *__restrict__ psi,
                                                       const DiagFullArgs<T> a) {
     using V = vec2_t<T>;
-    const uint64_t first = (uint64_t)blockIdx.x * (kBlock * PPT) + threadIdx.x;
-    V x[PPT];
+    const uint64_t step = (uint64_t)gridDim.x * (kBlock * PPT);
+    for (uint64_t first = (uint64_t)blockIdx.x * (kBlock * PPT) + threadIdx.x; first < a.len; first += step) {
+        V x[PPT];
 #pragma unroll
-    for (int p = 0; p < PPT; ++p) {
-        const uint64_t i = first + (uint64_t)p * kBlock;
-        if (i < a.len) x[p] = ldg<NTM>(&psi[i]);
-    }
+        for (int p = 0; p < PPT; ++p) {
+            const uint64_t i = first + (uint64_t)p * kBlock;
+            if (i < a.len) x[p] = ldg<NTM>(&psi[i]);
+        }
 #pragma unroll
-    for (int p = 0; p < PPT; ++p) {
-        const uint64_t i = first + (uint64_t)p * kBlock;
-        if (i >= a.len) continue;
-        uint32_t j = 0;
-        for (uint32_t t = 0; t < a.kt; ++t) j |= (uint32_t)((i >> a.tq[t]) & 1u) << t;
-        const T mr = a.dr[j], mi = a.di[j];
-        V y;
-        y.x = x[p].x * mr - x[p].y * mi;
-        y.y = x[p].x * mi + x[p].y * mr;
-        stg<NTM>(&psi[i], y);
+        for (int p = 0; p < PPT; ++p) {
+            const uint64_t i = first + (uint64_t)p * kBlock;
+            if (i >= a.len) continue;
+            uint32_t j = 0;
+            for (uint32_t t = 0; t < a.kt; ++t) j |= (uint32_t)((i >> a.tq[t]) & 1u) << t;
+            const T mr = a.dr[j], mi = a.di[j];
+            V y;
+            y.x = x[p].x * mr - x[p].y * mi;
+            y.y = x[p].x * mi + x[p].y * mr;
+            stg<NTM>(&psi[i], y);
+        }
     }
 }
 
@@ -193,26 +198,29 @@ template <typename T, int GPT, bool NTM, bool ONE>
 __global__ __launch_bounds__(kBlock) void k_diag_sparse(vec2_t<T> *__restrict__ psi,
                                                         const SparseArgs<T> a) {
     using V = vec2_t<T>;
-    const uint64_t first = (uint64_t)blockIdx.x * (kBlock * GPT) + threadIdx.x;
+    const uint64_t step = (uint64_t)gridDim.x * (kBlock * GPT);
+    for (uint64_t first = (uint64_t)blockIdx.x * (kBlock * GPT) + threadIdx.x; first < a.g.ngroups;
+         first += step) {
 #pragma unroll
-    for (int g = 0; g < GPT; ++g) {
-        const uint64_t i = first + (uint64_t)g * kBlock;
-        if (i >= a.g.ngroups) continue;
-        const uint64_t base = group_base(a.g, i);
-        if constexpr (ONE) {
-            const V x = ldg<NTM>(&psi[base + a.off[0]]);
-            V y;
-            y.x = x.x * a.dr[0] - x.y * a.di[0];
-            y.y = x.x * a.di[0] + x.y * a.dr[0];
-            stg<NTM>(&psi[base + a.off[0]], y);
-            continue;
-        }
-        for (uint32_t e = 0; e < a.nent; ++e) {
-            const V x = ldg<NTM>(&psi[base + a.off[e]]);
-            V y;
-            y.x = x.x * a.dr[e] - x.y * a.di[e];
-            y.y = x.x * a.di[e] + x.y * a.dr[e];
-            stg<NTM>(&psi[base + a.off[e]], y);
+        for (int g = 0; g < GPT; ++g) {
+            const uint64_t i = first + (uint64_t)g * kBlock;
+            if (i >= a.g.ngroups) continue;
+            const uint64_t base = group_base(a.g, i);
+            if constexpr (ONE) {
+                const V x = ldg<NTM>(&psi[base + a.off[0]]);
+                V y;
+                y.x = x.x * a.dr[0] - x.y * a.di[0];
+                y.y = x.x * a.di[0] + x.y * a.dr[0];
+                stg<NTM>(&psi[base + a.off[0]], y);
+                continue;
+            }
+            for (uint32_t e = 0; e < a.nent; ++e) {
+                const V x = ldg<NTM>(&psi[base + a.off[e]]);
+                V y;
+                y.x = x.x * a.dr[e] - x.y * a.di[e];
+                y.y = x.x * a.di[e] + x.y * a.dr[e];
+                stg<NTM>(&psi[base + a.off[e]], y);
+            }
         }
     }
 }
@@ -224,21 +232,24 @@ template <typename T, int GPT, bool NTM, bool TWO>
 __global__ __launch_bounds__(kBlock) void k_perm(vec2_t<T> *__restrict__ psi,
                                                  const SparseArgs<T> a) {
     using V = vec2_t<T>;
-    const uint64_t first = (uint64_t)blockIdx.x * (kBlock * GPT) + threadIdx.x;
+    const uint64_t step = (uint64_t)gridDim.x * (kBlock * GPT);
+    for (uint64_t first = (uint64_t)blockIdx.x * (kBlock * GPT) + threadIdx.x; first < a.g.ngroups;
+         first += step) {
 #pragma unroll
-    for (int g = 0; g < GPT; ++g) {
-        const uint64_t i = first + (uint64_t)g * kBlock;
-        if (i >= a.g.ngroups) continue;
-        const uint64_t base = group_base(a.g, i);
-        if constexpr (TWO) {
-            const V x0 = ldg<NTM>(&psi[base + a.src[0]]);
-            const V x1 = ldg<NTM>(&psi[base + a.src[1]]);
-            stg<NTM>(&psi[base + a.off[0]], x0);
-            stg<NTM>(&psi[base + a.off[1]], x1);
-        } else {
-            V tmp[64];
-            for (uint32_t e = 0; e < a.nent; ++e) tmp[e] = psi[base + a.src[e]];
-            for (uint32_t e = 0; e < a.nent; ++e) psi[base + a.off[e]] = tmp[e];
+        for (int g = 0; g < GPT; ++g) {
+            const uint64_t i = first + (uint64_t)g * kBlock;
+            if (i >= a.g.ngroups) continue;
+            const uint64_t base = group_base(a.g, i);
+            if constexpr (TWO) {
+                const V x0 = ldg<NTM>(&psi[base + a.src[0]]);
+                const V x1 = ldg<NTM>(&psi[base + a.src[1]]);
+                stg<NTM>(&psi[base + a.off[0]], x0);
+                stg<NTM>(&psi[base + a.off[1]], x1);
+            } else {
+                V tmp[64];
+                for (uint32_t e = 0; e < a.nent; ++e) tmp[e] = psi[base + a.src[e]];
+                for (uint32_t e = 0; e < a.nent; ++e) psi[base + a.off[e]] = tmp[e];
+            }
         }
     }
 }
@@ -386,7 +397,7 @@ uint64_t deposit(uint32_t j, const uint32_t *tq, uint32_t kt) {
 
 uint32_t grid_for(uint64_t work, uint64_t per_block) {
     uint64_t b = (work + per_block - 1) / per_block;
-    return (uint32_t)std::max<uint64_t>(b, 1);
+    return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(b, 1), kMaxGrid);
 }
 
 // Non-temporal access pays off when every contiguous run of the gate's

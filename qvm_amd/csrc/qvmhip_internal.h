@@ -21,6 +21,9 @@ constexpr int kMaxK = 6;            // widest dense block (64 x 64)
 constexpr int kMaxSorted = 16;      // targets + controls of one gate
 constexpr int kRedBlocks = 2048;    // partial sums of one reduction launch
 constexpr int kChunkLog2 = 8;       // sampling chunk: 256 amplitudes
+// A dispatch holds at most 2^32 work-items; sweep kernels cap their grid at
+// 2^22 workgroups and grid-stride over the rest (34-qubit states).
+constexpr uint64_t kMaxGrid = uint64_t(1) << 22;
 
 template <typename T> struct Vec2;
 template <> struct Vec2<double> { using type = double2; };

@@ -241,13 +241,15 @@ __global__ __launch_bounds__(kBlock) void k_chunk_sums(const vec2_t<T> *__restri
                                                        uint64_t len, uint64_t nchunks,
                                                        double *__restrict__ sums) {
     const int lane = threadIdx.x % kWave;
-    const uint64_t chunk = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
-    if (chunk >= nchunks) return;
-    double p[4];
-    chunk_lane_probs<T>(psi, len, chunk, lane, p);
-    const double local = ((p[0] + p[1]) + p[2]) + p[3];
-    const double incl = wave_inclusive_scan(local, lane);
-    if (lane == kWave - 1) sums[chunk] = incl;
+    const uint64_t step = (uint64_t)gridDim.x * (kBlock / kWave);
+    for (uint64_t chunk = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; chunk < nchunks;
+         chunk += step) {
+        double p[4];
+        chunk_lane_probs<T>(psi, len, chunk, lane, p);
+        const double local = ((p[0] + p[1]) + p[2]) + p[3];
+        const double incl = wave_inclusive_scan(local, lane);
+        if (lane == kWave - 1) sums[chunk] = incl;
+    }
 }
 
 // Device-wide exclusive scan of doubles, three phases.  Phase 1: totals of
@@ -465,7 +467,7 @@ int launch_force_measure(qvmhip_state *s, uint32_t q, int bit, double inv_norm, 
 // Builds chunk sums + exclusive chunk prefix in s->chunk_sum / s->chunk_pre.
 static int build_chunk_prefix(qvmhip_state *s) {
     const uint64_t nch = s->nchunks;
-    const uint32_t g1 = (uint32_t)((nch + (kBlock / kWave) - 1) / (kBlock / kWave));
+    const uint32_t g1 = (uint32_t)std::min<uint64_t>((nch + (kBlock / kWave) - 1) / (kBlock / kWave), kMaxGrid);
     if (s->dtype == QVMHIP_F64)
         hipLaunchKernelGGL(k_chunk_sums<double>, dim3(g1), dim3(kBlock), 0, s->stream,
                            (const double2 *)s->amps, s->len, nch, s->chunk_sum);

@@ -56,128 +56,132 @@ __global__ __launch_bounds__(NT) void k_tile(vec2_t<T> *__restrict__ psi, const 
     constexpr int R = S / NT;  // amplitudes per thread
     __shared__ V sa[S];
     const uint32_t t = threadIdx.x;
-    const uint64_t base = group_base(a.g, blockIdx.x) + thread_offset<NT>(a, t);
+    const uint64_t toff = thread_offset<NT>(a, t);
+    for (uint64_t tile = blockIdx.x; tile < a.g.ngroups; tile += gridDim.x) {
+        const uint64_t base = group_base(a.g, tile) + toff;
 
-    // load: element l = t + r*NT
-    {
-        V x[R];
+        // load: element l = t + r*NT (only this thread reads these slots back in
+        // the previous tile's store phase, so no barrier is needed before it)
+        {
+            V x[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) x[r] = psi[base + a.roff[r]];
+            for (int r = 0; r < R; ++r) x[r] = psi[base + a.roff[r]];
 #pragma unroll
-        for (int r = 0; r < R; ++r) sa[t + r * NT] = x[r];
-    }
-    __syncthreads();
-
-    for (uint32_t oi = 0; oi < nops; ++oi) {
-        const qvmhip_tile_op &op = ops[oi];
-        const uint32_t kind = op.kind;
-        const uint32_t q0 = op.q0, q1 = op.q1, cm = op.ctrl_mask;
-        if (kind == QVMHIP_TILE_DENSE1) {
-            const T u00r = (T)op.m[0], u00i = (T)op.m[1], u01r = (T)op.m[2], u01i = (T)op.m[3];
-            const T u10r = (T)op.m[4], u10i = (T)op.m[5], u11r = (T)op.m[6], u11i = (T)op.m[7];
-#pragma unroll
-            for (uint32_t p = t; p < S / 2; p += NT) {
-                const uint32_t ia = ins0(p, q0), ib = ia | (1u << q0);
-                if ((ia & cm) != cm) continue;
-                const V xa = sa[ia], xb = sa[ib];
-                T p0r, p0i, p1r, p1i;
-                V ya, yb;
-                cmul(xa.x, xa.y, u00r, u00i, p0r, p0i);
-                cmul(xb.x, xb.y, u01r, u01i, p1r, p1i);
-                ya.x = p0r + p1r;
-                ya.y = p0i + p1i;
-                cmul(xa.x, xa.y, u10r, u10i, p0r, p0i);
-                cmul(xb.x, xb.y, u11r, u11i, p1r, p1i);
-                yb.x = p0r + p1r;
-                yb.y = p0i + p1i;
-                sa[ia] = ya;
-                sa[ib] = yb;
-            }
-        } else if (kind == QVMHIP_TILE_DENSE2) {
-            const uint32_t lo = q0 < q1 ? q0 : q1, hi = q0 < q1 ? q1 : q0;
-#pragma unroll
-            for (uint32_t p = t; p < S / 4; p += NT) {
-                const uint32_t i0 = ins0(ins0(p, lo), hi);
-                if ((i0 & cm) != cm) continue;
-                const uint32_t idx[4] = {i0, i0 | (1u << q0), i0 | (1u << q1), i0 | (1u << q0) | (1u << q1)};
-                V x[4];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) x[j] = sa[idx[j]];
-#pragma unroll
-                for (int row = 0; row < 4; ++row) {
-                    T yr, yi;
-                    cmul(x[0].x, x[0].y, (T)op.m[8 * row], (T)op.m[8 * row + 1], yr, yi);
-#pragma unroll
-                    for (int c = 1; c < 4; ++c) {
-                        T pr, pi;
-                        cmul(x[c].x, x[c].y, (T)op.m[8 * row + 2 * c], (T)op.m[8 * row + 2 * c + 1], pr, pi);
-                        yr = yr + pr;
-                        yi = yi + pi;
-                    }
-                    V y;
-                    y.x = yr;
-                    y.y = yi;
-                    sa[idx[row]] = y;
-                }
-            }
-        } else if (kind == QVMHIP_TILE_DIAG1 || kind == QVMHIP_TILE_DIAG2) {
-            const bool two = kind == QVMHIP_TILE_DIAG2;
-#pragma unroll
-            for (uint32_t l = t; l < S; l += NT) {
-                if ((l & cm) != cm) continue;
-                uint32_t j = (l >> q0) & 1u;
-                if (two) j |= ((l >> q1) & 1u) << 1;
-                const V x = sa[l];
-                V y;
-                cmul(x.x, x.y, (T)op.m[2 * j], (T)op.m[2 * j + 1], y.x, y.y);
-                sa[l] = y;
-            }
-        } else if (kind == QVMHIP_TILE_CPERM) {
-            // target q0, control q1: swap (c=1,t=0) <-> (c=1,t=1)
-            const uint32_t lo = q0 < q1 ? q0 : q1, hi = q0 < q1 ? q1 : q0;
-#pragma unroll
-            for (uint32_t p = t; p < S / 4; p += NT) {
-                const uint32_t i0 = ins0(ins0(p, lo), hi) | (1u << q1);
-                if ((i0 & cm) != cm) continue;
-                const uint32_t i1 = i0 | (1u << q0);
-                const V x0 = sa[i0], x1 = sa[i1];
-                sa[i0] = x1;
-                sa[i1] = x0;
-            }
-        } else if (kind == QVMHIP_TILE_SWAP) {
-            const uint32_t lo = q0 < q1 ? q0 : q1, hi = q0 < q1 ? q1 : q0;
-#pragma unroll
-            for (uint32_t p = t; p < S / 4; p += NT) {
-                const uint32_t i0 = ins0(ins0(p, lo), hi);
-                if ((i0 & cm) != cm) continue;
-                const uint32_t ia = i0 | (1u << q0), ib = i0 | (1u << q1);
-                const V xa = sa[ia], xb = sa[ib];
-                sa[ia] = xb;
-                sa[ib] = xa;
-            }
-        } else if (kind == QVMHIP_TILE_CPHASE) {
-            const uint32_t lo = q0 < q1 ? q0 : q1, hi = q0 < q1 ? q1 : q0;
-            const T mr = (T)op.m[0], mi = (T)op.m[1];
-#pragma unroll
-            for (uint32_t p = t; p < S / 4; p += NT) {
-                const uint32_t i0 = ins0(ins0(p, lo), hi) | (1u << q0) | (1u << q1);
-                if ((i0 & cm) != cm) continue;
-                const V x = sa[i0];
-                V y;
-                cmul(x.x, x.y, mr, mi, y.x, y.y);
-                sa[i0] = y;
-            }
+            for (int r = 0; r < R; ++r) sa[t + r * NT] = x[r];
         }
         __syncthreads();
-    }
 
-    // store
-    {
-        V x[R];
+        for (uint32_t oi = 0; oi < nops; ++oi) {
+            const qvmhip_tile_op &op = ops[oi];
+            const uint32_t kind = op.kind;
+            const uint32_t q0 = op.q0, q1 = op.q1, cm = op.ctrl_mask;
+            if (kind == QVMHIP_TILE_DENSE1) {
+                const T u00r = (T)op.m[0], u00i = (T)op.m[1], u01r = (T)op.m[2], u01i = (T)op.m[3];
+                const T u10r = (T)op.m[4], u10i = (T)op.m[5], u11r = (T)op.m[6], u11i = (T)op.m[7];
 #pragma unroll
-        for (int r = 0; r < R; ++r) x[r] = sa[t + r * NT];
+                for (uint32_t p = t; p < S / 2; p += NT) {
+                    const uint32_t ia = ins0(p, q0), ib = ia | (1u << q0);
+                    if ((ia & cm) != cm) continue;
+                    const V xa = sa[ia], xb = sa[ib];
+                    T p0r, p0i, p1r, p1i;
+                    V ya, yb;
+                    cmul(xa.x, xa.y, u00r, u00i, p0r, p0i);
+                    cmul(xb.x, xb.y, u01r, u01i, p1r, p1i);
+                    ya.x = p0r + p1r;
+                    ya.y = p0i + p1i;
+                    cmul(xa.x, xa.y, u10r, u10i, p0r, p0i);
+                    cmul(xb.x, xb.y, u11r, u11i, p1r, p1i);
+                    yb.x = p0r + p1r;
+                    yb.y = p0i + p1i;
+                    sa[ia] = ya;
+                    sa[ib] = yb;
+                }
+            } else if (kind == QVMHIP_TILE_DENSE2) {
+                const uint32_t lo = q0 < q1 ? q0 : q1, hi = q0 < q1 ? q1 : q0;
 #pragma unroll
-        for (int r = 0; r < R; ++r) psi[base + a.roff[r]] = x[r];
+                for (uint32_t p = t; p < S / 4; p += NT) {
+                    const uint32_t i0 = ins0(ins0(p, lo), hi);
+                    if ((i0 & cm) != cm) continue;
+                    const uint32_t idx[4] = {i0, i0 | (1u << q0), i0 | (1u << q1), i0 | (1u << q0) | (1u << q1)};
+                    V x[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) x[j] = sa[idx[j]];
+#pragma unroll
+                    for (int row = 0; row < 4; ++row) {
+                        T yr, yi;
+                        cmul(x[0].x, x[0].y, (T)op.m[8 * row], (T)op.m[8 * row + 1], yr, yi);
+#pragma unroll
+                        for (int c = 1; c < 4; ++c) {
+                            T pr, pi;
+                            cmul(x[c].x, x[c].y, (T)op.m[8 * row + 2 * c], (T)op.m[8 * row + 2 * c + 1], pr, pi);
+                            yr = yr + pr;
+                            yi = yi + pi;
+                        }
+                        V y;
+                        y.x = yr;
+                        y.y = yi;
+                        sa[idx[row]] = y;
+                    }
+                }
+            } else if (kind == QVMHIP_TILE_DIAG1 || kind == QVMHIP_TILE_DIAG2) {
+                const bool two = kind == QVMHIP_TILE_DIAG2;
+#pragma unroll
+                for (uint32_t l = t; l < S; l += NT) {
+                    if ((l & cm) != cm) continue;
+                    uint32_t j = (l >> q0) & 1u;
+                    if (two) j |= ((l >> q1) & 1u) << 1;
+                    const V x = sa[l];
+                    V y;
+                    cmul(x.x, x.y, (T)op.m[2 * j], (T)op.m[2 * j + 1], y.x, y.y);
+                    sa[l] = y;
+                }
+            } else if (kind == QVMHIP_TILE_CPERM) {
+                // target q0, control q1: swap (c=1,t=0) <-> (c=1,t=1)
+                const uint32_t lo = q0 < q1 ? q0 : q1, hi = q0 < q1 ? q1 : q0;
+#pragma unroll
+                for (uint32_t p = t; p < S / 4; p += NT) {
+                    const uint32_t i0 = ins0(ins0(p, lo), hi) | (1u << q1);
+                    if ((i0 & cm) != cm) continue;
+                    const uint32_t i1 = i0 | (1u << q0);
+                    const V x0 = sa[i0], x1 = sa[i1];
+                    sa[i0] = x1;
+                    sa[i1] = x0;
+                }
+            } else if (kind == QVMHIP_TILE_SWAP) {
+                const uint32_t lo = q0 < q1 ? q0 : q1, hi = q0 < q1 ? q1 : q0;
+#pragma unroll
+                for (uint32_t p = t; p < S / 4; p += NT) {
+                    const uint32_t i0 = ins0(ins0(p, lo), hi);
+                    if ((i0 & cm) != cm) continue;
+                    const uint32_t ia = i0 | (1u << q0), ib = i0 | (1u << q1);
+                    const V xa = sa[ia], xb = sa[ib];
+                    sa[ia] = xb;
+                    sa[ib] = xa;
+                }
+            } else if (kind == QVMHIP_TILE_CPHASE) {
+                const uint32_t lo = q0 < q1 ? q0 : q1, hi = q0 < q1 ? q1 : q0;
+                const T mr = (T)op.m[0], mi = (T)op.m[1];
+#pragma unroll
+                for (uint32_t p = t; p < S / 4; p += NT) {
+                    const uint32_t i0 = ins0(ins0(p, lo), hi) | (1u << q0) | (1u << q1);
+                    if ((i0 & cm) != cm) continue;
+                    const V x = sa[i0];
+                    V y;
+                    cmul(x.x, x.y, mr, mi, y.x, y.y);
+                    sa[i0] = y;
+                }
+            }
+            __syncthreads();
+        }
+
+        // store
+        {
+            V x[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) x[r] = sa[t + r * NT];
+#pragma unroll
+            for (int r = 0; r < R; ++r) psi[base + a.roff[r]] = x[r];
+        }
     }
 }
 
@@ -207,7 +211,8 @@ int run_tile(qvmhip_state *s, const uint32_t *tq, uint32_t m, const qvmhip_tile_
         a.roff[r] = o;
     }
     prof_begin(s, QVMHIP_K_TILE);
-    hipLaunchKernelGGL((k_tile<T, M, NT>), dim3((uint32_t)a.g.ngroups), dim3(NT), 0, s->stream,
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(a.g.ngroups, kMaxGrid);
+    hipLaunchKernelGGL((k_tile<T, M, NT>), dim3(grid), dim3(NT), 0, s->stream,
                        (vec2_t<T> *)s->amps, a, ops_dev, nops);
     prof_end(s);
     QH_LAUNCH_CHECK();
