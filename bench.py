@@ -120,7 +120,8 @@ def main() -> int:
     per_launch_ms = dms / dcount
     unit_bytes = 2 * (1 << n) * 16  # dense 1q / diagonal sweep: read + write every amplitude
     roofline = {
-        "kernel": {"dense1": "qvmhip::k_dense<double, 1, 1, true|false>"}.get(dom, dom),
+        "kernel": {"dense1": "qvmhip::k_dense<double, 1, 1, NT>",
+                   "diag_full": "qvmhip::k_diag_full<double, 4, true>"}.get(dom, dom),
         "bound": "hbm",
         "achieved": round(unit_bytes / (per_launch_ms * 1e-3) / 1e9, 1),
         "peak": 8000.0,
@@ -131,7 +132,7 @@ def main() -> int:
         "avg_launch_ms": round(per_launch_ms, 4),
         "algorithmic_bytes_per_launch": unit_bytes,
     }
-    pmc = os.path.join(REPO, "profiles", "pmc_dense1_n30.json")
+    pmc = os.path.join(REPO, "profiles", f"pmc_{dom}_n30.json")
     if os.path.exists(pmc) and n == 30:
         with open(pmc) as f:
             roofline["traffic"] = json.load(f).get("hbm_bytes_per_launch")
