@@ -51,8 +51,10 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--workload", default="circuits", choices=["circuits", "precision"],
-                    help="circuits: the headline line (default); precision: FP32 vs FP64 report")
+    ap.add_argument("--workload", default="circuits", choices=["circuits", "precision", "batch"],
+                    help="circuits: the headline line (default); precision: FP32 vs FP64 report; "
+                         "batch: 512 QAOA instances in one graph")
+    ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--big-n", type=int, default=34, help="FP32-only size of the precision report")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -62,6 +64,8 @@ def main() -> int:
 
     if args.workload == "precision":
         return precision(args)
+    if args.workload == "batch":
+        return batch(args)
 
     import numpy as np
 
@@ -313,6 +317,74 @@ def precision(args) -> int:
                                                    "norm": nrm, "state_GiB": (1 << nb) * 8 / 2 ** 30}
             log(f"precision: f32 n={nb} {mode} {dt:.2f} s norm {nrm:.9f}")
         big.close()
+    print(json.dumps(out))
+    return 0
+
+
+def batch(args) -> int:
+    """Config 4: QAOA MaxCut instances (n even in [10, 24], random 3-regular
+    graphs, p in 1..4), <H_C> + 1024 shots each, FP64 and FP32, every run one
+    hipGraph launch."""
+    import numpy as np
+
+    from qvm_amd import circuits as C
+    from qvm_amd import gates as G
+    from qvm_amd.batch import qaoa_batch
+
+    inst = C.qaoa_instances(args.instances, seed=4)
+    circuits = [C.qaoa_maxcut(n, e, g, b) for n, e, g, b in inst]
+    upd = sum(C.amp_updates(c, n) for c, (n, _, _, _) in zip(circuits, inst))
+    out = {"metric": "QAOA batch: instances/s and amp-updates/s, one hipGraph launch per run",
+           "instances": len(inst), "gates": sum(len(c) for c in circuits), "amp_updates_per_run": upd,
+           "shots_per_instance": 1024, "results": {}}
+    hcs = {}
+    for dtype in ("f64", "f32"):
+        t0 = time.perf_counter()
+        b, _, _ = qaoa_batch(inst, dtype, shots=1024, seed=9, use_graph=True)
+        t_plan = time.perf_counter() - t0
+        b.run()
+        b.sync()
+        t0 = time.perf_counter()
+        reps = max(1, args.steps)
+        for _ in range(reps):
+            b.run()
+        b.sync()
+        dt = (time.perf_counter() - t0) / reps
+        hc, shots = b.results()
+        hcs[dtype] = hc
+        out["results"][dtype] = {"s_per_run": dt, "instances_per_s": len(inst) / dt, "amp_updates_per_s": upd / dt,
+                                 "plan_and_capture_s": t_plan, "launch_items": b.items,
+                                 "state_GiB": sum((1 << n) for n, _, _, _ in inst) * (16 if dtype == "f64" else 8) / 2 ** 30,
+                                 "mean_expected_cut": float(np.mean([0.5 * (len(e) - h) for (n, e, _, _), h in zip(inst, hc)]))}
+        log(f"batch {dtype}: {dt * 1e3:.1f} ms/run, {len(inst) / dt:.0f} instances/s")
+        b.close()
+    out["results"]["f32_max_abs_hc_diff_vs_f64"] = float(np.max(np.abs(hcs["f32"] - hcs["f64"])))
+    # eager (uncaptured) FP64 run for the graph's launch-overhead saving
+    b, _, _ = qaoa_batch(inst, "f64", shots=1024, seed=9, use_graph=False)
+    b.run()
+    b.sync()
+    t0 = time.perf_counter()
+    b.run()
+    b.sync()
+    out["results"]["f64_eager_s_per_run"] = time.perf_counter() - t0
+    b.close()
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        O.set_parallelization(19, args.cpu_threads)
+        t0 = time.perf_counter()
+        done, cu = 0, 0
+        for c, (n, _, _, _) in zip(circuits, inst):
+            psi = O.zero_state(n)
+            for U, q in c:
+                O.apply_compiled(psi, U, G.nat_tuple(*q))
+            done += 1
+            cu += C.amp_updates(c, n)
+            if time.perf_counter() - t0 > args.cpu_seconds:
+                break
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cu / dt, "unit": "amp-updates/s", "cores": args.cpu_threads, "kind": "port",
+                               "instances_per_s": done / dt,
+                               "sample": f"first {done} instances in generation order, FP64 compiled mode, {dt:.1f} s"}
     print(json.dumps(out))
     return 0
 

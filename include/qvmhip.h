@@ -235,6 +235,33 @@ QVMHIP_API int qvmhip_profile_summary(qvmhip_state *s, int kind, uint64_t *count
  * NULL) with HIP events on the handle's stream; *ms = mean per launch. */
 QVMHIP_API int qvmhip_time_program(qvmhip_program *p, int reps, double *ms);
 
+/* --- batches of small states (no single counterpart in the reference) ---
+ * Many independent states (the parameter sweeps of examples/qaoa.lisp, run
+ * there one QVM at a time) kept in one HBM buffer.  Each instance gets a
+ * program of tile passes and small gates; finalize schedules the programs
+ * level by level (all tile passes of a level and width in ONE batched kernel)
+ * followed by each instance's ZZ expectation and samples (cdf >= u rule), and
+ * captures the whole run as one hipGraph.  Every run starts from |0...0>. */
+typedef struct qvmhip_batch qvmhip_batch;
+QVMHIP_API int qvmhip_batch_create(uint32_t count, const uint32_t *nqubits, int dtype, qvmhip_batch **out);
+QVMHIP_API int qvmhip_batch_destroy(qvmhip_batch *b);
+QVMHIP_API int qvmhip_batch_add_tile_pass(qvmhip_batch *b, uint32_t inst, uint32_t m, const uint32_t *tile_qubits,
+                                          uint32_t nops, const qvmhip_tile_op *ops);
+/* 1..3-qubit gate, same convention as qvmhip_apply_matrix. */
+QVMHIP_API int qvmhip_batch_add_matrix(qvmhip_batch *b, uint32_t inst, const double *U, uint32_t k,
+                                       const uint32_t *qubits);
+/* Cost operator sum_e Z_a Z_b of instance `inst` (edges = 2*nedges qubits). */
+QVMHIP_API int qvmhip_batch_set_zz(qvmhip_batch *b, uint32_t inst, const uint32_t *edges, uint32_t nedges);
+/* `shots` draws per instance; u holds count*shots uniforms, instance-major. */
+QVMHIP_API int qvmhip_batch_set_shots(qvmhip_batch *b, uint64_t shots, const double *u);
+QVMHIP_API int qvmhip_batch_finalize(qvmhip_batch *b, int use_graph);
+QVMHIP_API int qvmhip_batch_run(qvmhip_batch *b);
+QVMHIP_API int qvmhip_batch_sync(qvmhip_batch *b);
+/* expectations[count], shots[count*shots] (either may be NULL). */
+QVMHIP_API int qvmhip_batch_results(qvmhip_batch *b, double *expectations, uint64_t *shots);
+QVMHIP_API int qvmhip_batch_read_amplitudes(qvmhip_batch *b, uint32_t inst, uint64_t start, uint64_t count,
+                                            double *out);
+
 /* --- fusion cost model (no counterpart in the reference) ----------------
  * A small MLP (din -> hidden -> hidden -> 1, ReLU, FP32) that predicts the
  * log launch time of a planned launch from its features; trained and run by

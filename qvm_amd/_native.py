@@ -106,6 +106,17 @@ SIGNATURES = {
     "qvmhip_tv_distance": [_vp, _vp, _dp],
     "qvmhip_profile_enable": [_vp, C.c_int],
     "qvmhip_profile_summary": [_vp, C.c_int, _u64p, _dp],
+    "qvmhip_batch_create": [_u32, _u32p, C.c_int, C.POINTER(_vp)],
+    "qvmhip_batch_destroy": [_vp],
+    "qvmhip_batch_add_tile_pass": [_vp, _u32, _u32, _u32p, _u32, C.POINTER(TileOp)],
+    "qvmhip_batch_add_matrix": [_vp, _u32, _dp, _u32, _u32p],
+    "qvmhip_batch_set_zz": [_vp, _u32, _u32p, _u32],
+    "qvmhip_batch_set_shots": [_vp, _u64, _dp],
+    "qvmhip_batch_finalize": [_vp, C.c_int],
+    "qvmhip_batch_run": [_vp],
+    "qvmhip_batch_sync": [_vp],
+    "qvmhip_batch_results": [_vp, _dp, _u64p],
+    "qvmhip_batch_read_amplitudes": [_vp, _u32, _u64, _u64, _dp],
     "qvmhip_mlp_create": [C.c_int, C.c_int, _fp, C.POINTER(_vp)],
     "qvmhip_mlp_destroy": [_vp],
     "qvmhip_mlp_num_params": [_vp, _ip],

@@ -30,6 +30,7 @@ SOURCES = {
     "fused.hip": [],
     "program.hip": [],
     "mlp.hip": [],
+    "batch.hip": [],
 }
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

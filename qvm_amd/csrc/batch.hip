@@ -1,0 +1,431 @@
+// batch.hip -- many small states run as one captured graph.
+//
+// The QAOA-style workload is hundreds of independent circuits on 10..24
+// qubits.  Launching each instance's kernels separately leaves the GPU idle
+// (a 2^10-amplitude tile pass is one workgroup), so a batch keeps every state
+// in one HBM buffer and runs level-synchronously: level L is the L-th planned
+// item of every instance, and all tile passes of a level with the same tile
+// width go out as ONE batched kernel (k_tile_batch) whose workgroups are
+// distributed over the instances by a prefix of their tile counts.  After the
+// last level, each instance's cost expectation and samples are computed.  The
+// whole sequence -- reset to |0...0>, levels, expectations, sampling -- is
+// captured once into a hipGraph and replayed with one graph launch.
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "qvmhip_internal.h"
+
+using namespace qvmhip;
+
+namespace {
+
+enum { kItemTile = 1, kItemGate = 2 };
+
+struct BItem {
+    int kind = 0;
+    uint32_t k = 0;            // tile m or gate width
+    uint32_t qubits[16] = {0};
+    size_t op_off = 0;         // tile ops in blob
+    uint32_t nops = 0;
+    std::vector<double> U;     // gate matrix (k <= 3)
+};
+
+struct BLaunch {
+    int kind;
+    uint32_t m;         // tile width
+    size_t ent_off;     // bytes into the entry buffer
+    uint32_t nent;
+    uint64_t blocks;
+    uint32_t inst;      // gate launches
+    const BItem *item;  // gate launches
+};
+
+template <typename T>
+__global__ void k_batch_reset(vec2_t<T> *__restrict__ buf, uint64_t total) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        vec2_t<T> z;
+        z.x = T(0);
+        z.y = T(0);
+        buf[i] = z;
+    }
+}
+
+template <typename T>
+__global__ void k_batch_ones(vec2_t<T> *__restrict__ buf, const uint64_t *__restrict__ starts, uint32_t count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) {
+        vec2_t<T> one;
+        one.x = T(1);
+        one.y = T(0);
+        buf[starts[i]] = one;
+    }
+}
+
+}  // namespace
+
+struct qvmhip_batch {
+    uint32_t count = 0;
+    int dtype = QVMHIP_F64;
+    size_t amp_bytes = 16;
+    std::vector<uint32_t> nq;
+    std::vector<uint64_t> offset;  // amplitude offset of each state
+    uint64_t total = 0;
+    void *buf = nullptr;
+    hipStream_t stream = nullptr;
+    // scratch shared by all views (work on one stream is serialised)
+    double *red = nullptr, *chunk_sum = nullptr, *chunk_pre = nullptr, *scan_tmp = nullptr;
+    void *mat = nullptr;
+    std::vector<qvmhip_state> views;
+    // programs
+    std::vector<std::vector<BItem>> prog;
+    std::vector<char> blob;
+    void *blob_dev = nullptr;
+    std::vector<std::vector<uint32_t>> edges;
+    uint32_t *edges_dev = nullptr;
+    std::vector<size_t> edge_off;
+    double *exp_dev = nullptr;
+    uint64_t shots = 0;
+    std::vector<double> u_host;
+    double *u_dev = nullptr;
+    uint64_t *out_dev = nullptr;
+    uint64_t *starts_dev = nullptr;
+    std::vector<BLaunch> launches;
+    std::vector<char> entries;
+    void *entries_dev = nullptr;
+    bool finalized = false, use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void free_batch(qvmhip_batch *b) {
+    if (!b) return;
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    for (void *p : {(void *)b->buf, (void *)b->red, (void *)b->chunk_sum, (void *)b->chunk_pre, (void *)b->scan_tmp,
+                    b->mat, b->blob_dev, (void *)b->edges_dev, (void *)b->exp_dev, (void *)b->u_dev,
+                    (void *)b->out_dev, (void *)b->starts_dev, b->entries_dev})
+        if (p) hipFree(p);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int check_inst(const qvmhip_batch *b, uint32_t inst) {
+    if (!b) return fail(QVMHIP_EINVAL, "null batch");
+    if (b->finalized) return fail(QVMHIP_EINVAL, "batch already finalized");
+    if (inst >= b->count) return fail(QVMHIP_EINVAL, "instance index out of range");
+    return QVMHIP_OK;
+}
+
+size_t push(std::vector<char> &blob, const void *p, size_t bytes) {
+    const size_t off = (blob.size() + 255) & ~size_t(255);
+    blob.resize(off + bytes);
+    std::memcpy(blob.data() + off, p, bytes);
+    return off;
+}
+
+// Enqueue the whole batch on its stream (kernel launches only, capturable).
+int enqueue(qvmhip_batch *b) {
+    const uint32_t rg = (uint32_t)std::min<uint64_t>((b->total + 255) / 256, 256 * 64);
+    if (b->dtype == QVMHIP_F64) {
+        hipLaunchKernelGGL(k_batch_reset<double>, dim3(rg), dim3(256), 0, b->stream, (double2 *)b->buf, b->total);
+        hipLaunchKernelGGL(k_batch_ones<double>, dim3((b->count + 255) / 256), dim3(256), 0, b->stream,
+                           (double2 *)b->buf, b->starts_dev, b->count);
+    } else {
+        hipLaunchKernelGGL(k_batch_reset<float>, dim3(rg), dim3(256), 0, b->stream, (float2 *)b->buf, b->total);
+        hipLaunchKernelGGL(k_batch_ones<float>, dim3((b->count + 255) / 256), dim3(256), 0, b->stream,
+                           (float2 *)b->buf, b->starts_dev, b->count);
+    }
+    QH_LAUNCH_CHECK();
+    for (const BLaunch &L : b->launches) {
+        int rc;
+        if (L.kind == kItemTile)
+            rc = launch_tile_batch(b->stream, b->dtype, L.m, b->buf, (const char *)b->entries_dev + L.ent_off, L.nent,
+                                   L.blocks);
+        else
+            rc = launch_gate(&b->views[L.inst], L.item->U.data(), L.item->k, L.item->qubits, 0);
+        if (rc) return rc;
+    }
+    for (uint32_t i = 0; i < b->count; ++i) {
+        if (!b->edges[i].empty()) {
+            int rc = expectation_zz_to_device(&b->views[i], b->edges_dev + b->edge_off[i] / sizeof(uint32_t),
+                                              (uint32_t)(b->edges[i].size() / 2), b->exp_dev + i);
+            if (rc) return rc;
+        }
+        if (b->shots) {
+            int rc = sample_to_device(&b->views[i], b->u_dev + (size_t)i * b->shots, b->shots,
+                                      b->out_dev + (size_t)i * b->shots, 1);
+            if (rc) return rc;
+        }
+    }
+    return QVMHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qvmhip_batch_create(uint32_t count, const uint32_t *nqubits, int dtype, qvmhip_batch **out) {
+    if (!out || !nqubits || count == 0) return fail(QVMHIP_EINVAL, "bad argument");
+    if (dtype != QVMHIP_F64 && dtype != QVMHIP_F32) return fail(QVMHIP_EINVAL, "dtype must be F64 or F32");
+    *out = nullptr;
+    qvmhip_batch *b = new qvmhip_batch();
+    b->count = count;
+    b->dtype = dtype;
+    b->amp_bytes = dtype == QVMHIP_F64 ? 16 : 8;
+    uint32_t nmax = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        if (nqubits[i] < 1 || nqubits[i] > 30) {
+            free_batch(b);
+            return fail(QVMHIP_EINVAL, "batch states have 1..30 qubits");
+        }
+        b->nq.push_back(nqubits[i]);
+        b->offset.push_back(b->total);
+        b->total += uint64_t(1) << nqubits[i];
+        nmax = std::max(nmax, nqubits[i]);
+    }
+    const uint64_t nch = std::max<uint64_t>(1, (uint64_t(1) << nmax) >> kChunkLog2);
+    const uint64_t nseg = (nch + 1023) / 1024;
+    hipError_t e = hipSuccess;
+    auto bail = [&](const char *what) {
+        free_batch(b);
+        return fail(e == hipErrorOutOfMemory ? QVMHIP_ENOMEM : QVMHIP_EDEVICE,
+                    std::string(what) + ": " + hipGetErrorString(e));
+    };
+    if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return bail("stream");
+    if ((e = hipMalloc(&b->buf, b->total * b->amp_bytes)) != hipSuccess) return bail("hipMalloc(states)");
+    if ((e = hipMalloc((void **)&b->red, sizeof(double) * (kRedBlocks * 2 + 16))) != hipSuccess) return bail("red");
+    if ((e = hipMalloc((void **)&b->chunk_sum, sizeof(double) * nch)) != hipSuccess) return bail("chunks");
+    if ((e = hipMalloc((void **)&b->chunk_pre, sizeof(double) * nch)) != hipSuccess) return bail("prefix");
+    if ((e = hipMalloc((void **)&b->scan_tmp, sizeof(double) * nseg)) != hipSuccess) return bail("scan");
+    if ((e = hipMalloc(&b->mat, 128 * 1024)) != hipSuccess) return bail("matrix slot");
+    if ((e = hipMalloc((void **)&b->exp_dev, sizeof(double) * count)) != hipSuccess) return bail("expectations");
+    if ((e = hipMemset(b->exp_dev, 0, sizeof(double) * count)) != hipSuccess) return bail("memset");
+    if ((e = hipMalloc((void **)&b->starts_dev, sizeof(uint64_t) * count)) != hipSuccess) return bail("starts");
+    if ((e = hipMemcpy(b->starts_dev, b->offset.data(), sizeof(uint64_t) * count, hipMemcpyHostToDevice)) !=
+        hipSuccess)
+        return bail("starts upload");
+    b->views.resize(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        qvmhip_state &v = b->views[i];
+        v.n = b->nq[i];
+        v.dtype = dtype;
+        v.len = uint64_t(1) << v.n;
+        v.amp_bytes = b->amp_bytes;
+        v.amps = (char *)b->buf + b->offset[i] * b->amp_bytes;
+        v.stream = b->stream;
+        v.own_stream = false;
+        v.red = b->red;
+        v.mat = b->mat;
+        v.chunk_sum = b->chunk_sum;
+        v.chunk_pre = b->chunk_pre;
+        v.scan_tmp = b->scan_tmp;
+        v.nchunks = std::max<uint64_t>(1, v.len >> kChunkLog2);
+    }
+    b->prog.resize(count);
+    b->edges.resize(count);
+    *out = b;
+    return QVMHIP_OK;
+}
+
+int qvmhip_batch_add_tile_pass(qvmhip_batch *b, uint32_t inst, uint32_t m, const uint32_t *tile_qubits,
+                               uint32_t nops, const qvmhip_tile_op *ops) {
+    int rc = check_inst(b, inst);
+    if (rc) return rc;
+    const uint32_t n = b->nq[inst];
+    if (m < 8 || m > n || m > 13 || (m == 13 && b->dtype == QVMHIP_F64))
+        return fail(QVMHIP_EINVAL, "tile size must be 8..min(n,12) (F64) or ..13 (F32)");
+    if (!tile_qubits || (nops && !ops)) return fail(QVMHIP_EINVAL, "null argument");
+    for (uint32_t i = 0; i < m; ++i) {
+        if (tile_qubits[i] >= n) return fail(QVMHIP_EINVAL, "tile qubit out of range");
+        if (i && tile_qubits[i] <= tile_qubits[i - 1]) return fail(QVMHIP_EINVAL, "tile qubits must ascend");
+    }
+    for (uint32_t o = 0; o < nops; ++o) {
+        const qvmhip_tile_op &op = ops[o];
+        if (op.kind < QVMHIP_TILE_DENSE1 || op.kind > QVMHIP_TILE_CPHASE)
+            return fail(QVMHIP_EINVAL, "unknown tile op kind");
+        const bool two = op.kind != QVMHIP_TILE_DENSE1 && op.kind != QVMHIP_TILE_DIAG1;
+        if (op.q0 >= m || (two && (op.q1 >= m || op.q1 == op.q0)) || (op.ctrl_mask >> m))
+            return fail(QVMHIP_EINVAL, "tile op qubit out of range");
+    }
+    if (!nops) return QVMHIP_OK;
+    BItem it;
+    it.kind = kItemTile;
+    it.k = m;
+    std::copy(tile_qubits, tile_qubits + m, it.qubits);
+    it.nops = nops;
+    it.op_off = push(b->blob, ops, sizeof(qvmhip_tile_op) * nops);
+    b->prog[inst].push_back(std::move(it));
+    return QVMHIP_OK;
+}
+
+int qvmhip_batch_add_matrix(qvmhip_batch *b, uint32_t inst, const double *U, uint32_t k, const uint32_t *qubits) {
+    int rc = check_inst(b, inst);
+    if (rc) return rc;
+    if (!U || !qubits || k < 1 || k > 3 || k > b->nq[inst])
+        return fail(QVMHIP_EINVAL, "batch gates are 1..3 qubits (wider gates go into tile passes or programs)");
+    for (uint32_t i = 0; i < k; ++i) {
+        if (qubits[i] >= b->nq[inst]) return fail(QVMHIP_EINVAL, "qubit out of range");
+        for (uint32_t j = 0; j < i; ++j)
+            if (qubits[i] == qubits[j]) return fail(QVMHIP_EINVAL, "repeated qubit in gate");
+    }
+    BItem it;
+    it.kind = kItemGate;
+    it.k = k;
+    std::copy(qubits, qubits + k, it.qubits);
+    it.U.assign(U, U + (size_t(2) << (2 * k)));
+    b->prog[inst].push_back(std::move(it));
+    return QVMHIP_OK;
+}
+
+int qvmhip_batch_set_zz(qvmhip_batch *b, uint32_t inst, const uint32_t *edges, uint32_t nedges) {
+    int rc = check_inst(b, inst);
+    if (rc) return rc;
+    if (nedges > 256 || (nedges && !edges)) return fail(QVMHIP_EINVAL, "at most 256 edges");
+    for (uint32_t i = 0; i < 2 * nedges; ++i)
+        if (edges[i] >= b->nq[inst]) return fail(QVMHIP_EINVAL, "edge qubit out of range");
+    b->edges[inst].assign(edges, edges + 2 * nedges);
+    return QVMHIP_OK;
+}
+
+int qvmhip_batch_set_shots(qvmhip_batch *b, uint64_t shots, const double *u) {
+    if (!b) return fail(QVMHIP_EINVAL, "null batch");
+    if (b->finalized) return fail(QVMHIP_EINVAL, "batch already finalized");
+    if (shots && !u) return fail(QVMHIP_EINVAL, "null uniforms");
+    for (uint64_t i = 0; i < shots * b->count; ++i)
+        if (!(u[i] >= 0.0 && u[i] <= 1.0)) return fail(QVMHIP_ERANGE, "uniform draw outside [0,1]");
+    b->shots = shots;
+    b->u_host.assign(u, u + shots * b->count);
+    return QVMHIP_OK;
+}
+
+int qvmhip_batch_finalize(qvmhip_batch *b, int use_graph) {
+    if (!b) return fail(QVMHIP_EINVAL, "null batch");
+    if (b->finalized) return fail(QVMHIP_EINVAL, "batch already finalized");
+    hipError_t e;
+    if (!b->blob.empty()) {
+        if ((e = hipMalloc(&b->blob_dev, b->blob.size())) != hipSuccess) return fail(QVMHIP_ENOMEM, "blob");
+        QH_CHECK(hipMemcpy(b->blob_dev, b->blob.data(), b->blob.size(), hipMemcpyHostToDevice));
+    }
+    std::vector<uint32_t> eh;
+    b->edge_off.assign(b->count, 0);
+    for (uint32_t i = 0; i < b->count; ++i) {
+        b->edge_off[i] = eh.size() * sizeof(uint32_t);
+        eh.insert(eh.end(), b->edges[i].begin(), b->edges[i].end());
+    }
+    if (!eh.empty()) {
+        QH_CHECK(hipMalloc((void **)&b->edges_dev, eh.size() * sizeof(uint32_t)));
+        QH_CHECK(hipMemcpy(b->edges_dev, eh.data(), eh.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    if (b->shots) {
+        QH_CHECK(hipMalloc((void **)&b->u_dev, b->u_host.size() * sizeof(double)));
+        QH_CHECK(hipMemcpy(b->u_dev, b->u_host.data(), b->u_host.size() * sizeof(double), hipMemcpyHostToDevice));
+        QH_CHECK(hipMalloc((void **)&b->out_dev, b->u_host.size() * sizeof(uint64_t)));
+    }
+    // levels: the L-th item of every instance; tile passes grouped by width
+    size_t depth = 0;
+    for (auto &p : b->prog) depth = std::max(depth, p.size());
+    const size_t esz = tile_entry_bytes();
+    std::vector<size_t> ent_host_off;  // per launch: offset into b->entries
+    for (size_t L = 0; L < depth; ++L) {
+        std::map<uint32_t, std::vector<uint32_t>> by_m;
+        for (uint32_t i = 0; i < b->count; ++i) {
+            if (L >= b->prog[i].size()) continue;
+            const BItem &it = b->prog[i][L];
+            if (it.kind == kItemTile)
+                by_m[it.k].push_back(i);
+            else
+                b->launches.push_back({kItemGate, 0, 0, 0, 0, i, &it});
+        }
+        for (auto &kv : by_m) {
+            BLaunch bl{kItemTile, kv.first, b->entries.size(), (uint32_t)kv.second.size(), 0, 0, nullptr};
+            uint64_t blocks = 0;
+            for (uint32_t i : kv.second) {
+                const BItem &it = b->prog[i][L];
+                std::vector<char> ent(esz);
+                uint64_t nt = 0;
+                build_tile_entry(ent.data(), b->nq[i], it.k, it.qubits, b->offset[i], blocks,
+                                 (const qvmhip_tile_op *)((const char *)b->blob_dev + it.op_off), it.nops, &nt);
+                b->entries.insert(b->entries.end(), ent.begin(), ent.end());
+                blocks += nt;
+            }
+            bl.blocks = blocks;
+            b->launches.push_back(bl);
+        }
+    }
+    if (!b->entries.empty()) {
+        QH_CHECK(hipMalloc(&b->entries_dev, b->entries.size()));
+        QH_CHECK(hipMemcpy(b->entries_dev, b->entries.data(), b->entries.size(), hipMemcpyHostToDevice));
+    }
+    b->use_graph = use_graph != 0;
+    if (b->use_graph) {
+        QH_CHECK(hipStreamSynchronize(b->stream));
+        QH_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue(b);
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(b->stream, &g);
+        if (rc) {
+            if (g) hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(QVMHIP_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        b->graph = g;
+        QH_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    b->finalized = true;
+    return QVMHIP_OK;
+}
+
+int qvmhip_batch_run(qvmhip_batch *b) {
+    if (!b) return fail(QVMHIP_EINVAL, "null batch");
+    if (!b->finalized) return fail(QVMHIP_EINVAL, "batch not finalized");
+    if (b->use_graph) {
+        QH_CHECK(hipGraphLaunch(b->exec, b->stream));
+        return QVMHIP_OK;
+    }
+    return enqueue(b);
+}
+
+int qvmhip_batch_sync(qvmhip_batch *b) {
+    if (!b) return fail(QVMHIP_EINVAL, "null batch");
+    QH_CHECK(hipStreamSynchronize(b->stream));
+    return QVMHIP_OK;
+}
+
+int qvmhip_batch_results(qvmhip_batch *b, double *expectations, uint64_t *shots) {
+    if (!b) return fail(QVMHIP_EINVAL, "null batch");
+    QH_CHECK(hipStreamSynchronize(b->stream));
+    if (expectations) QH_CHECK(hipMemcpy(expectations, b->exp_dev, sizeof(double) * b->count, hipMemcpyDeviceToHost));
+    if (shots && b->shots)
+        QH_CHECK(hipMemcpy(shots, b->out_dev, sizeof(uint64_t) * b->shots * b->count, hipMemcpyDeviceToHost));
+    return QVMHIP_OK;
+}
+
+int qvmhip_batch_read_amplitudes(qvmhip_batch *b, uint32_t inst, uint64_t start, uint64_t count, double *out) {
+    if (!b || inst >= b->count || (count && !out)) return fail(QVMHIP_EINVAL, "bad argument");
+    const uint64_t len = uint64_t(1) << b->nq[inst];
+    if (start > len || count > len - start) return fail(QVMHIP_EINVAL, "range out of bounds");
+    QH_CHECK(hipStreamSynchronize(b->stream));
+    if (b->dtype == QVMHIP_F64) {
+        QH_CHECK(hipMemcpy(out, (const double2 *)b->buf + b->offset[inst] + start, count * 16, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> tmp(2 * count);
+        QH_CHECK(hipMemcpy(tmp.data(), (const float2 *)b->buf + b->offset[inst] + start, count * 8,
+                           hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < 2 * count; ++i) out[i] = tmp[i];
+    }
+    return QVMHIP_OK;
+}
+
+int qvmhip_batch_destroy(qvmhip_batch *b) {
+    if (!b) return QVMHIP_OK;
+    hipStreamSynchronize(b->stream);
+    free_batch(b);
+    return QVMHIP_OK;
+}
+
+}  // extern "C"

@@ -173,6 +173,14 @@ int tv_distance(qvmhip_state *a, qvmhip_state *b, double *out);
 int probabilities(qvmhip_state *s, uint64_t start, uint64_t count, double *host_out);
 int convert_f32_to_f64(qvmhip_state *s, uint64_t start, uint64_t count, double *dev_out);
 int convert_f64_to_f32(qvmhip_state *s, uint64_t start, uint64_t count, const double *dev_in);
+// Capturable (kernel launches only) forms used by batches and programs.
+int expectation_zz_to_device(qvmhip_state *s, const uint32_t *edges_dev, uint32_t nedges, double *out_dev);
+int sample_to_device(qvmhip_state *s, const double *u_dev, uint64_t nshots, uint64_t *out_dev, int rule_ge);
+size_t tile_entry_bytes();
+void build_tile_entry(void *entry, uint32_t n, uint32_t m, const uint32_t *tq, uint64_t amp_offset,
+                      uint64_t block_start, const qvmhip_tile_op *ops_dev, uint32_t nops, uint64_t *ntiles);
+int launch_tile_batch(hipStream_t st, int dtype, uint32_t m, void *base, const void *entries_dev, uint32_t nent,
+                      uint64_t total_blocks);
 }  // namespace qvmhip
 
 #define QH_CHECK(expr)                                                                    \

@@ -341,47 +341,49 @@ __global__ __launch_bounds__(kBlock) void k_sample_search(const vec2_t<T> *__res
                                                           uint64_t nshots, int rule_ge,
                                                           uint64_t *__restrict__ out) {
     const int lane = threadIdx.x % kWave;
-    const uint64_t shot = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
-    if (shot >= nshots) return;
-    const double p = u[shot];
-    // last chunk c with pre[c] <= p (rule >) or pre[c] < p (rule >=); 0 if none
-    uint64_t lo = 0, hi = nchunks;  // invariant: answer in [lo, hi)
-    while (hi - lo > 1) {
-        const uint64_t mid = lo + (hi - lo) / 2;
-        const double v = pre[mid];
-        const bool ok = rule_ge ? (v < p) : (v <= p);
-        if (ok)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    uint64_t result = len - 1;
-    bool found = false;
-    for (uint64_t c = lo; c < nchunks && !found; ++c) {
-        double pr[4];
-        chunk_lane_probs<T>(psi, len, c, lane, pr);
-        const double local = ((pr[0] + pr[1]) + pr[2]) + pr[3];
-        const double incl = wave_inclusive_scan(local, lane);
-        const double up = __shfl_up(incl, 1, kWave);
-        double run = lane ? up : 0.0;  // exclusive lane offset
-        int hit = -1;
+    const uint64_t sstep = (uint64_t)gridDim.x * (kBlock / kWave);
+    for (uint64_t shot = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; shot < nshots;
+         shot += sstep) {
+        const double p = u[shot];
+        // last chunk c with pre[c] <= p (rule >) or pre[c] < p (rule >=); 0 if none
+        uint64_t lo = 0, hi = nchunks;  // invariant: answer in [lo, hi)
+        while (hi - lo > 1) {
+            const uint64_t mid = lo + (hi - lo) / 2;
+            const double v = pre[mid];
+            const bool ok = rule_ge ? (v < p) : (v <= p);
+            if (ok)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        uint64_t result = len - 1;
+        bool found = false;
+        for (uint64_t c = lo; c < nchunks && !found; ++c) {
+            double pr[4];
+            chunk_lane_probs<T>(psi, len, c, lane, pr);
+            const double local = ((pr[0] + pr[1]) + pr[2]) + pr[3];
+            const double incl = wave_inclusive_scan(local, lane);
+            const double up = __shfl_up(incl, 1, kWave);
+            double run = lane ? up : 0.0;  // exclusive lane offset
+            int hit = -1;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            run += pr[k];
-            const double C = pre[c] + run;
-            const bool ok = rule_ge ? (C >= p) : (C > p);
-            const uint64_t idx = (c << kChunkLog2) + (uint64_t)lane * 4 + k;
-            if (ok && hit < 0 && idx < len) hit = k;
+            for (int k = 0; k < 4; ++k) {
+                run += pr[k];
+                const double C = pre[c] + run;
+                const bool ok = rule_ge ? (C >= p) : (C > p);
+                const uint64_t idx = (c << kChunkLog2) + (uint64_t)lane * 4 + k;
+                if (ok && hit < 0 && idx < len) hit = k;
+            }
+            const unsigned long long mask = __ballot(hit >= 0);
+            if (mask) {
+                const int first = __ffsll((long long)mask) - 1;
+                const int k = __shfl(hit, first, kWave);
+                result = (c << kChunkLog2) + (uint64_t)first * 4 + (uint64_t)k;
+                found = true;
+            }
         }
-        const unsigned long long mask = __ballot(hit >= 0);
-        if (mask) {
-            const int first = __ffsll((long long)mask) - 1;
-            const int k = __shfl(hit, first, kWave);
-            result = (c << kChunkLog2) + (uint64_t)first * 4 + (uint64_t)k;
-            found = true;
-        }
+        if (lane == 0) out[shot] = result;
     }
-    if (lane == 0) out[shot] = result;
 }
 
 template <typename T>
@@ -535,6 +537,37 @@ int sample_shots(qvmhip_state *s, const double *u, uint64_t nshots, uint64_t *ou
             std::memcpy(out + first, stage_out, c * sizeof(uint64_t));
         }
     }
+    return QVMHIP_OK;
+}
+
+int sample_to_device(qvmhip_state *s, const double *u_dev, uint64_t nshots, uint64_t *out_dev, int rule_ge) {
+    if (nshots == 0) return QVMHIP_OK;
+    int rc = build_chunk_prefix(s);
+    if (rc) return rc;
+    const uint64_t wpb = kBlock / kWave;
+    const uint32_t nb = (uint32_t)std::min<uint64_t>((nshots + wpb - 1) / wpb, kMaxGrid);
+    if (s->dtype == QVMHIP_F64)
+        hipLaunchKernelGGL(k_sample_search<double>, dim3(nb), dim3(kBlock), 0, s->stream, (const double2 *)s->amps,
+                           s->len, s->nchunks, s->chunk_pre, u_dev, nshots, rule_ge, out_dev);
+    else
+        hipLaunchKernelGGL(k_sample_search<float>, dim3(nb), dim3(kBlock), 0, s->stream, (const float2 *)s->amps,
+                           s->len, s->nchunks, s->chunk_pre, u_dev, nshots, rule_ge, out_dev);
+    QH_LAUNCH_CHECK();
+    return QVMHIP_OK;
+}
+
+int expectation_zz_to_device(qvmhip_state *s, const uint32_t *edges_dev, uint32_t nedges, double *out_dev) {
+    if (nedges > 256) return fail(QVMHIP_EINVAL, "at most 256 edges");
+    const uint32_t grid = red_grid(s->len);
+    if (s->dtype == QVMHIP_F64)
+        hipLaunchKernelGGL(k_zz_partial<double>, dim3(grid), dim3(kBlock), 0, s->stream, (const double2 *)s->amps,
+                           s->len, edges_dev, nedges, s->red);
+    else
+        hipLaunchKernelGGL(k_zz_partial<float>, dim3(grid), dim3(kBlock), 0, s->stream, (const float2 *)s->amps,
+                           s->len, edges_dev, nedges, s->red);
+    QH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(kBlock), 0, s->stream, s->red, (int)grid, 1, out_dev);
+    QH_LAUNCH_CHECK();
     return QVMHIP_OK;
 }
 

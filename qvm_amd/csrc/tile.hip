@@ -48,16 +48,17 @@ __device__ inline void cmul(T ar, T ai, T br, T bi, T &r, T &i) {
     i = ar * bi + ai * br;
 }
 
+// One tile: load the 2^M amplitudes of tile `tile` into LDS, apply the op
+// list, store.  Shared by the single-state and the batched kernels.
 template <typename T, int M, int NT>
-__global__ __launch_bounds__(NT) void k_tile(vec2_t<T> *__restrict__ psi, const TileArgs a,
-                                             const qvmhip_tile_op *__restrict__ ops, uint32_t nops) {
+__device__ inline void tile_body(vec2_t<T> *__restrict__ psi, const TileArgs &a,
+                                 const qvmhip_tile_op *__restrict__ ops, uint32_t nops, uint64_t tile,
+                                 vec2_t<T> *sa, uint64_t toff) {
     using V = vec2_t<T>;
     constexpr int S = 1 << M;
     constexpr int R = S / NT;  // amplitudes per thread
-    __shared__ V sa[S];
     const uint32_t t = threadIdx.x;
-    const uint64_t toff = thread_offset<NT>(a, t);
-    for (uint64_t tile = blockIdx.x; tile < a.g.ngroups; tile += gridDim.x) {
+    {
         const uint64_t base = group_base(a.g, tile) + toff;
 
         // load: element l = t + r*NT (only this thread reads these slots back in
@@ -185,24 +186,64 @@ __global__ __launch_bounds__(NT) void k_tile(vec2_t<T> *__restrict__ psi, const 
     }
 }
 
+template <typename T, int M, int NT>
+__global__ __launch_bounds__(NT) void k_tile(vec2_t<T> *__restrict__ psi, const TileArgs a,
+                                             const qvmhip_tile_op *__restrict__ ops, uint32_t nops) {
+    __shared__ vec2_t<T> sa[1 << M];
+    const uint64_t toff = thread_offset<NT>(a, threadIdx.x);
+    for (uint64_t tile = blockIdx.x; tile < a.g.ngroups; tile += gridDim.x) {
+        tile_body<T, M, NT>(psi, a, ops, nops, tile, sa, toff);
+    }
+}
+
+// Batched form: many states of one buffer, one entry per state.  Entry e
+// owns blocks [block_start[e], block_start[e] + ntiles[e]) of the launch.
+struct TileEntry {
+    TileArgs args;
+    uint64_t amp_offset;   // first amplitude of the state in the batch buffer
+    uint64_t block_start;  // first block of the launch that works on this entry
+    const qvmhip_tile_op *ops;
+    uint32_t nops;
+    uint32_t pad;
+};
+
+template <typename T, int M, int NT>
+__global__ __launch_bounds__(NT) void k_tile_batch(vec2_t<T> *__restrict__ base, const TileEntry *__restrict__ ent,
+                                                   uint32_t nent, uint64_t total_blocks) {
+    __shared__ vec2_t<T> sa[1 << M];
+    for (uint64_t b = blockIdx.x; b < total_blocks; b += gridDim.x) {
+        uint32_t lo = 0, hi = nent;  // last entry with block_start <= b
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (ent[mid].block_start <= b)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const TileEntry &e = ent[lo];
+        const uint64_t toff = thread_offset<NT>(e.args, threadIdx.x);
+        tile_body<T, M, NT>(base + e.amp_offset, e.args, e.ops, e.nops, b - e.block_start, sa, toff);
+        __syncthreads();  // the next block id may reuse sa with other threads' slots
+    }
+}
+
 namespace {
 
-template <typename T, int M>
-int run_tile(qvmhip_state *s, const uint32_t *tq, uint32_t m, const qvmhip_tile_op *ops_dev,
-             uint32_t nops) {
-    constexpr int NT = M >= 12 ? 512 : 256;
-    constexpr int LOGNT = M >= 12 ? 9 : 8;
-    TileArgs a;
+constexpr int threads_for(int m) { return m >= 12 ? 512 : 256; }
+
+// Host-side tile arguments of an n-qubit state and tile qubits tq[0..m).
+void make_tile_args(uint32_t n, uint32_t m, const uint32_t *tq, TileArgs &a) {
+    const int NT = threads_for((int)m);
+    const int LOGNT = NT == 512 ? 9 : 8;
     std::memset(&a, 0, sizeof(a));
-    std::vector<uint32_t> pos(tq, tq + m);
     a.g.nsorted = m;
     for (uint32_t i = 0; i < m; ++i) {
-        a.g.sorted[i] = pos[i];
-        a.tq[i] = pos[i];
+        a.g.sorted[i] = tq[i];
+        a.tq[i] = tq[i];
     }
     a.g.ctrl_mask = 0;
-    a.g.ngroups = uint64_t(1) << (s->n - m);
-    constexpr int R = (1 << M) / NT;
+    a.g.ngroups = uint64_t(1) << (n - m);
+    const int R = (1 << m) / NT;
     for (int r = 0; r < R; ++r) {
         uint64_t o = 0;
         const uint32_t l = (uint32_t)r << LOGNT;
@@ -210,6 +251,14 @@ int run_tile(qvmhip_state *s, const uint32_t *tq, uint32_t m, const qvmhip_tile_
             if ((l >> b) & 1u) o |= uint64_t(1) << tq[b];
         a.roff[r] = o;
     }
+}
+
+template <typename T, int M>
+int run_tile(qvmhip_state *s, const uint32_t *tq, uint32_t m, const qvmhip_tile_op *ops_dev,
+             uint32_t nops) {
+    constexpr int NT = threads_for(M);
+    TileArgs a;
+    make_tile_args(s->n, m, tq, a);
     prof_begin(s, QVMHIP_K_TILE);
     const uint32_t grid = (uint32_t)std::min<uint64_t>(a.g.ngroups, kMaxGrid);
     hipLaunchKernelGGL((k_tile<T, M, NT>), dim3(grid), dim3(NT), 0, s->stream,
@@ -261,6 +310,61 @@ int launch_tile_pass(qvmhip_state *s, uint32_t m, const uint32_t *tile_qubits, u
         hipFree(tmp);
     }
     return rc;
+}
+
+}  // namespace qvmhip
+
+namespace qvmhip {
+
+size_t tile_entry_bytes() { return sizeof(TileEntry); }
+
+void build_tile_entry(void *entry, uint32_t n, uint32_t m, const uint32_t *tq, uint64_t amp_offset,
+                      uint64_t block_start, const qvmhip_tile_op *ops_dev, uint32_t nops, uint64_t *ntiles) {
+    TileEntry e;
+    std::memset(&e, 0, sizeof(e));
+    make_tile_args(n, m, tq, e.args);
+    e.amp_offset = amp_offset;
+    e.block_start = block_start;
+    e.ops = ops_dev;
+    e.nops = nops;
+    *ntiles = e.args.g.ngroups;
+    std::memcpy(entry, &e, sizeof(e));
+}
+
+namespace {
+template <typename T, int M>
+void tile_batch_go(hipStream_t st, void *base, const void *ent, uint32_t nent, uint64_t total) {
+    constexpr int NT = threads_for(M);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(total, kMaxGrid);
+    hipLaunchKernelGGL((k_tile_batch<T, M, NT>), dim3(grid), dim3(NT), 0, st, (vec2_t<T> *)base,
+                       (const TileEntry *)ent, nent, total);
+}
+template <typename T>
+int tile_batch_m(hipStream_t st, uint32_t m, void *base, const void *ent, uint32_t nent, uint64_t total) {
+    switch (m) {
+        case 8: tile_batch_go<T, 8>(st, base, ent, nent, total); break;
+        case 9: tile_batch_go<T, 9>(st, base, ent, nent, total); break;
+        case 10: tile_batch_go<T, 10>(st, base, ent, nent, total); break;
+        case 11: tile_batch_go<T, 11>(st, base, ent, nent, total); break;
+        case 12: tile_batch_go<T, 12>(st, base, ent, nent, total); break;
+        case 13:
+            if (sizeof(T) == 4) {
+                tile_batch_go<T, 13>(st, base, ent, nent, total);
+                break;
+            }
+            return fail(QVMHIP_EINVAL, "tile size 13 is FP32 only");
+        default:
+            return fail(QVMHIP_EINVAL, "tile size m must be 8..13");
+    }
+    QH_LAUNCH_CHECK();
+    return QVMHIP_OK;
+}
+}  // namespace
+
+int launch_tile_batch(hipStream_t st, int dtype, uint32_t m, void *base, const void *entries_dev, uint32_t nent,
+                      uint64_t total_blocks) {
+    return dtype == QVMHIP_F64 ? tile_batch_m<double>(st, m, base, entries_dev, nent, total_blocks)
+                               : tile_batch_m<float>(st, m, base, entries_dev, nent, total_blocks);
 }
 
 }  // namespace qvmhip

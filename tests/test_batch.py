@@ -1,0 +1,65 @@
+"""Batched QAOA runs (one captured graph for many instances) against the
+oracle, instance by instance."""
+import numpy as np
+import pytest
+
+from qvm_amd import circuits as CI
+from qvm_amd import gates as G
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_state(oracle, n, gates):
+    psi = oracle.zero_state(n)
+    for U, q in gates:
+        oracle.apply_matrix(psi, U, G.nat_tuple(*q))
+    return psi
+
+
+def zz_expectation(psi, edges):
+    idx = np.arange(psi.size)
+    p = np.abs(psi) ** 2
+    return sum(float(np.sum(p * (1 - 2 * (((idx >> a) ^ (idx >> b)) & 1)))) for a, b in edges)
+
+
+def test_qaoa_batch_matches_oracle(oracle):
+    from qvm_amd.batch import qaoa_batch
+    inst = CI.qaoa_instances(10, seed=4, nmin=10, nmax=16)
+    shots = 512
+    b, circuits, u = qaoa_batch(inst, "f64", shots=shots, seed=3)
+    b.run()
+    hc, samples = b.results()
+    b.run()  # graph replay: identical results
+    hc2, samples2 = b.results()
+    assert np.array_equal(hc, hc2) and np.array_equal(samples, samples2)
+    for i, (n, edges, gam, bet) in enumerate(inst):
+        print(f"instance {i}: n={n} p={len(gam)} edges={len(edges)}", flush=True)
+        ref = oracle_state(oracle, n, circuits[i])
+        got = b.amplitudes(i)
+        assert np.max(np.abs(got - ref)) < 1e-13
+        assert abs(hc[i] - zz_expectation(ref, edges)) < 1e-12
+        ui = u[i * shots:(i + 1) * shots]
+        want = oracle.sample_multiple(ref, ui)
+        cdf = oracle.cdf(ref)
+        for j in np.nonzero(samples[i] != want)[0]:
+            w = int(want[j])
+            lo = cdf[w - 1] if w else 0.0
+            assert min(abs(ui[j] - cdf[w]), abs(ui[j] - lo)) < 1e-13
+    b.close()
+
+
+def test_qaoa_batch_f32_and_eager(oracle):
+    from qvm_amd.batch import qaoa_batch
+    inst = CI.qaoa_instances(6, seed=5, nmin=10, nmax=14)
+    b64, circuits, _ = qaoa_batch(inst, "f64", shots=0, use_graph=False)
+    b64.run()
+    h64, _ = b64.results()
+    b32, _, _ = qaoa_batch(inst, "f32", shots=0)
+    b32.run()
+    h32, _ = b32.results()
+    assert np.max(np.abs(h64 - h32)) < 1e-4
+    for i, (n, edges, _, _) in enumerate(inst):
+        ref = oracle_state(oracle, n, circuits[i])
+        assert abs(h64[i] - zz_expectation(ref, edges)) < 1e-12
+    b64.close()
+    b32.close()
