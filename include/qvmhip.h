@@ -139,6 +139,10 @@ QVMHIP_API int qvmhip_prob_excited(qvmhip_state *s, uint32_t q, double *p1);
 QVMHIP_API int qvmhip_prob_ground(qvmhip_state *s, uint32_t q, double *p0);
 /* norm: src/wavefunction.lisp:332-346. */
 QVMHIP_API int qvmhip_norm(qvmhip_state *s, double *out);
+/* (psum #'probability state): the sum of |psi_i|^2 before the square root of
+ * norm, as the stochastic Kraus evolution accumulates it
+ * (src/apply-gate.lisp:16-39). */
+QVMHIP_API int qvmhip_total_probability(qvmhip_state *s, double *out);
 /* normalize-wavefunction: src/wavefunction.lisp:349-364 (length <= 0:
  * compute the norm first). */
 QVMHIP_API int qvmhip_normalize(qvmhip_state *s, double length);
@@ -186,6 +190,16 @@ QVMHIP_API int qvmhip_write_amplitudes(qvmhip_state *s, uint64_t start, uint64_t
                                        const double *in);
 /* copy-wavefunction (src/wavefunction.lisp:93-109): dst <- src, same n/dtype. */
 QVMHIP_API int qvmhip_copy(qvmhip_state *dst, const qvmhip_state *src);
+/* Wire formats of the server's octet-stream replies: amplitudes
+ * [start, start+count) as big-endian IEEE doubles re, im (16 octets each,
+ * write-complex-double-float-as-binary, app/src/utilities.lisp:60-75, sent by
+ * the wavefunction request, app/src/handle-request.lisp:132-152), and
+ * probabilities re*re + im*im as big-endian doubles (8 octets each,
+ * write-double-float-as-binary, app/src/utilities.lisp:77-85, sent by the
+ * probabilities request, app/src/handle-request.lisp:154-175).  FP32 states
+ * are widened to double first.  out holds 16*count or 8*count octets. */
+QVMHIP_API int qvmhip_read_wavefunction_be(qvmhip_state *s, uint64_t start, uint64_t count, uint8_t *out);
+QVMHIP_API int qvmhip_read_probabilities_be(qvmhip_state *s, uint64_t start, uint64_t count, uint8_t *out);
 
 /* --- compiled programs (compile-loaded-program / compile-instruction) ---- */
 /* A program records gates, tile passes and fused blocks against one state,

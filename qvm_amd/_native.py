@@ -82,6 +82,7 @@ SIGNATURES = {
     "qvmhip_prob_excited": [_vp, _u32, _dp],
     "qvmhip_prob_ground": [_vp, _u32, _dp],
     "qvmhip_norm": [_vp, _dp],
+    "qvmhip_total_probability": [_vp, _dp],
     "qvmhip_normalize": [_vp, C.c_double],
     "qvmhip_probabilities": [_vp, _u64, _u64, _dp],
     "qvmhip_expectation_zz": [_vp, _u32p, _u32, _dp],
@@ -92,6 +93,8 @@ SIGNATURES = {
     "qvmhip_sample_measure_all": [_vp, _dp, _u64, _u64p],
     "qvmhip_sample": [_vp, _dp, _u64, _u64p],
     "qvmhip_read_amplitudes": [_vp, _u64, _u64, _dp],
+    "qvmhip_read_wavefunction_be": [_vp, _u64, _u64, C.c_void_p],
+    "qvmhip_read_probabilities_be": [_vp, _u64, _u64, C.c_void_p],
     "qvmhip_write_amplitudes": [_vp, _u64, _u64, _dp],
     "qvmhip_copy": [_vp, _vp],
     "qvmhip_program_create": [_vp, C.POINTER(_vp)],

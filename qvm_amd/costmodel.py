@@ -241,6 +241,16 @@ class CostModel:
             json.dump({"din": DIN, "hidden": HIDDEN, "params": self.mlp.params().tolist(),
                        "records": recs or []}, f)
 
+    _shared: "CostModel | None" = None
+
+    @classmethod
+    def shared(cls) -> "CostModel":
+        """The model trained on MI355X launch timings that ships with the
+        package (loaded once per process)."""
+        if cls._shared is None:
+            cls._shared = cls.load(DEFAULT_MODEL)
+        return cls._shared
+
     @classmethod
     def load(cls, path: str) -> "CostModel":
         with open(path) as f:

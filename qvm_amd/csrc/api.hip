@@ -298,6 +298,13 @@ int qvmhip_norm(qvmhip_state *s, double *out) {
     return QVMHIP_OK;
 }
 
+int qvmhip_total_probability(qvmhip_state *s, double *out) {
+    if (!s || !out) return fail(QVMHIP_EINVAL, "null argument");
+    int rc = reduce_norm2(s, s->red + kRedBlocks * 2);
+    if (rc) return rc;
+    return scalar_to_host(s, s->red + kRedBlocks * 2, out);
+}
+
 int qvmhip_normalize(qvmhip_state *s, double length) {
     if (!s) return fail(QVMHIP_EINVAL, "null state");
     double len = length;
@@ -460,28 +467,21 @@ int qvmhip_read_amplitudes(qvmhip_state *s, uint64_t start, uint64_t count, doub
     if (!s || (count && !out)) return fail(QVMHIP_EINVAL, "null argument");
     if (start > s->len || count > s->len - start) return fail(QVMHIP_EINVAL, "range out of bounds");
     if (!count) return QVMHIP_OK;
-    const uint64_t per = s->stage_bytes / 16;  // complex doubles per stage
-    double *wide = nullptr;
-    if (s->dtype == QVMHIP_F32) QH_CHECK(hipMalloc((void **)&wide, per * 16));
-    for (uint64_t off = 0; off < count; off += per) {
-        const uint64_t c = std::min<uint64_t>(per, count - off);
-        const void *src;
-        if (s->dtype == QVMHIP_F64) {
-            src = (const double2 *)s->amps + start + off;
-        } else {
-            int rc = convert_f32_to_f64(s, start + off, c, wide);
-            if (rc) {
-                hipFree(wide);
-                return rc;
-            }
-            src = wide;
-        }
-        QH_CHECK(hipMemcpyAsync(s->stage, src, c * 16, hipMemcpyDeviceToHost, s->stream));
-        QH_CHECK(hipStreamSynchronize(s->stream));
-        std::memcpy(out + 2 * off, s->stage, c * 16);
-    }
-    if (wide) hipFree(wide);
-    return QVMHIP_OK;
+    return export_range(s, start, count, kExportAmps, out);
+}
+
+int qvmhip_read_wavefunction_be(qvmhip_state *s, uint64_t start, uint64_t count, uint8_t *out) {
+    if (!s || (count && !out)) return fail(QVMHIP_EINVAL, "null argument");
+    if (start > s->len || count > s->len - start) return fail(QVMHIP_EINVAL, "range out of bounds");
+    if (!count) return QVMHIP_OK;
+    return export_range(s, start, count, kExportAmpsBE, out);
+}
+
+int qvmhip_read_probabilities_be(qvmhip_state *s, uint64_t start, uint64_t count, uint8_t *out) {
+    if (!s || (count && !out)) return fail(QVMHIP_EINVAL, "null argument");
+    if (start > s->len || count > s->len - start) return fail(QVMHIP_EINVAL, "range out of bounds");
+    if (!count) return QVMHIP_OK;
+    return export_range(s, start, count, kExportProbsBE, out);
 }
 
 int qvmhip_write_amplitudes(qvmhip_state *s, uint64_t start, uint64_t count, const double *in) {

@@ -425,6 +425,26 @@ __global__ __launch_bounds__(kBlock) void k_narrow(const double2 *__restrict__ i
     }
 }
 
+// Host export of a range as 64-bit words: amplitudes as (re, im) doubles or
+// probabilities re*re + im*im, in native or big-endian byte order (the
+// server's octet-stream replies, app/src/utilities.lisp:40-87).
+template <typename T, bool PROBS, bool BE>
+__global__ __launch_bounds__(kBlock) void k_export(const vec2_t<T> *__restrict__ psi, uint64_t start,
+                                                   uint64_t count, uint64_t *__restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += stride) {
+        const vec2_t<T> a = psi[start + i];
+        if constexpr (PROBS) {
+            const uint64_t w = __double_as_longlong(prob<T>(a));
+            out[i] = BE ? __builtin_bswap64(w) : w;
+        } else {
+            const uint64_t wr = __double_as_longlong((double)a.x), wi = __double_as_longlong((double)a.y);
+            out[2 * i] = BE ? __builtin_bswap64(wr) : wr;
+            out[2 * i + 1] = BE ? __builtin_bswap64(wi) : wi;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers.
 
@@ -678,26 +698,94 @@ int inner_product(qvmhip_state *a, qvmhip_state *b, double *re, double *im) {
     return QVMHIP_OK;
 }
 
-int probabilities(qvmhip_state *s, uint64_t start, uint64_t count, double *host_out) {
-    // chunked through the pinned staging buffer
-    const uint64_t per = s->stage_bytes / sizeof(double);
-    double *dev = nullptr;
-    QH_CHECK(hipMalloc((void **)&dev, per * sizeof(double)));
-    for (uint64_t off = 0; off < count; off += per) {
-        const uint64_t c = std::min<uint64_t>(per, count - off);
-        if (s->dtype == QVMHIP_F64)
-            hipLaunchKernelGGL(k_probs<double>, dim3(sweep_grid(c)), dim3(kBlock), 0, s->stream,
-                               (const double2 *)s->amps, start + off, c, dev);
-        else
-            hipLaunchKernelGGL(k_probs<float>, dim3(sweep_grid(c)), dim3(kBlock), 0, s->stream,
-                               (const float2 *)s->amps, start + off, c, dev);
-        QH_LAUNCH_CHECK();
-        QH_CHECK(hipMemcpyAsync(s->stage, dev, c * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-        QH_CHECK(hipStreamSynchronize(s->stream));
-        std::memcpy(host_out + off, s->stage, c * sizeof(double));
+namespace {
+template <typename T>
+void launch_export(qvmhip_state *s, int mode, uint64_t start, uint64_t count, uint64_t *dev) {
+    const vec2_t<T> *psi = (const vec2_t<T> *)s->amps;
+    const dim3 grid(sweep_grid(count)), block(kBlock);
+    switch (mode) {
+        case kExportAmps: hipLaunchKernelGGL((k_export<T, false, false>), grid, block, 0, s->stream, psi, start, count, dev); break;
+        case kExportAmpsBE: hipLaunchKernelGGL((k_export<T, false, true>), grid, block, 0, s->stream, psi, start, count, dev); break;
+        case kExportProbs: hipLaunchKernelGGL((k_export<T, true, false>), grid, block, 0, s->stream, psi, start, count, dev); break;
+        default: hipLaunchKernelGGL((k_export<T, true, true>), grid, block, 0, s->stream, psi, start, count, dev); break;
     }
-    hipFree(dev);
-    return QVMHIP_OK;
+}
+}  // namespace
+
+// Chunked device -> host export.  The pinned stage is split in two halves;
+// chunk i is converted on the device and copied into half i%2 while the host
+// copies chunk i-1 out of the other half.  FP64 amplitudes in native order
+// need no conversion and are copied straight from the state.
+int export_range(qvmhip_state *s, uint64_t start, uint64_t count, int mode, void *host_out) {
+    const size_t eb = (mode == kExportAmps || mode == kExportAmpsBE) ? 16 : 8;
+    const bool direct = mode == kExportAmps && s->dtype == QVMHIP_F64;
+    const uint64_t per = (s->stage_bytes / 2) / eb;
+    uint64_t *dev[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int rc = QVMHIP_OK;
+    auto cleanup = [&]() {
+        for (int h = 0; h < 2; ++h) {
+            if (dev[h]) hipFree(dev[h]);
+            if (ev[h]) hipEventDestroy(ev[h]);
+        }
+    };
+    for (int h = 0; h < 2; ++h) {
+        hipError_t e = hipEventCreateWithFlags(&ev[h], hipEventDisableTiming);
+        if (e == hipSuccess && !direct && count > (uint64_t)h * per) e = hipMalloc((void **)&dev[h], per * eb);
+        if (e != hipSuccess) {
+            cleanup();
+            return fail(e == hipErrorOutOfMemory ? QVMHIP_ENOMEM : QVMHIP_EDEVICE,
+                        std::string("export buffers: ") + hipGetErrorString(e));
+        }
+    }
+    char *out = (char *)host_out;
+    char *half[2] = {(char *)s->stage, (char *)s->stage + per * eb};
+    const uint64_t nchunks = (count + per - 1) / per;
+    for (uint64_t i = 0; i <= nchunks && rc == QVMHIP_OK; ++i) {
+        if (i < nchunks) {
+            const int h = (int)(i & 1);
+            const uint64_t off = i * per, c = std::min<uint64_t>(per, count - off);
+            const void *src;
+            if (direct) {
+                src = (const double2 *)s->amps + start + off;
+            } else {
+                if (s->dtype == QVMHIP_F64)
+                    launch_export<double>(s, mode, start + off, c, dev[h]);
+                else
+                    launch_export<float>(s, mode, start + off, c, dev[h]);
+                hipError_t e = hipGetLastError();
+                if (e != hipSuccess) {
+                    rc = fail(QVMHIP_EDEVICE, std::string("export kernel: ") + hipGetErrorString(e));
+                    break;
+                }
+                src = dev[h];
+            }
+            hipError_t e = hipMemcpyAsync(half[h], src, c * eb, hipMemcpyDeviceToHost, s->stream);
+            if (e == hipSuccess) e = hipEventRecord(ev[h], s->stream);
+            if (e != hipSuccess) {
+                rc = fail(QVMHIP_EDEVICE, std::string("export copy: ") + hipGetErrorString(e));
+                break;
+            }
+        }
+        if (i >= 1) {
+            const uint64_t j = i - 1;
+            const int h = (int)(j & 1);
+            const uint64_t off = j * per, c = std::min<uint64_t>(per, count - off);
+            hipError_t e = hipEventSynchronize(ev[h]);
+            if (e != hipSuccess) {
+                rc = fail(QVMHIP_EDEVICE, std::string("export wait: ") + hipGetErrorString(e));
+                break;
+            }
+            std::memcpy(out + off * eb, half[h], c * eb);
+        }
+    }
+    if (rc != QVMHIP_OK) hipStreamSynchronize(s->stream);  // no copy may still target the stage
+    cleanup();
+    return rc;
+}
+
+int probabilities(qvmhip_state *s, uint64_t start, uint64_t count, double *host_out) {
+    return export_range(s, start, count, kExportProbs, host_out);
 }
 
 int convert_f32_to_f64(qvmhip_state *s, uint64_t start, uint64_t count, double *dev_out) {

@@ -381,6 +381,11 @@ def compile_gates(state, gates: List[Gate], mode: str = "tiled", use_graph: bool
                           reorder)
     elif mode == "fused":
         plan = plan_fused(gates, nq, kmax)
+    elif mode == "auto":
+        # the candidate plan the trained launch-cost model prices lowest
+        from . import costmodel as CM
+        cm = CM.CostModel.shared()
+        _, plan, _ = cm.choose(gates, nq, state.dtype)
     else:
         raise ValueError(f"unknown plan mode {mode!r}")
     return CompiledProgram(build_program(state, plan, use_graph), plan, mode)

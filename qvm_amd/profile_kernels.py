@@ -23,7 +23,8 @@ def main(argv=None) -> int:
     ap.add_argument("--n", type=int, default=30)
     ap.add_argument("--dtype", default="f64")
     ap.add_argument("--kernel", default="dense1",
-                    choices=["dense1", "diag_full", "perm", "diag_sparse", "fused4", "fused5", "tile"])
+                    choices=["dense1", "diag_full", "perm", "diag_sparse", "fused3", "fused4", "fused5", "fused6",
+                             "tile"])
     ap.add_argument("--reps", type=int, default=8)
     a = ap.parse_args(argv)
     st = HipPureState(a.n, a.dtype)
@@ -38,7 +39,7 @@ def main(argv=None) -> int:
             st.apply_matrix(G.gate_matrix("CNOT"), [q, (q + 7) % a.n])
         elif a.kernel == "diag_sparse":
             st.apply_matrix(G.gate_matrix("CPHASE", [0.1 + r]), [q, (q + 7) % a.n])
-        elif a.kernel in ("fused4", "fused5"):
+        elif a.kernel.startswith("fused"):
             k = int(a.kernel[-1])
             Z = rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k))
             st.apply_fused(np.linalg.qr(Z)[0], [int(x) for x in rng.choice(a.n, k, replace=False)])

@@ -12,7 +12,11 @@ operations (names in snake_case):
     measure             src/measurement.lisp:87-105
     measure_all         src/measurement.lisp:122-143
     amplitudes          src/qvm.lisp:63-82
-    compile_loaded_program  src/qvm.lisp:166-175 (plain / tiled / fused plans)
+    compile_loaded_program  src/qvm.lisp:166-175 (plain / tiled / fused / auto
+                        plans; measure chains -> measure-all,
+                        src/compile-gate.lisp:538-599)
+    perform_expectation app/src/api/expectation.lisp:78-89 (pure state)
+    wavefunction / probabilities octets  app/src/handle-request.lisp:132-175
 
 Random draws come from ``qvm.rng`` (a callable returning uniforms in [0,1));
 the reference draws from mt19937 (src/utilities.lisp:306-335), which is not
@@ -20,7 +24,8 @@ vendored, so callers that need a reproducible stream pass their own.
 """
 from __future__ import annotations
 
-from typing import Callable, Optional
+from dataclasses import dataclass
+from typing import Callable, List, Optional, Tuple
 
 import numpy as np
 
@@ -31,6 +36,52 @@ from .state import HipPureState, Program
 
 class QVMError(RuntimeError):
     pass
+
+
+@dataclass
+class MeasureAll:
+    """measure-all pseudo-instruction (src/compile-gate.lisp:540-545):
+    storage lists (qubit, (name, index)) for the MEASUREs of the chain that
+    store their bit."""
+    storage: List[Tuple[int, Tuple[str, int]]]
+
+
+def _measure_chain_at(i: int, code) -> int:
+    """Length of the run of measurements starting at i
+    (measure-chain-at, src/compile-gate.lisp:554-563)."""
+    n = 0
+    for ins in code[i:]:
+        if not isinstance(ins, Q.Measure):
+            break
+        n += 1
+    return n
+
+
+def compile_measure_chains(code, n: int) -> list:
+    """Replace every run of MEASUREs that covers all qubits 0..n-1 by one
+    MeasureAll followed by NOPs for the rest of the run
+    (find-measure-chains / compile-measure-chains,
+    src/compile-gate.lisp:565-599)."""
+    code = list(code)
+    i = 0
+    chains = []
+    while i < len(code):
+        c = _measure_chain_at(i, code)
+        if c > 0:
+            if c >= n:
+                bits = 0
+                for m in code[i:i + c]:
+                    bits |= 1 << m.qubit
+                if bits == (1 << n) - 1:
+                    chains.append((i, code[i:i + c]))
+            i += c
+        else:
+            i += 1
+    for start, chain in chains:
+        for k in range(start, start + len(chain)):
+            code[k] = Q.Nop()
+        code[start] = MeasureAll([(m.qubit, m.target) for m in chain if m.target is not None])
+    return code
 
 
 class _UniformStream:
@@ -90,7 +141,7 @@ class PureStateQVM:
         if self.program is None:
             raise QVMError("no program loaded")
         if self._compiled is not None:
-            self._compiled.run()
+            self._compiled.run(self)
             self.pc = len(self.program.executable_code)
             return self
         code = self.program.executable_code
@@ -125,6 +176,12 @@ class PureStateQVM:
                 _, bit = self.measure(ins.qubit)
                 if bit == 1:
                     apply_gate_to_state(self.state, G.gate_matrix("X"), [ins.qubit])
+            return self.pc + 1
+        if isinstance(ins, MeasureAll):
+            # transition on measure-all (src/transition.lisp:130-135)
+            _, bits = self.measure_all()
+            for q, (name, idx) in ins.storage:
+                self.memory[name][idx] = bits[q]
             return self.pc + 1
         if isinstance(ins, Q.Halt):
             return None
@@ -167,24 +224,55 @@ class PureStateQVM:
         return complex(self.state.read_amplitudes(i, 1)[0])
 
     # -- compilation -----------------------------------------------------
-    def compile_loaded_program(self, mode: str = "tiled", use_graph: bool = True, **kw) -> "CompiledProgram":
-        """Plan the loaded gate-only program into engine launches and capture
-        them (plain: one launch per gate; tiled: LDS tile passes; fused:
-        dense MFMA blocks)."""
+    def compile_loaded_program(self, mode: str = "tiled", use_graph: bool = True,
+                               measure_chains: bool = True, **kw) -> "CompiledProgram":
+        """compile-loaded-program (src/qvm.lisp:166-175).  Measure chains over
+        all qubits become one measure-all (*compile-measure-chains*); every
+        maximal run of gate applications is planned into engine launches
+        (plain: one launch per gate; tiled: LDS tile passes; fused: dense
+        MFMA blocks; auto: the plan the trained launch-cost model prices
+        lowest) and captured; the other instructions run through
+        ``transition``.  Gate parameters are evaluated against classical
+        memory when the program is compiled."""
         from . import planner
         if self.program is None:
             raise QVMError("no program loaded")
-        gates = []
-        for ins in self.program.executable_code:
+        code = self.program.executable_code
+        if measure_chains:
+            code = compile_measure_chains(code, self.number_of_qubits)
+        steps = []
+        run: list = []
+
+        def flush():
+            if run:
+                steps.append(planner.compile_gates(self.state, list(run), mode=mode, use_graph=use_graph, **kw))
+                run.clear()
+
+        for ins in code:
             if isinstance(ins, Q.GateApplication):
                 U = Q.resolve_gate_matrix(ins, self.program.gate_definitions, self.memory)
-                gates.append((U, list(ins.qubits)))
-            elif isinstance(ins, (Q.Nop, Q.Halt)):
+                if G.num_qubits_of(U) != len(ins.qubits):
+                    raise QVMError(f"Gate {ins.name} takes {G.num_qubits_of(U)} qubit(s), "
+                                   f"applied to {len(ins.qubits)}")
+                run.append((U, list(ins.qubits)))
+            elif isinstance(ins, Q.Nop):
                 continue
             else:
-                raise QVMError("compile_loaded_program handles gate-only programs")
-        self._compiled = planner.compile_gates(self.state, gates, mode=mode, use_graph=use_graph, **kw)
+                flush()
+                steps.append(ins)
+        flush()
+        self._compiled = CompiledProgram.from_steps(steps, mode)
         return self._compiled
+
+    # -- wire formats (app/src/handle-request.lisp:132-175) ---------------
+    def wavefunction_octets(self) -> bytes:
+        """The wavefunction reply: every amplitude as big-endian re, im
+        doubles (16 octets each, octets-required-for-quantum-state)."""
+        return self.state.wavefunction_octets()
+
+    def probabilities_octets(self) -> bytes:
+        """The probabilities reply: |psi_i|^2 as big-endian doubles."""
+        return self.state.probabilities_octets()
 
     def close(self) -> None:
         self.state.close()
@@ -208,12 +296,69 @@ def apply_gate_to_state(state: HipPureState, U: np.ndarray, quil_qubits) -> None
 
 
 class CompiledProgram:
-    """Handle returned by compile_loaded_program."""
+    """Handle returned by compile_loaded_program: captured gate programs and
+    the non-gate instructions between them, in program order."""
 
-    def __init__(self, program: Program, plan: list, mode: str):
+    def __init__(self, program: Optional[Program], plan: list, mode: str):
         self.program = program
         self.plan = plan
         self.mode = mode
+        self.steps: list = [self] if program is not None else []
 
-    def run(self) -> None:
-        self.program.run()
+    @classmethod
+    def from_steps(cls, steps: list, mode: str) -> "CompiledProgram":
+        cp = cls(None, [], mode)
+        cp.steps = steps
+        cp.plan = [it for st in steps if isinstance(st, CompiledProgram) for it in st.plan]
+        return cp
+
+    def run(self, qvm: Optional["PureStateQVM"] = None) -> None:
+        if self.program is not None:
+            self.program.run()
+            return
+        for st in self.steps:
+            if isinstance(st, CompiledProgram):
+                st.run()
+            else:
+                if qvm is None:
+                    raise QVMError("this compiled program has non-gate instructions; run it through its QVM")
+                if qvm.transition(st) is None:
+                    return  # HALT
+
+    def close(self) -> None:
+        if self.program is not None:
+            self.program.close()
+        for st in self.steps:
+            if isinstance(st, CompiledProgram) and st is not self:
+                st.close()
+
+
+def perform_expectation(state_prep, operators, num_qubits: int, dtype: str = "f64", **kw) -> List[float]:
+    """Expectation values <prep|O|prep> of operator programs on one prepared
+    state (pure-state branch of perform-expectation,
+    app/src/api/expectation.lisp:42-89): run the state preparation, keep a
+    device copy of the prepared state, and for each operator program restore
+    the copy, run the program and take the inner product with the prepared
+    state (a device reduction).  Returns the real parts; the reference
+    asserts |imag| < 1e-14."""
+    q = make_qvm(num_qubits, dtype=dtype, **kw)
+    q.load_program(state_prep)
+    q.run()
+    prepared = HipPureState(num_qubits, dtype=dtype)
+    prepared.copy_from(q.state)
+    out = []
+    tol = 1e-14 if dtype == "f64" else 1e-6
+    try:
+        for i, op in enumerate(operators):
+            if i > 0:
+                q.state.copy_from(prepared)
+            q.load_program(op)
+            q.run()
+            v = prepared.inner_product(q.state)
+            if abs(v.imag) >= tol:
+                raise QVMError(f"expectation value with imaginary part {v.imag!r}")
+            out.append(float(v.real))
+    finally:
+        prepared.close()
+        q.close()
+    return out

@@ -124,6 +124,12 @@ class HipPureState:
         N.check(N.lib().qvmhip_norm(self._h, C.byref(out)))
         return out.value
 
+    def total_probability(self) -> float:
+        """sum_i |psi_i|^2 (no square root)."""
+        out = C.c_double()
+        N.check(N.lib().qvmhip_total_probability(self._h, C.byref(out)))
+        return out.value
+
     def normalize(self, length: float = 0.0) -> None:
         N.check(N.lib().qvmhip_normalize(self._h, float(length)))
 
@@ -182,6 +188,22 @@ class HipPureState:
         N.check(N.lib().qvmhip_read_amplitudes(self._h, start, count, N.dptr(out.view(np.float64))))
         return out
 
+    def wavefunction_octets(self, start: int = 0, count: int | None = None) -> bytes:
+        """Amplitudes as the server's big-endian octet stream: re, im doubles,
+        16 octets per amplitude (app/src/utilities.lisp:60-75)."""
+        count = self.length - start if count is None else count
+        out = np.empty(16 * count, dtype=np.uint8)
+        N.check(N.lib().qvmhip_read_wavefunction_be(self._h, start, count, out.ctypes.data_as(C.c_void_p)))
+        return out.tobytes()
+
+    def probabilities_octets(self, start: int = 0, count: int | None = None) -> bytes:
+        """|psi_i|^2 as big-endian doubles, 8 octets each (the probabilities
+        reply, app/src/handle-request.lisp:154-175)."""
+        count = self.length - start if count is None else count
+        out = np.empty(8 * count, dtype=np.uint8)
+        N.check(N.lib().qvmhip_read_probabilities_be(self._h, start, count, out.ctypes.data_as(C.c_void_p)))
+        return out.tobytes()
+
     def write_amplitudes(self, amps, start: int = 0) -> None:
         a = np.ascontiguousarray(np.asarray(amps, dtype=np.complex128))
         N.check(N.lib().qvmhip_write_amplitudes(self._h, start, a.size, N.dptr(a.view(np.float64))))
@@ -218,6 +240,15 @@ class HipPureState:
     def copy_from(self, other: "HipPureState") -> None:
         """copy-wavefunction (src/wavefunction.lisp:93-109)."""
         N.check(N.lib().qvmhip_copy(self._h, other._h))
+
+    def swap_contents(self, other: "HipPureState") -> None:
+        """Exchange the device buffers of two states of one size and dtype
+        (the rotatef of the state and trial amplitude vectors,
+        src/apply-gate.lisp:37).  Programs built on either state keep the
+        buffer they were built on."""
+        if other.num_qubits != self.num_qubits or other.dtype != self.dtype:
+            raise ValueError("states differ in size or dtype")
+        self._h, other._h = other._h, self._h
 
 
 class Program:

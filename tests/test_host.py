@@ -230,3 +230,51 @@ def test_error_paths_without_a_device():
     assert nbytes.value >= 16 << 30
     assert L.qvmhip_apply_matrix(None, None, 1, None) == N.EINVAL
     assert L.qvmhip_measure(None, 0, 0.5, 0, None, None) == N.EINVAL
+    assert L.qvmhip_read_wavefunction_be(None, 0, 1, None) == N.EINVAL
+    assert L.qvmhip_total_probability(None, None) == N.EINVAL
+
+
+# --- compiled measure chains and noise helpers (host logic) ------------------
+
+def test_measure_chains_compile_to_measure_all():
+    from qvm_amd import qvm as QV
+    p = Q.parse_quil("DECLARE ro BIT[3]\nH 0\nMEASURE 2 ro[2]\nMEASURE 0\nMEASURE 1 ro[0]\nX 1\n"
+                     "MEASURE 0 ro[1]\nMEASURE 1 ro[2]")
+    code = QV.compile_measure_chains(p.executable_code, 3)
+    # the first run covers qubits {0, 1, 2}: one MeasureAll then NOPs; the
+    # second run covers only {0, 1} and stays as it is
+    assert isinstance(code[1], QV.MeasureAll)
+    assert code[1].storage == [(2, ("ro", 2)), (1, ("ro", 0))]  # MEASURE 0 (discard) stores nothing
+    assert isinstance(code[2], Q.Nop) and isinstance(code[3], Q.Nop)
+    assert isinstance(code[5], Q.Measure) and isinstance(code[6], Q.Measure)
+    # a run longer than n that still covers every qubit is one chain
+    p2 = Q.parse_quil("MEASURE 0\nMEASURE 1\nMEASURE 0")
+    c2 = QV.compile_measure_chains(p2.executable_code, 2)
+    assert isinstance(c2[0], QV.MeasureAll) and all(isinstance(x, Q.Nop) for x in c2[1:])
+    # too short: untouched
+    assert QV.compile_measure_chains(p2.executable_code[:1], 2)[0] == p2.executable_code[0]
+
+
+def test_noise_helpers():
+    from qvm_amd import noise as NZ
+    ops = NZ.make_pauli_noise_map(0.1, 0.0, 0.25)
+    assert len(ops) == 4
+    NZ.check_kraus_ops(ops)
+    np.testing.assert_allclose(ops[0], np.sqrt(0.65) * np.eye(2))
+    np.testing.assert_allclose(ops[3], np.sqrt(0.25) * G.gate_matrix("Z"))
+    assert len(NZ.make_pauli_noise_map(0.5, 0.25, 0.25)) == 3  # no identity term at sum 1
+    g = NZ.make_pauli_perturbed_1q_gate("H", 0.1, 0.1, 0.1)
+    NZ.check_kraus_ops(g)
+    np.testing.assert_allclose(g[1], np.sqrt(0.1) * G.gate_matrix("X") @ G.gate_matrix("H"))
+    with pytest.raises(Exception):
+        NZ.check_kraus_ops([np.eye(2), np.eye(2)])
+    with pytest.raises(Exception):
+        NZ.make_pauli_noise_map(0.6, 0.6, 0.0)
+    NZ.check_povm((0.9, 0.2, 0.1, 0.8))
+    with pytest.raises(Exception):
+        NZ.check_povm((0.9, 0.2, 0.2, 0.8))
+    # perturb-measurement (src/channel-qvm.lisp:185-195): 0 iff r <= p(0|actual)
+    assert NZ.perturb_measurement(0, 0.9, 0.2, 0.1, 0.8, 0.9) == 0
+    assert NZ.perturb_measurement(0, 0.9, 0.2, 0.1, 0.8, 0.91) == 1
+    assert NZ.perturb_measurement(1, 0.9, 0.2, 0.1, 0.8, 0.2) == 0
+    assert NZ.perturb_measurement(1, 0.9, 0.2, 0.1, 0.8, 0.21) == 1
