@@ -177,6 +177,7 @@ def main() -> int:
     st.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n, ga, args.cpu_seconds, args.cpu_threads)
+        out["config1"] = config1(args.cpu_threads, HipPureState)
     print(json.dumps(out))
     return 0
 
@@ -240,10 +241,13 @@ def run_paths(args, n, ga, gb, nat_a, upd_a, upd_b, u, st, HipPureState, P, np):
     res["tiled_exact_f64"] = record("tiled_exact_f64", s2, P.plan_tiled(ga, n, m64, 4, reorder=False),
                                     P.plan_tiled(gb, n, m64, 4, reorder=False))
     res["tiled_exact_f64"]["bit_identical_to_plain"] = res["tiled_exact_f64"]["max_abs_diff_vs_plain_f64"] == 0.0
-    res["tiled_f64"] = record("tiled_f64", s2, P.plan_tiled(ga, n, m64, 4), P.plan_tiled(gb, n, m64, 4))
     res["fused_f64"] = record("fused_f64", s2, P.plan_fused(ga, n, 4), P.plan_fused(gb, n, 4))
     s2.close()
-    s3 = HipPureState(n, "f32")
+    s2 = HipPureState(n, "f64", fma_tiles=True)  # reordered passes, FMA-contracted
+    res["tiled_f64"] = record("tiled_f64", s2, P.plan_tiled(ga, n, m64, 4, merge_1q=True),
+                              P.plan_tiled(gb, n, m64, 4, merge_1q=True))
+    s2.close()
+    s3 = HipPureState(n, "f32", fma_tiles=True)
     m32 = P.default_tile_m("f32")
     res["tiled_f32"] = record("tiled_f32", s3, P.plan_tiled(ga, n, m32, 5, merge_1q=True),
                               P.plan_tiled(gb, n, m32, 5, merge_1q=True))
@@ -491,6 +495,55 @@ def cpu_baseline(n, ga, seconds, threads):
     return {"value": upd / dt, "unit": "amp-updates/s", "cores": threads, "kind": "port",
             "sample": f"first {done} gates of the {n}-qubit layered circuit, FP64, compiled mode "
                       f"(pdotimes over complement groups), {threads} threads, {dt:.1f} s"}
+
+
+def config1(threads, HipPureState):
+    """The CPU-runnable configuration in full on both sides: 20 qubits, 20
+    layered RX/RZ+CNOT layers (990 gates), then 10^4 measure-all shots of the
+    prepared state (rule C(b) > p, the state reloaded per shot).  The GPU runs
+    the plain path and the batched shot kernel; the oracle runs compiled mode
+    on `threads` threads for the gates and the reference's bisection sampler
+    for a bounded sample of the shots (scaled to 10^4)."""
+    from oracle import oracle as O
+    from qvm_amd import circuits as C
+    from qvm_amd import gates as G
+    import numpy as np
+    n, shots = 20, 10_000
+    gates = C.layered(n, 20, seed=1)
+    nat = [(U, G.nat_tuple(*q)) for U, q in gates]
+    u = C.uniforms(7, shots)
+    st = HipPureState(n, "f64")
+    st.sync()
+    t0 = time.perf_counter()
+    for U, q in nat:
+        st.apply_matrix(U, q)
+    st.sync()
+    g_gates = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    gshots = st.sample_measure_all(u)
+    g_shots = time.perf_counter() - t0
+    amps = st.amplitudes()
+    st.close()
+    O.set_parallelization(19, threads)
+    psi = O.zero_state(n)
+    t0 = time.perf_counter()
+    for U, q in nat:
+        O.apply_compiled(psi, U, q)
+    c_gates = time.perf_counter() - t0
+    k, t0 = 0, time.perf_counter()
+    mism = 0
+    while k < shots and time.perf_counter() - t0 < 5.0:
+        b = O.sample_bisect(psi, float(u[k]))
+        mism += int(b != int(gshots[k]))
+        k += 1
+    c_shot = (time.perf_counter() - t0) / max(k, 1)
+    upd = C.amp_updates(gates, n)
+    return {"n_qubits": n, "gates": len(gates), "shots": shots, "amp_updates": upd,
+            "gpu_s_gates": g_gates, "gpu_s_shots": g_shots, "gpu_amp_updates_per_s": upd / g_gates,
+            "cpu_s_gates": c_gates, "cpu_s_shots_scaled": c_shot * shots, "cpu_threads": threads,
+            "cpu_shots_timed": k, "cpu_amp_updates_per_s": upd / c_gates,
+            "state_max_abs_diff_gpu_vs_oracle": float(np.max(np.abs(amps - psi))),
+            "shots_differing_in_timed_sample": mism}
 
 
 if __name__ == "__main__":

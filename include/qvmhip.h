@@ -69,11 +69,17 @@ typedef struct {
     double m[32];
 } qvmhip_tile_op;
 
+/* qvmhip_config.flags: tile passes on this state contract their complex
+ * products with fused multiply-adds (fewer roundings and instructions; the
+ * results are then no longer bit-identical to gate-by-gate application).
+ * The plain gate kernels always keep the reference's rounding. */
+#define QVMHIP_FLAG_FMA_TILES 1u
+
 typedef struct {
     uint32_t struct_size;      /* sizeof(qvmhip_config)                       */
     int32_t device;            /* HIP device ordinal; -1 = current device      */
     uint64_t memory_cap_bytes; /* refuse states larger than this; 0 = no cap   */
-    uint32_t flags;            /* reserved, 0                                  */
+    uint32_t flags;            /* QVMHIP_FLAG_* bits, 0 for the defaults       */
 } qvmhip_config;
 
 typedef struct qvmhip_state qvmhip_state;
@@ -126,11 +132,18 @@ QVMHIP_API int qvmhip_apply_permutation(qvmhip_state *s, const uint32_t *perm, u
 QVMHIP_API int qvmhip_apply_fused(qvmhip_state *s, const double *U, uint32_t k,
                                   const uint32_t *qubits);
 /* Apply a sequence of small gates to every 2^m-amplitude block spanned by
- * tile_qubits (ascending, m <= 12 for F64, 13 for F32) with the block
- * resident in LDS: one HBM sweep for the whole sequence.  The gate-by-gate
- * arithmetic is the same as qvmhip_apply_matrix. */
+ * tile_qubits (ascending, 8 <= m <= 12 for F64, 13 for F32) with the block
+ * resident on chip (registers, re-laid out through LDS when a target needs
+ * it): one HBM sweep for the whole sequence.  The gate-by-gate arithmetic is
+ * the same as qvmhip_apply_matrix, so an in-order pass is bit-identical. */
 QVMHIP_API int qvmhip_apply_tile_pass(qvmhip_state *s, uint32_t m, const uint32_t *tile_qubits,
                                       uint32_t nops, const qvmhip_tile_op *ops);
+/* The slot-space program a tile pass of width m runs for dtype (the op list
+ * with register relayouts inserted; no counterpart in the reference -- it
+ * exposes the tile scheduler for inspection and tests).  Writes up to cap ops
+ * to out and the full count to *nout; QVMHIP_ERANGE when cap is too small. */
+QVMHIP_API int qvmhip_tile_schedule(int dtype, uint32_t m, uint32_t nops, const qvmhip_tile_op *ops,
+                                    qvmhip_tile_op *out, uint32_t cap, uint32_t *nout);
 
 /* --- reductions --------------------------------------------------------- */
 /* wavefunction-excited-state-probability: src/wavefunction.lisp:62-70. */

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "_lib", "libqvmhip.so")
 
 F64 = 0
 F32 = 1
+FLAG_FMA_TILES = 1  # qvmhip_config.flags
 
 OK = 0
 EINVAL = -1
@@ -32,6 +33,10 @@ TILE_DIAG2 = 4
 TILE_CPERM = 5
 TILE_SWAP = 6
 TILE_CPHASE = 7
+# slot-space kinds emitted by the tile scheduler (qvmhip_tile_schedule)
+TILE_XREG = 16
+TILE_RSWAP = 17
+TILE_REMAP = 18
 
 
 class QVMHipError(RuntimeError):
@@ -79,6 +84,7 @@ SIGNATURES = {
     "qvmhip_apply_permutation": [_vp, _u32p, _u32, _u32p],
     "qvmhip_apply_fused": [_vp, _dp, _u32, _u32p],
     "qvmhip_apply_tile_pass": [_vp, _u32, _u32p, _u32, C.POINTER(TileOp)],
+    "qvmhip_tile_schedule": [C.c_int, _u32, _u32, C.POINTER(TileOp), C.POINTER(TileOp), _u32, _u32p],
     "qvmhip_prob_excited": [_vp, _u32, _dp],
     "qvmhip_prob_ground": [_vp, _u32, _dp],
     "qvmhip_norm": [_vp, _dp],
@@ -196,3 +202,14 @@ def fptr(a: np.ndarray):
 
 def cmat(U) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(U, dtype=np.complex128))
+
+
+def tile_schedule(dtype: str, m: int, ops) -> list:
+    """The slot-space program the engine runs for a tile pass (see
+    qvmhip_tile_schedule)."""
+    arr = (TileOp * max(1, len(ops)))(*ops)
+    cap = 4 * len(ops) + 8
+    out = (TileOp * cap)()
+    n = C.c_uint32()
+    check(lib().qvmhip_tile_schedule(F64 if dtype == "f64" else F32, m, len(ops), arr, out, cap, C.byref(n)))
+    return [out[i] for i in range(n.value)]

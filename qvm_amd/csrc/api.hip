@@ -120,6 +120,7 @@ int qvmhip_create(uint32_t n, int dtype, const qvmhip_config *cfg, qvmhip_state 
     if (rc) return rc;
     if (cfg && cfg->struct_size != sizeof(qvmhip_config))
         return fail(QVMHIP_EINVAL, "qvmhip_config.struct_size mismatch");
+    if (cfg && (cfg->flags & ~QVMHIP_FLAG_FMA_TILES)) return fail(QVMHIP_EINVAL, "unknown config flags");
     if (cfg && cfg->memory_cap_bytes && need > cfg->memory_cap_bytes)
         return fail(QVMHIP_ENOMEM, "state needs " + std::to_string(need) + " bytes, cap is " +
                                        std::to_string(cfg->memory_cap_bytes));
@@ -137,6 +138,7 @@ int qvmhip_create(uint32_t n, int dtype, const qvmhip_config *cfg, qvmhip_state 
     s->n = n;
     s->dtype = dtype;
     s->device = dev;
+    s->flags = cfg ? cfg->flags : 0u;
     s->len = uint64_t(1) << n;
     s->amp_bytes = dtype == QVMHIP_F64 ? 16 : 8;
     s->nchunks = std::max<uint64_t>(1, s->len >> kChunkLog2);
@@ -256,19 +258,26 @@ int qvmhip_apply_tile_pass(qvmhip_state *s, uint32_t m, const uint32_t *tile_qub
         if (tile_qubits[i] >= s->n) return fail(QVMHIP_EINVAL, "tile qubit out of range");
         if (i && tile_qubits[i] <= tile_qubits[i - 1]) return fail(QVMHIP_EINVAL, "tile qubits must ascend");
     }
-    for (uint32_t o = 0; o < nops; ++o) {
-        const qvmhip_tile_op &op = ops[o];
-        if (op.kind < QVMHIP_TILE_DENSE1 || op.kind > QVMHIP_TILE_CPHASE)
-            return fail(QVMHIP_EINVAL, "unknown tile op kind");
-        const bool two = op.kind != QVMHIP_TILE_DENSE1 && op.kind != QVMHIP_TILE_DIAG1;
-        if (op.q0 >= m || (two && (op.q1 >= m || op.q1 == op.q0)))
-            return fail(QVMHIP_EINVAL, "tile op qubit out of range");
-        if (op.ctrl_mask >> m) return fail(QVMHIP_EINVAL, "tile op control outside the tile");
-        if ((op.ctrl_mask >> op.q0) & 1u || (two && ((op.ctrl_mask >> op.q1) & 1u)))
-            return fail(QVMHIP_EINVAL, "tile op control overlaps its targets");
-    }
+    int rc = check_tile_ops(m, nops, ops);
+    if (rc) return rc;
     if (nops == 0) return QVMHIP_OK;
     return launch_tile_pass(s, m, tile_qubits, nops, ops, 0);
+}
+
+int qvmhip_tile_schedule(int dtype, uint32_t m, uint32_t nops, const qvmhip_tile_op *ops, qvmhip_tile_op *out,
+                         uint32_t cap, uint32_t *nout) {
+    if (!nout || (nops && !ops) || (cap && !out)) return fail(QVMHIP_EINVAL, "null argument");
+    if (dtype != QVMHIP_F64 && dtype != QVMHIP_F32) return fail(QVMHIP_EINVAL, "unknown dtype");
+    if (m < 8 || m > 13 || (m == 13 && dtype == QVMHIP_F64))
+        return fail(QVMHIP_EINVAL, "tile size m must be 8..12 (F64) or 8..13 (F32)");
+    int rc = check_tile_ops(m, nops, ops);
+    if (rc) return rc;
+    std::vector<qvmhip_tile_op> sched;
+    schedule_tile_ops(dtype, m, ops, nops, sched);
+    *nout = (uint32_t)sched.size();
+    if (sched.size() > cap) return fail(QVMHIP_ERANGE, "output capacity too small");
+    std::copy(sched.begin(), sched.end(), out);
+    return QVMHIP_OK;
 }
 
 int qvmhip_prob_excited(qvmhip_state *s, uint32_t q, double *p1) {

@@ -244,21 +244,17 @@ int qvmhip_batch_add_tile_pass(qvmhip_batch *b, uint32_t inst, uint32_t m, const
         if (tile_qubits[i] >= n) return fail(QVMHIP_EINVAL, "tile qubit out of range");
         if (i && tile_qubits[i] <= tile_qubits[i - 1]) return fail(QVMHIP_EINVAL, "tile qubits must ascend");
     }
-    for (uint32_t o = 0; o < nops; ++o) {
-        const qvmhip_tile_op &op = ops[o];
-        if (op.kind < QVMHIP_TILE_DENSE1 || op.kind > QVMHIP_TILE_CPHASE)
-            return fail(QVMHIP_EINVAL, "unknown tile op kind");
-        const bool two = op.kind != QVMHIP_TILE_DENSE1 && op.kind != QVMHIP_TILE_DIAG1;
-        if (op.q0 >= m || (two && (op.q1 >= m || op.q1 == op.q0)) || (op.ctrl_mask >> m))
-            return fail(QVMHIP_EINVAL, "tile op qubit out of range");
-    }
+    rc = check_tile_ops(m, nops, ops);
+    if (rc) return rc;
     if (!nops) return QVMHIP_OK;
     BItem it;
     it.kind = kItemTile;
     it.k = m;
     std::copy(tile_qubits, tile_qubits + m, it.qubits);
-    it.nops = nops;
-    it.op_off = push(b->blob, ops, sizeof(qvmhip_tile_op) * nops);
+    std::vector<qvmhip_tile_op> sched;
+    schedule_tile_ops(b->dtype, m, ops, nops, sched);
+    it.nops = (uint32_t)sched.size();
+    it.op_off = push(b->blob, sched.data(), sizeof(qvmhip_tile_op) * sched.size());
     b->prog[inst].push_back(std::move(it));
     return QVMHIP_OK;
 }
@@ -348,7 +344,7 @@ int qvmhip_batch_finalize(qvmhip_batch *b, int use_graph) {
                 const BItem &it = b->prog[i][L];
                 std::vector<char> ent(esz);
                 uint64_t nt = 0;
-                build_tile_entry(ent.data(), b->nq[i], it.k, it.qubits, b->offset[i], blocks,
+                build_tile_entry(ent.data(), b->dtype, b->nq[i], it.k, it.qubits, b->offset[i], blocks,
                                  (const qvmhip_tile_op *)((const char *)b->blob_dev + it.op_off), it.nops, &nt);
                 b->entries.insert(b->entries.end(), ent.begin(), ent.end());
                 blocks += nt;

@@ -1,24 +1,25 @@
 // fused.hip -- dense fused k-qubit blocks (k = 3..6) on the matrix cores.
 //
 // Applying a 2^k x 2^k complex U to every complement group is a tall-skinny
-// complex GEMM Y = U X, X = 2^k x 2^(n-k).  It is computed as one real GEMM
-// with the interleaved real embedding
-//     A[2i][2j] = Re U_ij   A[2i][2j+1] = -Im U_ij
-//     A[2i+1][2j] = Im U_ij A[2i+1][2j+1] = Re U_ij
-// so the state's own (re, im) interleaving is the real X (rows 2j, 2j+1 of a
-// column are Re/Im of the column's amplitude j) -- no deinterleave pass.
-// The four real products per complex product are all kept (no 3-multiply
-// form), so rounding stays that of an ordinary complex matvec with FMA.
+// complex GEMM Y = U X, X = 2^k x 2^(n-k) (one column per group).  It runs as
+// split-complex MFMAs:
+//     Y_re += U_re X_re + U_im (-X_im)      Y_im += U_im X_re + U_re X_im
+// with v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32.  The operand layout of
+// these instructions puts B[k][j] in lane j + 16k, so one lane's B values for
+// the four products of a k-step are the real and imaginary part of ONE
+// amplitude (row 4*ks + k of column j): every lane loads whole complex
+// amplitudes straight from HBM, and the result layout hands every lane the
+// real and imaginary part of the same output amplitudes, stored straight
+// back.  No LDS staging of X, no barriers in the column loop.
 //
-// Workgroup: 4 wavefronts, 64 complement groups (columns) per tile; wave w
-// owns columns 16w..16w+15 and all 2D/16 row tiles.  The A fragments of U
-// are loaded once per workgroup into registers (k <= 5) or LDS (k = 6, rows
-// padded by one element so the 16 lanes of a fragment hit distinct banks).
-// Each X tile is gathered with coalesced loads one tile ahead (software
-// pipelined, so HBM latency overlaps the MFMAs), staged in LDS, multiplied
-// with v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32, written back into the
-// wave's own LDS columns and stored with the same coalesced pattern.  The
-// workgroup grid-strides over tiles.
+// Rows of U are renumbered so that bit b of a row index is the b-th lowest
+// target qubit; a k-step's four rows then differ in the two lowest targets,
+// and 16 consecutive complement groups x those rows are contiguous runs.  The
+// A fragments (U_re, U_im per 16x4 block and lane) are built once per
+// workgroup in LDS from U in nat-tuple order.  Each wave takes 16 complement
+// groups at a time, grid-striding, with the next group's loads issued before
+// the current group's MFMAs.  The four real products per complex product are
+// all kept, so rounding is that of a complex matvec with FMA.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -48,124 +49,140 @@ template <> struct Mfma<float> {
 };
 
 struct FusedOff {
-    uint64_t o[64];
+    uint64_t o[64];  // offset of sorted-order row a: sum of 2^sorted[b] over the bits of a
+    uint32_t nat[6]; // position in the nat-tuple of the b-th lowest target
 };
-
-constexpr int kFusedCols = 64;
 
 template <typename T, int K>
 __global__ __launch_bounds__(256) void k_fused(vec2_t<T> *__restrict__ psi, const double *__restrict__ U,
-                                               GroupSpec g, FusedOff off, int group_fastest) {
+                                               GroupSpec g, FusedOff off) {
     using V = vec2_t<T>;
     using v4 = typename Mfma<T>::v4;
     constexpr int D = 1 << K;
-    constexpr int R2 = 2 * D;       // real rows = real cols of A
-    constexpr int LU = 2 * D + 2;   // padded complex row of U (in T words)
-    constexpr int LX = R2 + 2;      // padded real column of X
-    constexpr int MT = R2 / 16;     // row tiles
-    constexpr int KS = R2 / 4;      // k-steps
-    constexpr int NCB = kFusedCols;
-    constexpr int EPT = NCB * D / 256;   // amplitudes per thread per tile
-    // A fragments live in registers up to k = 5 (MT*KS values per lane);
-    // the 64 x 64 matrix of k = 6 is read from LDS.
-    constexpr bool AREG = K <= 5;
-    __shared__ T su[AREG ? 1 : D * LU];
-    __shared__ T sx[NCB * LX];
+    constexpr int MT = D < 16 ? 1 : D / 16;  // 16-row output blocks
+    constexpr int KS = D / 4;                // 4-row k-steps
+    __shared__ V fr[MT * KS * 64];
 
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int lj = lane & 15, kk = lane >> 4;
-    // A fragment of lane: row r = mt*16 + lj, col c = 4*ks + kk; the real
-    // embedding picks Re or Im of U[r/2][c/2] and a sign from the parities.
-    const int rpar = lj & 1, cpar = kk & 1;
-    const int use_im = rpar != cpar ? 1 : 0;
-    const T sgn = (rpar == 0 && cpar == 1) ? T(-1) : T(1);
-    T afrag[AREG ? MT * KS : 1];
-    if constexpr (AREG) {
+    // A fragments: fr[(mt*KS + ks)*64 + lane] = U_s[mt*16 + i][ks*4 + k],
+    // lane = i + 16k, U_s = U with rows and columns in sorted-target order.
+    for (int e = threadIdx.x; e < MT * KS * 64; e += 256) {
+        const int ln = e & 63, blk = e >> 6;
+        const int mt = blk / KS, ks = blk % KS;
+        const int rs = mt * 16 + (ln & 15), cs = ks * 4 + (ln >> 4);
+        V v;
+        v.x = T(0);
+        v.y = T(0);
+        if (rs < D) {
+            int rn = 0, cn = 0;
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-            for (int ks = 0; ks < KS; ++ks) {
-                const int ri = (mt * 16 + lj) >> 1, cj = (4 * ks + kk) >> 1;
-                afrag[mt * KS + ks] = sgn * (T)U[2 * (ri * D + cj) + use_im];
+            for (int b = 0; b < K; ++b) {
+                rn |= ((rs >> b) & 1) << off.nat[b];
+                cn |= ((cs >> b) & 1) << off.nat[b];
             }
-    } else {
-        for (int e = threadIdx.x; e < D * D; e += 256) {
-            const int i = e / D, j = e % D;
-            su[i * LU + 2 * j] = (T)U[2 * e];
-            su[i * LU + 2 * j + 1] = (T)U[2 * e + 1];
+            v.x = (T)U[2 * (rn * D + cn)];
+            v.y = (T)U[2 * (rn * D + cn) + 1];
         }
+        fr[e] = v;
     }
+    __syncthreads();
 
-    // element e of a tile -> (column gi, amplitude a)
-    auto col_of = [&](int e) { return group_fastest ? (e % NCB) : (e / D); };
-    auto amp_of = [&](int e) { return group_fastest ? (e / NCB) : (e % D); };
-    auto load_tile = [&](uint64_t tile, V *x) {
+    const int lane = threadIdx.x & 63;
+    const int j = lane & 15, kk = lane >> 4;
+    const uint64_t ncols = g.ngroups;
+    const uint64_t ncg = (ncols + 15) / 16;
+    const uint64_t wstride = (uint64_t)gridDim.x * 4;
+    // lane parts of the row offsets: loads take row 4*ks + kk; the result
+    // rows of register e are kk + 4e (f64) or 4*kk + e (f32)
+    const uint64_t lo_ld = off.o[kk];
+    const uint64_t lo_st = sizeof(T) == 8 ? off.o[kk] : off.o[(4 * kk) & (D - 1)];
+    const bool st_lane_ok = sizeof(T) == 8 || 4 * kk < D;
+
+    auto load = [&](uint64_t cg, V (&xv)[KS], uint64_t &base, bool &ok) {
+        const uint64_t col = cg * 16 + j;
+        ok = col < ncols;
+        base = group_base(g, ok ? col : 0);
 #pragma unroll
-        for (int j = 0; j < EPT; ++j) {
-            const int e = threadIdx.x + 256 * j;
-            const uint64_t grp = tile * NCB + col_of(e);
-            if (grp < g.ngroups) {
-                x[j] = psi[group_base(g, grp) + off.o[amp_of(e)]];
+        for (int ks = 0; ks < KS; ++ks) {
+            if (ok) {
+                xv[ks] = psi[base + (off.o[4 * ks] | lo_ld)];
             } else {
-                x[j].x = T(0);
-                x[j].y = T(0);
+                xv[ks].x = T(0);
+                xv[ks].y = T(0);
             }
         }
     };
 
-    const uint64_t ntiles = (g.ngroups + NCB - 1) / NCB;
-    V pre[EPT];
-    uint64_t tile = blockIdx.x;
-    if (tile < ntiles) load_tile(tile, pre);
-    for (; tile < ntiles; tile += gridDim.x) {
-        __syncthreads();  // previous tile fully stored
+    // k <= 5: the next column group's loads are issued before this group's
+    // MFMAs (register double buffer); k = 6 has no registers to spare for it
+    // and relies on the other waves of the SIMD instead.
+    constexpr bool PF = K <= 5;
+    uint64_t cg = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    V xn[PF ? KS : 1];
+    uint64_t bn = 0;
+    bool okn = false;
+    if constexpr (PF) {
+        if (cg < ncg) load(cg, xn, bn, okn);
+    }
+    for (; cg < ncg; cg += wstride) {
+        V xv[KS];
+        uint64_t base;
+        bool ok;
+        if constexpr (PF) {
 #pragma unroll
-        for (int j = 0; j < EPT; ++j) {
-            const int e = threadIdx.x + 256 * j;
-            sx[col_of(e) * LX + 2 * amp_of(e)] = pre[j].x;
-            sx[col_of(e) * LX + 2 * amp_of(e) + 1] = pre[j].y;
+            for (int ks = 0; ks < KS; ++ks) xv[ks] = xn[ks];
+            base = bn;
+            ok = okn;
+            if (cg + wstride < ncg) load(cg + wstride, xn, bn, okn);  // in flight during the MFMAs
+        } else {
+            load(cg, xv, base, ok);
         }
-        __syncthreads();
-        const uint64_t next = tile + gridDim.x;
-        if (next < ntiles) load_tile(next, pre);  // in flight during the MFMAs
-        v4 acc[MT];
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt) acc[mt] = v4{0, 0, 0, 0};
-        const T *xcol = &sx[(wave * 16 + lj) * LX];
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
-            const T b = xcol[4 * ks + kk];
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt) {
-                T a;
-                if constexpr (AREG) {
-                    a = afrag[mt * KS + ks];
-                } else {
-                    const int ri = (mt * 16 + lj) >> 1, cj = (4 * ks + kk) >> 1;
-                    a = sgn * su[ri * LU + 2 * cj + use_im];
-                }
-                acc[mt] = Mfma<T>::mma(a, b, acc[mt]);
-            }
-        }
-        // D fragment: register e of this lane is row Mfma<T>::row(kk, e) of
-        // column lj.  Each wave only overwrites its own 16 columns, which no
-        // other wave reads.
-        T *ycol = &sx[(wave * 16 + lj) * LX];
+
+        v4 ar[MT], ai[MT];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
-#pragma unroll
-            for (int e = 0; e < 4; ++e) ycol[mt * 16 + Mfma<T>::row(kk, e)] = acc[mt][e];
+            ar[mt] = v4{0, 0, 0, 0};
+            ai[mt] = v4{0, 0, 0, 0};
         }
-        __syncthreads();
+        // A fragments of k-step ks+1 are read from LDS while ks multiplies;
+        // the scheduling barrier keeps the compiler from hoisting all of
+        // them (k = 6 would need 256 registers for them)
+        V an[MT];
 #pragma unroll
-        for (int j = 0; j < EPT; ++j) {
-            const int e = threadIdx.x + 256 * j;
-            const uint64_t grp = tile * NCB + col_of(e);
-            if (grp >= g.ngroups) continue;
-            V y;
-            y.x = sx[col_of(e) * LX + 2 * amp_of(e)];
-            y.y = sx[col_of(e) * LX + 2 * amp_of(e) + 1];
-            psi[group_base(g, grp) + off.o[amp_of(e)]] = y;
+        for (int mt = 0; mt < MT; ++mt) an[mt] = fr[mt * KS * 64 + lane];
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            V ac[MT];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) ac[mt] = an[mt];
+            if (ks + 1 < KS) {
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) an[mt] = fr[(mt * KS + ks + 1) * 64 + lane];
+            }
+            const T br = xv[ks].x, bi = xv[ks].y, nbi = -xv[ks].y;
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                ar[mt] = Mfma<T>::mma(ac[mt].x, br, ar[mt]);
+                ai[mt] = Mfma<T>::mma(ac[mt].y, br, ai[mt]);
+                ar[mt] = Mfma<T>::mma(ac[mt].y, nbi, ar[mt]);
+                ai[mt] = Mfma<T>::mma(ac[mt].x, bi, ai[mt]);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (ok && st_lane_ok) {
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    // uniform part of the result row: 4e + 16mt (f64) or e + 16mt (f32)
+                    const int hi = (sizeof(T) == 8 ? 4 * e : e) + 16 * mt;
+                    if (hi < D) {
+                        V y;
+                        y.x = ar[mt][e];
+                        y.y = ai[mt][e];
+                        psi[base + (off.o[hi] | lo_st)] = y;
+                    }
+                }
+            }
         }
     }
 }
@@ -173,7 +190,6 @@ __global__ __launch_bounds__(256) void k_fused(vec2_t<T> *__restrict__ psi, cons
 namespace {
 template <typename T, int K>
 int run_fused(qvmhip_state *s, const void *mat_dev, const uint32_t *qubits) {
-    constexpr int D = 1 << K;
     GroupSpec g;
     std::memset(&g, 0, sizeof(g));
     std::vector<uint32_t> pos(qubits, qubits + K);
@@ -183,19 +199,20 @@ int run_fused(qvmhip_state *s, const void *mat_dev, const uint32_t *qubits) {
     g.ngroups = uint64_t(1) << (s->n - K);
     FusedOff off;
     std::memset(&off, 0, sizeof(off));
-    bool has_q0 = false;
-    for (int a = 0; a < D; ++a) {
+    for (int b = 0; b < K; ++b)
+        for (int c = 0; c < K; ++c)
+            if (qubits[c] == pos[b]) off.nat[b] = (uint32_t)c;
+    for (int a = 0; a < 64; ++a) {
         uint64_t o = 0;
         for (int b = 0; b < K; ++b)
-            if ((a >> b) & 1) o |= uint64_t(1) << qubits[b];
+            if ((a >> b) & 1) o |= uint64_t(1) << pos[b];
         off.o[a] = o;
     }
-    for (int b = 0; b < K; ++b) has_q0 |= qubits[b] == 0;
-    const uint64_t ntiles = (g.ngroups + kFusedCols - 1) / kFusedCols;
-    const uint32_t grid = (uint32_t)std::min<uint64_t>(ntiles, 256 * 8);
+    const uint64_t ncg = (g.ngroups + 15) / 16;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((ncg + 3) / 4, 256 * 8);
     prof_begin(s, QVMHIP_K_FUSED);
     hipLaunchKernelGGL((k_fused<T, K>), dim3(grid), dim3(256), 0, s->stream, (vec2_t<T> *)s->amps,
-                       (const double *)mat_dev, g, off, has_q0 ? 0 : 1);
+                       (const double *)mat_dev, g, off);
     prof_end(s);
     QH_LAUNCH_CHECK();
     return QVMHIP_OK;

@@ -134,23 +134,18 @@ int qvmhip_program_add_tile_pass(qvmhip_program *p, uint32_t m, const uint32_t *
         if (tile_qubits[i] >= p->s->n) return fail(QVMHIP_EINVAL, "tile qubit out of range");
         if (i && tile_qubits[i] <= tile_qubits[i - 1]) return fail(QVMHIP_EINVAL, "tile qubits must ascend");
     }
-    for (uint32_t o = 0; o < nops; ++o) {
-        const qvmhip_tile_op &op = ops[o];
-        if (op.kind < QVMHIP_TILE_DENSE1 || op.kind > QVMHIP_TILE_CPHASE)
-            return fail(QVMHIP_EINVAL, "unknown tile op kind");
-        const bool two = op.kind != QVMHIP_TILE_DENSE1 && op.kind != QVMHIP_TILE_DIAG1;
-        if (op.q0 >= m || (two && (op.q1 >= m || op.q1 == op.q0)))
-            return fail(QVMHIP_EINVAL, "tile op qubit out of range");
-        if (op.ctrl_mask >> m) return fail(QVMHIP_EINVAL, "tile op control outside the tile");
-    }
+    int rc = check_tile_ops(m, nops, ops);
+    if (rc) return rc;
     if (nops == 0) return QVMHIP_OK;
     Item it;
     it.kind = kTile;
     it.k = m;
     std::copy(tile_qubits, tile_qubits + m, it.qubits);
-    it.nops = nops;
-    it.bytes = sizeof(qvmhip_tile_op) * nops;
-    it.host_off = push_blob(p, ops, it.bytes);
+    std::vector<qvmhip_tile_op> sched;
+    schedule_tile_ops(p->s->dtype, m, ops, nops, sched);
+    it.nops = (uint32_t)sched.size();
+    it.bytes = sizeof(qvmhip_tile_op) * sched.size();
+    it.host_off = push_blob(p, sched.data(), it.bytes);
     p->items.push_back(std::move(it));
     return QVMHIP_OK;
 }

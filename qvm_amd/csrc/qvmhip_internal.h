@@ -163,6 +163,8 @@ int reduce_norm2(qvmhip_state *s, double *out_dev);
 int launch_force_measure(qvmhip_state *s, uint32_t q, int bit, double inv_norm, int compiled_form);
 int sample_shots(qvmhip_state *s, const double *u, uint64_t nshots, uint64_t *out, int rule_ge);
 int collapse_to_basis(qvmhip_state *s, const uint64_t *index_dev);
+// ops_on_device = 0: public ops on the host (scheduled here); 1: an already
+// scheduled program on the device.
 int launch_tile_pass(qvmhip_state *s, uint32_t m, const uint32_t *tile_qubits, uint32_t nops,
                      const qvmhip_tile_op *ops_dev_or_host, int ops_on_device);
 int launch_fused_dense(qvmhip_state *s, const void *mat_dev, uint32_t k, const uint32_t *qubits);
@@ -179,8 +181,13 @@ int convert_f64_to_f32(qvmhip_state *s, uint64_t start, uint64_t count, const do
 // Capturable (kernel launches only) forms used by batches and programs.
 int expectation_zz_to_device(qvmhip_state *s, const uint32_t *edges_dev, uint32_t nedges, double *out_dev);
 int sample_to_device(qvmhip_state *s, const double *u_dev, uint64_t nshots, uint64_t *out_dev, int rule_ge);
+// Tile passes: validation of a public op list (tile-local bit positions) and
+// its translation into the slot-space program the tile kernels run.
+int check_tile_ops(uint32_t m, uint32_t nops, const qvmhip_tile_op *ops);
+void schedule_tile_ops(int dtype, uint32_t m, const qvmhip_tile_op *in, uint32_t nin,
+                       std::vector<qvmhip_tile_op> &out);
 size_t tile_entry_bytes();
-void build_tile_entry(void *entry, uint32_t n, uint32_t m, const uint32_t *tq, uint64_t amp_offset,
+void build_tile_entry(void *entry, int dtype, uint32_t n, uint32_t m, const uint32_t *tq, uint64_t amp_offset,
                       uint64_t block_start, const qvmhip_tile_op *ops_dev, uint32_t nops, uint64_t *ntiles);
 int launch_tile_batch(hipStream_t st, int dtype, uint32_t m, void *base, const void *entries_dev, uint32_t nent,
                       uint64_t total_blocks);

@@ -1,19 +1,33 @@
-// tile.hip -- gate sequences applied to LDS-resident blocks of the state.
+// tile.hip -- gate sequences applied to register-resident blocks of the state.
 //
 // A tile pass names m "tile qubits".  For every assignment of the other n-m
-// qubits, one workgroup loads the 2^m amplitudes spanned by the tile qubits
-// into LDS, applies the pass's gates one after another (each a 1- or 2-qubit
-// op on local positions, with __syncthreads between ops) and writes the block
-// back: the whole gate sequence costs one HBM read and one write of the state.
-// Tile qubit sets that include qubits 0..L-1 give contiguous runs of 2^L
-// amplitudes per row, so loads and stores stay coalesced; high-order gate
-// targets are brought in simply by naming them as tile qubits (the load is a
-// strided gather of those runs), which is the qubit permutation into an
-// LDS-resident block.
+// qubits one workgroup loads the 2^m amplitudes spanned by the tile qubits,
+// applies the pass's gates one after another and writes the block back: the
+// whole gate sequence costs one HBM read and one write of the state.  Tile
+// qubit sets that include qubits 0..L-1 give contiguous runs of 2^L
+// amplitudes, so loads and stores stay coalesced; high-order gate targets are
+// brought in by naming them as tile qubits (the load is a strided gather of
+// those runs), which is the qubit permutation into an on-chip block.
+//
+// Inside the workgroup the block lives in registers.  Slot s = t + NT*r of
+// the block is register r of thread t, so the low TB = m - RB slot bits are
+// thread bits and the high RB bits are register bits (R = 2^RB amplitudes per
+// thread).  A gate whose non-diagonal targets sit on register bits runs in
+// registers with no data exchange; diagonal gates and control tests run on any
+// bit.  When a gate needs a target that sits on a thread bit, the block is
+// re-laid out through LDS (REMAP: every thread stores its registers, and loads
+// back the slots of a new bit permutation).  Uncontrolled SWAPs are pure
+// relabelings of that permutation and move no data.  The host scheduler
+// (schedule_tile_ops) turns a pass's op list, written against tile-local bit
+// positions, into this slot-space program: it picks which local bits become
+// register bits at each REMAP by next use (farthest next use is evicted), and
+// ends the pass with the identity layout so the store is coalesced.
 //
 // Arithmetic per gate is the same as the plain kernels (gates.hip): separate
-// complex products summed left to right, built with -ffp-contract=off.
+// complex products summed left to right, built with -ffp-contract=off, so an
+// in-order pass is bit-identical to gate-by-gate application.
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <vector>
 
@@ -21,178 +35,442 @@
 
 namespace qvmhip {
 
+// Slot-space op kinds that only the scheduler emits (after the public ones).
+constexpr uint32_t kTileXReg = 16;   // X on register bit (q0 - TB) where ctrl_mask holds
+constexpr uint32_t kTileRSwap = 17;  // exchange register bits q0 and q1 (both < RB)
+constexpr uint32_t kTileRemap = 18;  // relayout through LDS; new slot bit j <- old bit src[j]
+
+// Register bits per thread: 4 (16 amplitudes; 32 registers would spill),
+// fewer for small tiles so a workgroup keeps at least 64 threads.
+__host__ __device__ constexpr int tile_reg_bits(int m, bool f64) {
+    (void)f64;
+    return (m - 6) < 4 ? (m - 6) : 4;
+}
+
 struct TileArgs {
     GroupSpec g;        // tile base: inject zeros at the m tile positions
-    uint64_t toff_bits; // unused padding for alignment
     uint32_t tq[16];    // global positions of the tile qubits (ascending)
-    uint64_t roff[32];  // offsets of the per-thread rounds r (bits >= log2(NT))
+    uint64_t roff[32];  // global offset of register r (slot bits >= TB)
 };
 
-template <int NT>
+template <int TB>
 __device__ inline uint64_t thread_offset(const TileArgs &a, uint32_t t) {
     uint64_t o = 0;
 #pragma unroll
-    for (int b = 0; (1 << b) < NT; ++b)
+    for (int b = 0; b < TB; ++b)
         if ((t >> b) & 1u) o |= uint64_t(1) << a.tq[b];
     return o;
 }
 
-__device__ inline uint32_t ins0(uint32_t x, uint32_t q) {
-    const uint32_t lo = x & ((1u << q) - 1u);
-    return ((x >> q) << (q + 1)) | lo;
+__device__ __forceinline__ double qfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float qfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// Complex product.  FMA = false: four separately rounded products, as the
+// plain kernels; FMA = true: one product and one fused multiply-add per part.
+template <typename T, bool FMA = false>
+__device__ __forceinline__ void cmul(T ar, T ai, T br, T bi, T &r, T &i) {
+    if constexpr (FMA) {
+        r = qfma(ar, br, -(ai * bi));
+        i = qfma(ar, bi, ai * br);
+    } else {
+        r = ar * br - ai * bi;
+        i = ar * bi + ai * br;
+    }
+}
+
+// y = a*u + b*v (complex): the sum of two rounded products, or (FMA) one
+// product and three fused multiply-adds per part.
+template <typename T, bool FMA>
+__device__ __forceinline__ void cmac2(T ar, T ai, T ur, T ui, T br, T bi, T vr, T vi, T &yr, T &yi) {
+    if constexpr (FMA) {
+        yr = qfma(ar, ur, qfma(-ai, ui, qfma(br, vr, -(bi * vi))));
+        yi = qfma(ar, ui, qfma(ai, ur, qfma(br, vi, bi * vr)));
+    } else {
+        T p0r, p0i, p1r, p1i;
+        cmul(ar, ai, ur, ui, p0r, p0i);
+        cmul(br, bi, vr, vi, p1r, p1i);
+        yr = p0r + p1r;
+        yi = p0i + p1i;
+    }
+}
+
+// Operand i of a scheduled op: FP64 passes keep the doubles of the public op;
+// for FP32 passes the scheduler stores the values as floats in the same
+// bytes, so the kernel reads them without a conversion.
+template <typename T>
+__device__ __forceinline__ T mval(const qvmhip_tile_op &op, int i) {
+    if constexpr (sizeof(T) == 8)
+        return op.m[i];
+    else
+        return ((const float *)op.m)[i];
+}
+
+// register index with a zero inserted at bit I
+template <int I>
+__device__ constexpr int rins0(int p) {
+    return ((p >> I) << (I + 1)) | (p & ((1 << I) - 1));
 }
 
 template <typename T>
-__device__ inline void cmul(T ar, T ai, T br, T bi, T &r, T &i) {
-    r = ar * br - ai * bi;
-    i = ar * bi + ai * br;
+__device__ __forceinline__ vec2_t<T> vsel(bool c, vec2_t<T> a, vec2_t<T> b) {
+    vec2_t<T> r;
+    r.x = c ? a.x : b.x;
+    r.y = c ? a.y : b.y;
+    return r;
 }
 
-// One tile: load the 2^M amplitudes of tile `tile` into LDS, apply the op
-// list, store.  Shared by the single-state and the batched kernels.
-template <typename T, int M, int NT>
-__device__ inline void tile_body(vec2_t<T> *__restrict__ psi, const TileArgs &a,
-                                 const qvmhip_tile_op *__restrict__ ops, uint32_t nops, uint64_t tile,
-                                 vec2_t<T> *sa, uint64_t toff) {
+// The register ops compute every pair unconditionally and, for controlled
+// ops (CTRL), keep the old values by selection where the control bits are not
+// all 1 (lane part lane_ok, register part rcm): straight-line code, so the
+// pairs of all registers interleave.
+
+template <typename T, int R, int I, bool CTRL, bool FMA>
+__device__ __forceinline__ void reg_dense1(vec2_t<T> (&x)[R], const qvmhip_tile_op &op, bool lane_ok,
+                                           uint32_t rcm) {
     using V = vec2_t<T>;
-    constexpr int S = 1 << M;
-    constexpr int R = S / NT;  // amplitudes per thread
-    const uint32_t t = threadIdx.x;
-    {
-        const uint64_t base = group_base(a.g, tile) + toff;
-
-        // load: element l = t + r*NT (only this thread reads these slots back in
-        // the previous tile's store phase, so no barrier is needed before it)
-        {
-            V x[R];
+    const T u00r = mval<T>(op, 0), u00i = mval<T>(op, 1), u01r = mval<T>(op, 2), u01i = mval<T>(op, 3);
+    const T u10r = mval<T>(op, 4), u10i = mval<T>(op, 5), u11r = mval<T>(op, 6), u11i = mval<T>(op, 7);
 #pragma unroll
-            for (int r = 0; r < R; ++r) x[r] = psi[base + a.roff[r]];
-#pragma unroll
-            for (int r = 0; r < R; ++r) sa[t + r * NT] = x[r];
-        }
-        __syncthreads();
-
-        for (uint32_t oi = 0; oi < nops; ++oi) {
-            const qvmhip_tile_op &op = ops[oi];
-            const uint32_t kind = op.kind;
-            const uint32_t q0 = op.q0, q1 = op.q1, cm = op.ctrl_mask;
-            if (kind == QVMHIP_TILE_DENSE1) {
-                const T u00r = (T)op.m[0], u00i = (T)op.m[1], u01r = (T)op.m[2], u01i = (T)op.m[3];
-                const T u10r = (T)op.m[4], u10i = (T)op.m[5], u11r = (T)op.m[6], u11i = (T)op.m[7];
-#pragma unroll
-                for (uint32_t p = t; p < S / 2; p += NT) {
-                    const uint32_t ia = ins0(p, q0), ib = ia | (1u << q0);
-                    if ((ia & cm) != cm) continue;
-                    const V xa = sa[ia], xb = sa[ib];
-                    T p0r, p0i, p1r, p1i;
-                    V ya, yb;
-                    cmul(xa.x, xa.y, u00r, u00i, p0r, p0i);
-                    cmul(xb.x, xb.y, u01r, u01i, p1r, p1i);
-                    ya.x = p0r + p1r;
-                    ya.y = p0i + p1i;
-                    cmul(xa.x, xa.y, u10r, u10i, p0r, p0i);
-                    cmul(xb.x, xb.y, u11r, u11i, p1r, p1i);
-                    yb.x = p0r + p1r;
-                    yb.y = p0i + p1i;
-                    sa[ia] = ya;
-                    sa[ib] = yb;
-                }
-            } else if (kind == QVMHIP_TILE_DENSE2) {
-                const uint32_t lo = q0 < q1 ? q0 : q1, hi = q0 < q1 ? q1 : q0;
-#pragma unroll
-                for (uint32_t p = t; p < S / 4; p += NT) {
-                    const uint32_t i0 = ins0(ins0(p, lo), hi);
-                    if ((i0 & cm) != cm) continue;
-                    const uint32_t idx[4] = {i0, i0 | (1u << q0), i0 | (1u << q1), i0 | (1u << q0) | (1u << q1)};
-                    V x[4];
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) x[j] = sa[idx[j]];
-#pragma unroll
-                    for (int row = 0; row < 4; ++row) {
-                        T yr, yi;
-                        cmul(x[0].x, x[0].y, (T)op.m[8 * row], (T)op.m[8 * row + 1], yr, yi);
-#pragma unroll
-                        for (int c = 1; c < 4; ++c) {
-                            T pr, pi;
-                            cmul(x[c].x, x[c].y, (T)op.m[8 * row + 2 * c], (T)op.m[8 * row + 2 * c + 1], pr, pi);
-                            yr = yr + pr;
-                            yi = yi + pi;
-                        }
-                        V y;
-                        y.x = yr;
-                        y.y = yi;
-                        sa[idx[row]] = y;
-                    }
-                }
-            } else if (kind == QVMHIP_TILE_DIAG1 || kind == QVMHIP_TILE_DIAG2) {
-                const bool two = kind == QVMHIP_TILE_DIAG2;
-#pragma unroll
-                for (uint32_t l = t; l < S; l += NT) {
-                    if ((l & cm) != cm) continue;
-                    uint32_t j = (l >> q0) & 1u;
-                    if (two) j |= ((l >> q1) & 1u) << 1;
-                    const V x = sa[l];
-                    V y;
-                    cmul(x.x, x.y, (T)op.m[2 * j], (T)op.m[2 * j + 1], y.x, y.y);
-                    sa[l] = y;
-                }
-            } else if (kind == QVMHIP_TILE_CPERM) {
-                // target q0, control q1: swap (c=1,t=0) <-> (c=1,t=1)
-                const uint32_t lo = q0 < q1 ? q0 : q1, hi = q0 < q1 ? q1 : q0;
-#pragma unroll
-                for (uint32_t p = t; p < S / 4; p += NT) {
-                    const uint32_t i0 = ins0(ins0(p, lo), hi) | (1u << q1);
-                    if ((i0 & cm) != cm) continue;
-                    const uint32_t i1 = i0 | (1u << q0);
-                    const V x0 = sa[i0], x1 = sa[i1];
-                    sa[i0] = x1;
-                    sa[i1] = x0;
-                }
-            } else if (kind == QVMHIP_TILE_SWAP) {
-                const uint32_t lo = q0 < q1 ? q0 : q1, hi = q0 < q1 ? q1 : q0;
-#pragma unroll
-                for (uint32_t p = t; p < S / 4; p += NT) {
-                    const uint32_t i0 = ins0(ins0(p, lo), hi);
-                    if ((i0 & cm) != cm) continue;
-                    const uint32_t ia = i0 | (1u << q0), ib = i0 | (1u << q1);
-                    const V xa = sa[ia], xb = sa[ib];
-                    sa[ia] = xb;
-                    sa[ib] = xa;
-                }
-            } else if (kind == QVMHIP_TILE_CPHASE) {
-                const uint32_t lo = q0 < q1 ? q0 : q1, hi = q0 < q1 ? q1 : q0;
-                const T mr = (T)op.m[0], mi = (T)op.m[1];
-#pragma unroll
-                for (uint32_t p = t; p < S / 4; p += NT) {
-                    const uint32_t i0 = ins0(ins0(p, lo), hi) | (1u << q0) | (1u << q1);
-                    if ((i0 & cm) != cm) continue;
-                    const V x = sa[i0];
-                    V y;
-                    cmul(x.x, x.y, mr, mi, y.x, y.y);
-                    sa[i0] = y;
-                }
-            }
-            __syncthreads();
-        }
-
-        // store
-        {
-            V x[R];
-#pragma unroll
-            for (int r = 0; r < R; ++r) x[r] = sa[t + r * NT];
-#pragma unroll
-            for (int r = 0; r < R; ++r) psi[base + a.roff[r]] = x[r];
+    for (int p = 0; p < R / 2; ++p) {
+        const int ra = rins0<I>(p), rb = ra | (1 << I);
+        const V xa = x[ra], xb = x[rb];
+        V ya, yb;
+        cmac2<T, FMA>(xa.x, xa.y, u00r, u00i, xb.x, xb.y, u01r, u01i, ya.x, ya.y);
+        cmac2<T, FMA>(xa.x, xa.y, u10r, u10i, xb.x, xb.y, u11r, u11i, yb.x, yb.y);
+        if constexpr (CTRL) {
+            const bool ok = lane_ok && ((uint32_t)ra & rcm) == rcm;
+            x[ra] = vsel<T>(ok, ya, xa);
+            x[rb] = vsel<T>(ok, yb, xb);
+        } else {
+            x[ra] = ya;
+            x[rb] = yb;
         }
     }
 }
 
-template <typename T, int M, int NT>
-__global__ __launch_bounds__(NT) void k_tile(vec2_t<T> *__restrict__ psi, const TileArgs a,
-                                             const qvmhip_tile_op *__restrict__ ops, uint32_t nops) {
-    __shared__ vec2_t<T> sa[1 << M];
-    const uint64_t toff = thread_offset<NT>(a, threadIdx.x);
+// 4x4 on register bits 0 (matrix bit 0) and 1 (matrix bit 1)
+template <typename T, int R, bool CTRL, bool FMA>
+__device__ __forceinline__ void reg_dense2(vec2_t<T> (&x)[R], const qvmhip_tile_op &op, bool lane_ok,
+                                           uint32_t rcm) {
+    using V = vec2_t<T>;
+#pragma unroll
+    for (int p = 0; p < R / 4; ++p) {
+        const int r0 = p << 2;
+        V in[4], out[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) in[j] = x[r0 | j];
+#pragma unroll
+        for (int row = 0; row < 4; ++row) {
+            T yr, yi;
+            cmul<T, FMA>(in[0].x, in[0].y, mval<T>(op, 8 * row), mval<T>(op, 8 * row + 1), yr, yi);
+#pragma unroll
+            for (int c = 1; c < 4; ++c) {
+                const T ur = mval<T>(op, 8 * row + 2 * c), ui = mval<T>(op, 8 * row + 2 * c + 1);
+                if constexpr (FMA) {
+                    yr = qfma(in[c].x, ur, qfma(-in[c].y, ui, yr));
+                    yi = qfma(in[c].x, ui, qfma(in[c].y, ur, yi));
+                } else {
+                    T pr, pi;
+                    cmul(in[c].x, in[c].y, ur, ui, pr, pi);
+                    yr = yr + pr;
+                    yi = yi + pi;
+                }
+            }
+            out[row].x = yr;
+            out[row].y = yi;
+        }
+        const bool ok = !CTRL || (lane_ok && ((uint32_t)r0 & rcm) == rcm);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) x[r0 | j] = CTRL ? vsel<T>(ok, out[j], in[j]) : out[j];
+    }
+}
+
+template <typename T, int R, int I>
+__device__ __forceinline__ void reg_x(vec2_t<T> (&x)[R], bool lane_ok, uint32_t rcm) {
+    using V = vec2_t<T>;
+#pragma unroll
+    for (int p = 0; p < R / 2; ++p) {
+        const int ra = rins0<I>(p), rb = ra | (1 << I);
+        const bool ok = lane_ok && ((uint32_t)ra & rcm) == rcm;
+        const V xa = x[ra], xb = x[rb];
+        x[ra] = vsel<T>(ok, xb, xa);
+        x[rb] = vsel<T>(ok, xa, xb);
+    }
+}
+
+// controlled SWAP of register bits 0 and 1
+template <typename T, int R>
+__device__ __forceinline__ void reg_swap01(vec2_t<T> (&x)[R], bool lane_ok, uint32_t rcm) {
+    using V = vec2_t<T>;
+#pragma unroll
+    for (int p = 0; p < R / 4; ++p) {
+        const int r0 = p << 2;
+        const bool ok = lane_ok && ((uint32_t)r0 & rcm) == rcm;
+        const V a = x[r0 | 1], b = x[r0 | 2];
+        x[r0 | 1] = vsel<T>(ok, b, a);
+        x[r0 | 2] = vsel<T>(ok, a, b);
+    }
+}
+
+// relabel register bits A < B (no arithmetic)
+template <int A, int B>
+__device__ constexpr int bit_swapped(int r) {
+    return (((r >> A) & 1) == ((r >> B) & 1)) ? r : (r ^ (1 << A) ^ (1 << B));
+}
+
+template <typename T, int R, int A, int B>
+__device__ __forceinline__ void reg_rswap(vec2_t<T> (&x)[R]) {
+    using V = vec2_t<T>;
+    V y[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) y[r] = x[bit_swapped<A, B>(r)];
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[r] = y[r];
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void reg_rswap_any(vec2_t<T> (&x)[R], uint32_t a, uint32_t b) {
+    const uint32_t code = a * 8 + b;
+    switch (code) {
+#define QH_RS(A, B)                                                 \
+    case A * 8 + B:                                                 \
+        if constexpr ((1 << B) < R) reg_rswap<T, R, A, B>(x);       \
+        break;
+        QH_RS(0, 1) QH_RS(0, 2) QH_RS(0, 3) QH_RS(0, 4) QH_RS(1, 2) QH_RS(1, 3) QH_RS(1, 4)
+        QH_RS(2, 3) QH_RS(2, 4) QH_RS(3, 4)
+#undef QH_RS
+        default: break;
+    }
+}
+
+// 4-way select of a complex entry of m by a (per-lane) index j
+template <typename T>
+__device__ __forceinline__ void pick4(const qvmhip_tile_op &op, uint32_t j, T &dr, T &di) {
+    const T r0 = mval<T>(op, 0), i0 = mval<T>(op, 1), r1 = mval<T>(op, 2), i1 = mval<T>(op, 3);
+    const T r2 = mval<T>(op, 4), i2 = mval<T>(op, 5), r3 = mval<T>(op, 6), i3 = mval<T>(op, 7);
+    const T ra = (j & 1u) ? r1 : r0, ia = (j & 1u) ? i1 : i0;
+    const T rb = (j & 1u) ? r3 : r2, ib = (j & 1u) ? i3 : i2;
+    dr = (j & 2u) ? rb : ra;
+    di = (j & 2u) ? ib : ia;
+}
+
+// diagonal 1q / 2q and CPHASE on any slot bits.  The diagonal index has a
+// lane part (target bits that are thread bits, the same for all registers)
+// and a register part (uniform per register), so the per-lane entry is
+// selected once per op and the per-register choice is scalar.
+template <typename T, int R, int TB, bool CTRL, bool FMA>
+__device__ __forceinline__ void reg_diag(vec2_t<T> (&x)[R], const qvmhip_tile_op &op, uint32_t t, bool lane_ok,
+                                         uint32_t rcm) {
+    using V = vec2_t<T>;
+    const uint32_t kind = op.kind, q0 = op.q0, q1 = op.q1;
+    constexpr uint32_t lanes = (1u << TB) - 1u;
+    if (kind == QVMHIP_TILE_CPHASE) {
+        const uint32_t need = op.ctrl_mask | (1u << q0) | (1u << q1);
+        const uint32_t rn = need >> TB;
+        const T mr = mval<T>(op, 0), mi = mval<T>(op, 1);
+        if ((need & lanes) == 0) {
+            // every condition bit is a register bit: a uniform test per
+            // register, the other registers are not touched
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (((uint32_t)r & rn) == rn) {
+                    V y;
+                    cmul<T, FMA>(x[r].x, x[r].y, mr, mi, y.x, y.y);
+                    x[r] = y;
+                }
+            }
+            return;
+        }
+        const bool lok = (t & need) == (need & lanes);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const bool ok = lok && ((uint32_t)r & rn) == rn;
+            V y;
+            cmul<T, FMA>(x[r].x, x[r].y, mr, mi, y.x, y.y);
+            x[r] = vsel<T>(ok, y, x[r]);
+        }
+        return;
+    }
+    const bool two = kind == QVMHIP_TILE_DIAG2;
+    const bool reg0 = q0 >= TB, reg1 = two && q1 >= TB;
+    if (reg0 && (!two || reg1) && (op.ctrl_mask & lanes) == 0) {
+        // index and control bits are all register bits: uniform entries
+        const uint32_t rs0 = q0 - TB, rs1 = two ? q1 - TB : 0u;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (((uint32_t)r & rcm) == rcm) {
+                const uint32_t jr = (((uint32_t)r >> rs0) & 1u) | (two ? ((((uint32_t)r >> rs1) & 1u) << 1) : 0u);
+                T dr, di;
+                pick4<T>(op, jr, dr, di);
+                V y;
+                cmul<T, FMA>(x[r].x, x[r].y, dr, di, y.x, y.y);
+                x[r] = y;
+            }
+        }
+        return;
+    }
+    // lane part of the index
+    const uint32_t jl = (reg0 ? 0u : ((t >> q0) & 1u)) | ((two && !reg1) ? (((t >> q1) & 1u) << 1) : 0u);
+    // register part: bit rs0 of r has weight w0, bit rs1 weight 2 (none: shift
+    // past the register bits)
+    const uint32_t rsa = reg0 ? q0 - TB : 31u, wa = 1u;
+    const uint32_t rsb = reg1 ? q1 - TB : 31u, wb = 2u;
+    // the up to four per-lane candidates, indexed by the register part
+    T cr[4], ci[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) pick4<T>(op, jl | (uint32_t)k, cr[k], ci[k]);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint32_t jr = (((uint32_t)r >> rsa) & 1u) * wa | (((uint32_t)r >> rsb) & 1u) * wb;
+        // jr is uniform: a scalar choice among the per-lane candidates
+        const T dr = jr == 0 ? cr[0] : jr == 1 ? cr[1] : jr == 2 ? cr[2] : cr[3];
+        const T di = jr == 0 ? ci[0] : jr == 1 ? ci[1] : jr == 2 ? ci[2] : ci[3];
+        V y;
+        cmul<T, FMA>(x[r].x, x[r].y, dr, di, y.x, y.y);
+        if constexpr (CTRL) {
+            const bool ok = lane_ok && ((uint32_t)r & rcm) == rcm;
+            x[r] = vsel<T>(ok, y, x[r]);
+        } else {
+            x[r] = y;
+        }
+    }
+}
+
+// LDS address of slot s: register row r = s >> TB of NT slots is padded by
+// one element, so a thread's store of register r is t + r*(NT+1) (a fixed
+// offset from t), and a lane bit that a REMAP takes from register bit b of the
+// old layout steps by (NT+1)*2^b, i.e. 2^b modulo the bank count: with the
+// scheduler pairing vacated lane bit p with evicted register bit p, the lanes
+// of one LDS access hit distinct banks.
+template <int TB>
+__device__ __forceinline__ uint32_t lds_addr(uint32_t s) {
+    return s + (s >> TB);
+}
+
+template <typename T, int R, int TB>
+__device__ __forceinline__ void reg_remap(vec2_t<T> (&x)[R], vec2_t<T> *sa, const qvmhip_tile_op &op, uint32_t t) {
+    constexpr uint32_t NT = 1u << TB;
+    const uint8_t *src = (const uint8_t *)op.m;  // src[j]: old slot bit of new slot bit j
+    __syncthreads();  // readers of the previous layout are done with sa
+#pragma unroll
+    for (int r = 0; r < R; ++r) sa[t + (uint32_t)r * (NT + 1)] = x[r];
+    __syncthreads();
+    uint32_t base = 0;
+#pragma unroll
+    for (int j = 0; j < TB; ++j)
+        if ((t >> j) & 1u) base |= 1u << src[j];
+    uint32_t pw[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) pw[b] = (1 << b) < R ? (1u << src[TB + b]) : 0u;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        uint32_t off = 0;
+#pragma unroll
+        for (int b = 0; (1 << b) < R; ++b)
+            if ((r >> b) & 1) off |= pw[b];
+        x[r] = sa[lds_addr<TB>(base | off)];
+    }
+}
+
+// One tile: load the 2^M amplitudes of tile `tile` into registers, run the
+// scheduled op list, store.  Shared by the single-state and batched kernels.
+template <typename T, int M, int RB>
+__device__ __forceinline__ void tile_load(const vec2_t<T> *__restrict__ psi, const TileArgs &a, uint64_t tile,
+                                          uint64_t toff, vec2_t<T> (&x)[1 << RB]) {
+    const vec2_t<T> *p = psi + group_base(a.g, tile) + toff;
+#pragma unroll
+    for (int r = 0; r < (1 << RB); ++r) x[r] = p[a.roff[r]];
+}
+
+template <typename T, int M, int RB>
+__device__ __forceinline__ void tile_store(vec2_t<T> *__restrict__ psi, const TileArgs &a, uint64_t tile,
+                                           uint64_t toff, const vec2_t<T> (&x)[1 << RB]) {
+    vec2_t<T> *p = psi + group_base(a.g, tile) + toff;
+    // recompute the store addresses instead of keeping them live across
+    // the op loop
+    asm volatile("" : "+v"(p));
+#pragma unroll
+    for (int r = 0; r < (1 << RB); ++r) p[a.roff[r]] = x[r];
+}
+
+// Run a scheduled op list on the block held in x.
+template <typename T, int M, int RB, bool FMA>
+__device__ inline void tile_ops(vec2_t<T> (&x)[1 << RB], const qvmhip_tile_op *__restrict__ ops, uint32_t nops,
+                                vec2_t<T> *sa) {
+    constexpr int R = 1 << RB, TB = M - RB;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t oi = 0; oi < nops; ++oi) {
+        const qvmhip_tile_op &op = ops[oi];
+        const uint32_t kind = op.kind, cm = op.ctrl_mask;
+        const bool lane_ok = (t & cm) == (cm & ((1u << TB) - 1u));
+        const uint32_t rcm = cm >> TB;
+        const bool ctrl = cm != 0;
+        switch (kind) {
+            case QVMHIP_TILE_DENSE1:
+#define QH_D1(I)                                                             \
+    case I:                                                                  \
+        if constexpr (I < RB) {                                              \
+            if (ctrl)                                                        \
+                reg_dense1<T, R, I, true, FMA>(x, op, lane_ok, rcm);            \
+            else                                                             \
+                reg_dense1<T, R, I, false, FMA>(x, op, lane_ok, rcm);            \
+        }                                                                    \
+        break;
+                switch (op.q0 - TB) {
+                    QH_D1(0) QH_D1(1) QH_D1(2) QH_D1(3) QH_D1(4)
+                    default: break;
+                }
+#undef QH_D1
+                break;
+            case QVMHIP_TILE_DENSE2:
+                if (ctrl)
+                    reg_dense2<T, R, true, FMA>(x, op, lane_ok, rcm);
+                else
+                    reg_dense2<T, R, false, FMA>(x, op, lane_ok, rcm);
+                break;
+            case kTileXReg:
+                switch (op.q0 - TB) {
+                    case 0: reg_x<T, R, 0>(x, lane_ok, rcm); break;
+                    case 1: reg_x<T, R, 1>(x, lane_ok, rcm); break;
+                    case 2: if constexpr (RB > 2) reg_x<T, R, 2>(x, lane_ok, rcm); break;
+                    case 3: if constexpr (RB > 3) reg_x<T, R, 3>(x, lane_ok, rcm); break;
+                    case 4: if constexpr (RB > 4) reg_x<T, R, 4>(x, lane_ok, rcm); break;
+                    default: break;
+                }
+                break;
+            case QVMHIP_TILE_SWAP:
+                reg_swap01<T, R>(x, lane_ok, rcm);
+                break;
+            case kTileRSwap:
+                reg_rswap_any<T, R>(x, op.q0, op.q1);
+                break;
+            case kTileRemap:
+                reg_remap<T, R, TB>(x, sa, op, t);
+                break;
+            default:  // DIAG1, DIAG2, CPHASE
+                if (ctrl)
+                    reg_diag<T, R, TB, true, FMA>(x, op, t, lane_ok, rcm);
+                else
+                    reg_diag<T, R, TB, false, FMA>(x, op, t, lane_ok, rcm);
+                break;
+        }
+    }
+
+}
+
+template <typename T, int M, int RB, bool FMA>
+__global__ __launch_bounds__(1 << (M - RB)) void k_tile(vec2_t<T> *__restrict__ psi, const TileArgs a,
+                                                       const qvmhip_tile_op *__restrict__ ops, uint32_t nops) {
+    using V = vec2_t<T>;
+    __shared__ V sa[(1 << M) + (1 << RB)];
+    const uint64_t toff = thread_offset<M - RB>(a, threadIdx.x);
+    // One tile per workgroup (grid-striding only past the dispatch limit):
+    // many resident workgroups overlap each other's loads, ops and stores.
     for (uint64_t tile = blockIdx.x; tile < a.g.ngroups; tile += gridDim.x) {
-        tile_body<T, M, NT>(psi, a, ops, nops, tile, sa, toff);
+        V x[1 << RB];
+        tile_load<T, M, RB>(psi, a, tile, toff, x);
+        tile_ops<T, M, RB, FMA>(x, ops, nops, sa);
+        tile_store<T, M, RB>(psi, a, tile, toff, x);
     }
 }
 
@@ -207,10 +485,11 @@ struct TileEntry {
     uint32_t pad;
 };
 
-template <typename T, int M, int NT>
-__global__ __launch_bounds__(NT) void k_tile_batch(vec2_t<T> *__restrict__ base, const TileEntry *__restrict__ ent,
-                                                   uint32_t nent, uint64_t total_blocks) {
-    __shared__ vec2_t<T> sa[1 << M];
+template <typename T, int M, int RB>
+__global__ __launch_bounds__(1 << (M - RB)) void k_tile_batch(vec2_t<T> *__restrict__ base,
+                                                             const TileEntry *__restrict__ ent, uint32_t nent,
+                                                             uint64_t total_blocks) {
+    __shared__ vec2_t<T> sa[(1 << M) + (1 << RB)];
     for (uint64_t b = blockIdx.x; b < total_blocks; b += gridDim.x) {
         uint32_t lo = 0, hi = nent;  // last entry with block_start <= b
         while (hi - lo > 1) {
@@ -221,20 +500,232 @@ __global__ __launch_bounds__(NT) void k_tile_batch(vec2_t<T> *__restrict__ base,
                 hi = mid;
         }
         const TileEntry &e = ent[lo];
-        const uint64_t toff = thread_offset<NT>(e.args, threadIdx.x);
-        tile_body<T, M, NT>(base + e.amp_offset, e.args, e.ops, e.nops, b - e.block_start, sa, toff);
-        __syncthreads();  // the next block id may reuse sa with other threads' slots
+        const uint64_t toff = thread_offset<M - RB>(e.args, threadIdx.x);
+        vec2_t<T> x[1 << RB];
+        tile_load<T, M, RB>(base + e.amp_offset, e.args, b - e.block_start, toff, x);
+        tile_ops<T, M, RB, false>(x, e.ops, e.nops, sa);
+        tile_store<T, M, RB>(base + e.amp_offset, e.args, b - e.block_start, toff, x);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+
+int check_tile_ops(uint32_t m, uint32_t nops, const qvmhip_tile_op *ops) {
+    for (uint32_t o = 0; o < nops; ++o) {
+        const qvmhip_tile_op &op = ops[o];
+        if (op.kind < QVMHIP_TILE_DENSE1 || op.kind > QVMHIP_TILE_CPHASE)
+            return fail(QVMHIP_EINVAL, "unknown tile op kind");
+        const bool two = op.kind != QVMHIP_TILE_DENSE1 && op.kind != QVMHIP_TILE_DIAG1;
+        if (op.q0 >= m || (two && (op.q1 >= m || op.q1 == op.q0)))
+            return fail(QVMHIP_EINVAL, "tile op qubit out of range");
+        if (m < 32 && (op.ctrl_mask >> m)) return fail(QVMHIP_EINVAL, "tile op control outside the tile");
+        if ((op.ctrl_mask >> op.q0) & 1u || (two && ((op.ctrl_mask >> op.q1) & 1u)))
+            return fail(QVMHIP_EINVAL, "tile op control overlaps its targets");
+    }
+    return QVMHIP_OK;
+}
+
+namespace {
+
+// Local bits an op needs on register bits (its non-diagonal targets).
+uint32_t reg_need(const qvmhip_tile_op &op) {
+    switch (op.kind) {
+        case QVMHIP_TILE_DENSE1: return 1u << op.q0;
+        case QVMHIP_TILE_DENSE2: return (1u << op.q0) | (1u << op.q1);
+        case QVMHIP_TILE_CPERM: return 1u << op.q0;
+        case QVMHIP_TILE_SWAP: return op.ctrl_mask ? ((1u << op.q0) | (1u << op.q1)) : 0u;
+        default: return 0u;
+    }
+}
+
+struct Layout {
+    int m, tb, rb;
+    int l2s[16], s2l[16];  // local bit <-> slot bit
+    void swap_slots(int a, int b) {
+        const int la = s2l[a], lb = s2l[b];
+        s2l[a] = lb;
+        s2l[b] = la;
+        l2s[la] = b;
+        l2s[lb] = a;
+    }
+    bool identity() const {
+        for (int i = 0; i < m; ++i)
+            if (l2s[i] != i) return false;
+        return true;
+    }
+    uint32_t slot_mask(uint32_t local_mask) const {
+        uint32_t o = 0;
+        for (int i = 0; i < m; ++i)
+            if ((local_mask >> i) & 1u) o |= 1u << l2s[i];
+        return o;
+    }
+};
+
+qvmhip_tile_op remap_op(const Layout &from, const int *new_s2l) {
+    qvmhip_tile_op o;
+    std::memset(&o, 0, sizeof(o));
+    o.kind = kTileRemap;
+    uint8_t src[16] = {0};
+    for (int j = 0; j < from.m; ++j) src[j] = (uint8_t)from.l2s[new_s2l[j]];
+    std::memcpy(o.m, src, sizeof(src));
+    return o;
+}
+
+}  // namespace
+
+// Turn a pass's op list (tile-local bit positions) into the slot-space
+// program the kernel runs.  `pin` places the targets of 2-qubit dense ops and
+// controlled SWAPs on register bits 0 and 1.
+void schedule_tile_ops(int dtype, uint32_t m, const qvmhip_tile_op *in, uint32_t nin,
+                       std::vector<qvmhip_tile_op> &out) {
+    Layout L;
+    L.m = (int)m;
+    L.rb = tile_reg_bits((int)m, dtype == QVMHIP_F64);
+    L.tb = L.m - L.rb;
+    for (int i = 0; i < L.m; ++i) L.l2s[i] = L.s2l[i] = i;
+    out.clear();
+    out.reserve(nin + nin / 2 + 2);
+    std::vector<uint32_t> need(nin);
+    for (uint32_t i = 0; i < nin; ++i) need[i] = reg_need(in[i]);
+    // next_use[i][l]: first op index >= i that needs local bit l on a register
+    std::vector<std::array<uint32_t, 16>> next_use(nin + 1);
+    next_use[nin].fill(UINT32_MAX);
+    for (int i = (int)nin - 1; i >= 0; --i) {
+        next_use[i] = next_use[i + 1];
+        for (int l = 0; l < L.m; ++l)
+            if ((need[i] >> l) & 1u) next_use[i][l] = (uint32_t)i;
+    }
+    auto reg_local_mask = [&]() {
+        uint32_t msk = 0;
+        for (int b = 0; b < L.rb; ++b) msk |= 1u << L.s2l[L.tb + b];
+        return msk;
+    };
+    for (uint32_t i = 0; i < nin; ++i) {
+        const qvmhip_tile_op &op = in[i];
+        if (op.kind == QVMHIP_TILE_SWAP && op.ctrl_mask == 0) {
+            L.swap_slots(L.l2s[op.q0], L.l2s[op.q1]);  // relabel only
+            continue;
+        }
+        const bool pin2 = op.kind == QVMHIP_TILE_DENSE2 || op.kind == QVMHIP_TILE_SWAP;
+        if (need[i] & ~reg_local_mask()) {
+            // New register set: this op's targets, then the local bits with the
+            // nearest next use (ties keep bits that are already registers).
+            std::vector<int> order(L.m);
+            for (int l = 0; l < L.m; ++l) order[l] = l;
+            const uint32_t cur = reg_local_mask();
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+                const uint64_t ka = ((uint64_t)next_use[i][a] << 1) | (((cur >> a) & 1u) ? 0u : 1u);
+                const uint64_t kb = ((uint64_t)next_use[i][b] << 1) | (((cur >> b) & 1u) ? 0u : 1u);
+                return ka < kb;
+            });
+            uint32_t nset = 0;
+            for (int k = 0; k < L.rb; ++k) nset |= 1u << order[k];
+            int ns2l[16];
+            for (int j = 0; j < L.m; ++j) ns2l[j] = L.s2l[j];
+            // register bits: keep residents in place, then fill freed positions
+            bool taken[16] = {false};
+            int placed_l[16];
+            for (int l = 0; l < 16; ++l) placed_l[l] = -1;
+            if (pin2) {
+                ns2l[L.tb + 0] = (int)op.q0;
+                ns2l[L.tb + 1] = (int)op.q1;
+                taken[0] = taken[1] = true;
+                placed_l[op.q0] = L.tb + 0;
+                placed_l[op.q1] = L.tb + 1;
+            }
+            for (int b = 0; b < L.rb; ++b) {
+                const int l = L.s2l[L.tb + b];
+                if (((nset >> l) & 1u) && placed_l[l] < 0 && !taken[b]) {
+                    ns2l[L.tb + b] = l;
+                    taken[b] = true;
+                    placed_l[l] = L.tb + b;
+                }
+            }
+            std::vector<int> entering;
+            for (int l = 0; l < L.m; ++l)
+                if (((nset >> l) & 1u) && placed_l[l] < 0) entering.push_back(l);
+            for (int b = 0, e = 0; b < L.rb; ++b)
+                if (!taken[b]) {
+                    ns2l[L.tb + b] = entering[e];
+                    placed_l[entering[e]] = L.tb + b;
+                    taken[b] = true;
+                    ++e;
+                }
+            // thread bits: the slots vacated by entering bits take the evicted
+            // bits; pair vacated thread bit p with an evicted bit whose old
+            // register position is p when there is one (distinct LDS banks)
+            std::vector<int> vacated, evicted;
+            for (int p = 0; p < L.tb; ++p)
+                if ((nset >> L.s2l[p]) & 1u) vacated.push_back(p);
+            for (int b = 0; b < L.rb; ++b) {
+                const int l = L.s2l[L.tb + b];
+                if (!((nset >> l) & 1u)) evicted.push_back(l);
+            }
+            std::vector<bool> used(evicted.size(), false);
+            for (int p : vacated) {
+                int pick = -1;
+                for (size_t e = 0; e < evicted.size(); ++e)
+                    if (!used[e] && L.l2s[evicted[e]] - L.tb == p) pick = (int)e;
+                if (pick < 0)
+                    for (size_t e = 0; e < evicted.size() && pick < 0; ++e)
+                        if (!used[e]) pick = (int)e;
+                used[pick] = true;
+                ns2l[p] = evicted[pick];
+            }
+            out.push_back(remap_op(L, ns2l));
+            for (int j = 0; j < L.m; ++j) {
+                L.s2l[j] = ns2l[j];
+                L.l2s[ns2l[j]] = j;
+            }
+        } else if (pin2) {
+            // both targets are registers; move them to register bits 0 and 1
+            for (int want = 0; want < 2; ++want) {
+                const int l = want == 0 ? (int)op.q0 : (int)op.q1;
+                const int b = L.l2s[l] - L.tb;
+                if (b != want) {
+                    qvmhip_tile_op r;
+                    std::memset(&r, 0, sizeof(r));
+                    r.kind = kTileRSwap;
+                    r.q0 = (uint32_t)std::min(b, want);
+                    r.q1 = (uint32_t)std::max(b, want);
+                    out.push_back(r);
+                    L.swap_slots(L.tb + b, L.tb + want);
+                }
+            }
+        }
+        qvmhip_tile_op o = op;
+        o.ctrl_mask = L.slot_mask(op.ctrl_mask);
+        o.q0 = (uint32_t)L.l2s[op.q0];
+        const bool two = op.kind != QVMHIP_TILE_DENSE1 && op.kind != QVMHIP_TILE_DIAG1;
+        o.q1 = two ? (uint32_t)L.l2s[op.q1] : 0u;
+        if (op.kind == QVMHIP_TILE_CPERM) {  // X on the target where the control is 1
+            o.kind = kTileXReg;
+            o.ctrl_mask |= 1u << L.l2s[op.q1];
+            o.q1 = 0;
+        }
+        out.push_back(o);
+    }
+    if (!L.identity()) {
+        int ns2l[16];
+        for (int j = 0; j < L.m; ++j) ns2l[j] = j;
+        out.push_back(remap_op(L, ns2l));
+    }
+    if (dtype == QVMHIP_F32) {
+        for (qvmhip_tile_op &o : out) {
+            if (o.kind == kTileRemap || o.kind == kTileRSwap || o.kind == kTileXReg) continue;
+            float f[32];
+            for (int i = 0; i < 32; ++i) f[i] = (float)o.m[i];
+            std::memset(o.m, 0, sizeof(o.m));
+            std::memcpy(o.m, f, sizeof(f));
+        }
     }
 }
 
 namespace {
 
-constexpr int threads_for(int m) { return m >= 12 ? 512 : 256; }
-
 // Host-side tile arguments of an n-qubit state and tile qubits tq[0..m).
-void make_tile_args(uint32_t n, uint32_t m, const uint32_t *tq, TileArgs &a) {
-    const int NT = threads_for((int)m);
-    const int LOGNT = NT == 512 ? 9 : 8;
+void make_tile_args(uint32_t n, uint32_t m, const uint32_t *tq, int tb, TileArgs &a) {
     std::memset(&a, 0, sizeof(a));
     a.g.nsorted = m;
     for (uint32_t i = 0; i < m; ++i) {
@@ -243,10 +734,10 @@ void make_tile_args(uint32_t n, uint32_t m, const uint32_t *tq, TileArgs &a) {
     }
     a.g.ctrl_mask = 0;
     a.g.ngroups = uint64_t(1) << (n - m);
-    const int R = (1 << m) / NT;
+    const int R = 1 << (m - tb);
     for (int r = 0; r < R; ++r) {
         uint64_t o = 0;
-        const uint32_t l = (uint32_t)r << LOGNT;
+        const uint32_t l = (uint32_t)r << tb;
         for (uint32_t b = 0; b < m; ++b)
             if ((l >> b) & 1u) o |= uint64_t(1) << tq[b];
         a.roff[r] = o;
@@ -256,13 +747,18 @@ void make_tile_args(uint32_t n, uint32_t m, const uint32_t *tq, TileArgs &a) {
 template <typename T, int M>
 int run_tile(qvmhip_state *s, const uint32_t *tq, uint32_t m, const qvmhip_tile_op *ops_dev,
              uint32_t nops) {
-    constexpr int NT = threads_for(M);
+    constexpr int RB = tile_reg_bits(M, sizeof(T) == 8);
+    constexpr int NT = 1 << (M - RB);
     TileArgs a;
-    make_tile_args(s->n, m, tq, a);
+    make_tile_args(s->n, m, tq, M - RB, a);
     prof_begin(s, QVMHIP_K_TILE);
     const uint32_t grid = (uint32_t)std::min<uint64_t>(a.g.ngroups, kMaxGrid);
-    hipLaunchKernelGGL((k_tile<T, M, NT>), dim3(grid), dim3(NT), 0, s->stream,
-                       (vec2_t<T> *)s->amps, a, ops_dev, nops);
+    if (s->flags & QVMHIP_FLAG_FMA_TILES)
+        hipLaunchKernelGGL((k_tile<T, M, RB, true>), dim3(grid), dim3(NT), 0, s->stream, (vec2_t<T> *)s->amps, a,
+                           ops_dev, nops);
+    else
+        hipLaunchKernelGGL((k_tile<T, M, RB, false>), dim3(grid), dim3(NT), 0, s->stream, (vec2_t<T> *)s->amps, a,
+                           ops_dev, nops);
     prof_end(s);
     QH_LAUNCH_CHECK();
     return QVMHIP_OK;
@@ -278,7 +774,7 @@ int run_tile_m(qvmhip_state *s, uint32_t m, const uint32_t *tq, const qvmhip_til
         case 11: return run_tile<T, 11>(s, tq, m, ops_dev, nops);
         case 12: return run_tile<T, 12>(s, tq, m, ops_dev, nops);
         case 13:
-            if (sizeof(T) == 4) return run_tile<T, 13>(s, tq, m, ops_dev, nops);
+            if constexpr (sizeof(T) == 4) return run_tile<T, 13>(s, tq, m, ops_dev, nops);
             break;
     }
     return fail(QVMHIP_EINVAL, "tile size m must be 8..12 (F64) or 8..13 (F32)");
@@ -291,15 +787,21 @@ int launch_tile_pass(qvmhip_state *s, uint32_t m, const uint32_t *tile_qubits, u
     const qvmhip_tile_op *dev = ops;
     qvmhip_tile_op *tmp = nullptr;
     if (!ops_on_device) {
+        std::vector<qvmhip_tile_op> sched;
+        schedule_tile_ops(s->dtype, m, ops, nops, sched);
+        nops = (uint32_t)sched.size();
         // Operands go through the matrix slot (64 KiB = 240 ops); longer lists
-        // get a temporary buffer released after the stream drains.
+        // get a temporary buffer released after the stream drains.  The slot
+        // is reused by the next call, so wait for any earlier reader first.
         const size_t bytes = sizeof(qvmhip_tile_op) * (size_t)nops;
         if (bytes <= 64 * 1024) {
-            QH_CHECK(hipMemcpyAsync(s->mat, ops, bytes, hipMemcpyHostToDevice, s->stream));
+            QH_CHECK(hipMemcpyAsync(s->mat, sched.data(), bytes, hipMemcpyHostToDevice, s->stream));
+            QH_CHECK(hipStreamSynchronize(s->stream));  // sched is a host temporary
             dev = (const qvmhip_tile_op *)s->mat;
         } else {
             QH_CHECK(hipMalloc((void **)&tmp, bytes));
-            QH_CHECK(hipMemcpyAsync(tmp, ops, bytes, hipMemcpyHostToDevice, s->stream));
+            QH_CHECK(hipMemcpyAsync(tmp, sched.data(), bytes, hipMemcpyHostToDevice, s->stream));
+            QH_CHECK(hipStreamSynchronize(s->stream));
             dev = tmp;
         }
     }
@@ -312,17 +814,13 @@ int launch_tile_pass(qvmhip_state *s, uint32_t m, const uint32_t *tile_qubits, u
     return rc;
 }
 
-}  // namespace qvmhip
-
-namespace qvmhip {
-
 size_t tile_entry_bytes() { return sizeof(TileEntry); }
 
-void build_tile_entry(void *entry, uint32_t n, uint32_t m, const uint32_t *tq, uint64_t amp_offset,
+void build_tile_entry(void *entry, int dtype, uint32_t n, uint32_t m, const uint32_t *tq, uint64_t amp_offset,
                       uint64_t block_start, const qvmhip_tile_op *ops_dev, uint32_t nops, uint64_t *ntiles) {
     TileEntry e;
     std::memset(&e, 0, sizeof(e));
-    make_tile_args(n, m, tq, e.args);
+    make_tile_args(n, m, tq, (int)m - tile_reg_bits((int)m, dtype == QVMHIP_F64), e.args);
     e.amp_offset = amp_offset;
     e.block_start = block_start;
     e.ops = ops_dev;
@@ -334,9 +832,10 @@ void build_tile_entry(void *entry, uint32_t n, uint32_t m, const uint32_t *tq, u
 namespace {
 template <typename T, int M>
 void tile_batch_go(hipStream_t st, void *base, const void *ent, uint32_t nent, uint64_t total) {
-    constexpr int NT = threads_for(M);
+    constexpr int RB = tile_reg_bits(M, sizeof(T) == 8);
+    constexpr int NT = 1 << (M - RB);
     const uint32_t grid = (uint32_t)std::min<uint64_t>(total, kMaxGrid);
-    hipLaunchKernelGGL((k_tile_batch<T, M, NT>), dim3(grid), dim3(NT), 0, st, (vec2_t<T> *)base,
+    hipLaunchKernelGGL((k_tile_batch<T, M, RB>), dim3(grid), dim3(NT), 0, st, (vec2_t<T> *)base,
                        (const TileEntry *)ent, nent, total);
 }
 template <typename T>
@@ -348,7 +847,7 @@ int tile_batch_m(hipStream_t st, uint32_t m, void *base, const void *ent, uint32
         case 11: tile_batch_go<T, 11>(st, base, ent, nent, total); break;
         case 12: tile_batch_go<T, 12>(st, base, ent, nent, total); break;
         case 13:
-            if (sizeof(T) == 4) {
+            if constexpr (sizeof(T) == 4) {
                 tile_batch_go<T, 13>(st, base, ent, nent, total);
                 break;
             }

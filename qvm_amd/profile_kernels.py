@@ -26,8 +26,9 @@ def main(argv=None) -> int:
                     choices=["dense1", "diag_full", "perm", "diag_sparse", "fused3", "fused4", "fused5", "fused6",
                              "tile"])
     ap.add_argument("--reps", type=int, default=8)
+    ap.add_argument("--fma", action="store_true", help="tile passes with FMA-contracted products")
     a = ap.parse_args(argv)
-    st = HipPureState(a.n, a.dtype)
+    st = HipPureState(a.n, a.dtype, fma_tiles=a.fma)
     rng = np.random.default_rng(0)
     for r in range(a.reps):
         q = int(rng.integers(0, a.n))
@@ -46,7 +47,8 @@ def main(argv=None) -> int:
         elif a.kernel == "tile":
             from . import circuits as C
             from . import planner as P
-            plan = P.plan_tiled(C.layered(a.n, 1, seed=r), a.n, P.default_tile_m(a.dtype), P.default_low(a.dtype))
+            plan = P.plan_tiled(C.layered(a.n, 1, seed=r), a.n, P.default_tile_m(a.dtype), P.default_low(a.dtype),
+                                merge_1q=a.fma)
             P.execute_plan(st, plan)
     st.sync()
     other = HipPureState(a.n, a.dtype)

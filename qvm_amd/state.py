@@ -31,13 +31,18 @@ def _dtype_code(dtype) -> int:
 class HipPureState:
     """A 2^n-amplitude statevector on one GPU (|0...0> on creation)."""
 
-    def __init__(self, num_qubits: int, dtype="f64", device: int = -1, memory_cap_bytes: int = 0):
+    def __init__(self, num_qubits: int, dtype="f64", device: int = -1, memory_cap_bytes: int = 0,
+                 fma_tiles: bool = False):
+        """fma_tiles: tile passes contract complex products with fused
+        multiply-adds (faster; no longer bit-identical to gate-by-gate)."""
         L = N.lib()
         self._h = C.c_void_p(None)
-        cfg = N.Config(C.sizeof(N.Config), int(device), int(memory_cap_bytes), 0)
+        cfg = N.Config(C.sizeof(N.Config), int(device), int(memory_cap_bytes),
+                       N.FLAG_FMA_TILES if fma_tiles else 0)
         N.check(L.qvmhip_create(int(num_qubits), _dtype_code(dtype), C.byref(cfg), C.byref(self._h)))
         self.num_qubits = int(num_qubits)
         self.dtype = "f64" if _dtype_code(dtype) == N.F64 else "f32"
+        self.fma_tiles = bool(fma_tiles)
 
     # -- lifetime ---------------------------------------------------------
     def close(self) -> None:
@@ -249,6 +254,7 @@ class HipPureState:
         if other.num_qubits != self.num_qubits or other.dtype != self.dtype:
             raise ValueError("states differ in size or dtype")
         self._h, other._h = other._h, self._h
+        self.fma_tiles, other.fma_tiles = other.fma_tiles, self.fma_tiles
 
 
 class Program:

@@ -350,6 +350,24 @@ def test_tile_pass_f32(S, oracle):
     st.close()
 
 
+@pytest.mark.parametrize("dtype,m", [("f64", 9), ("f64", 11), ("f64", 12), ("f32", 10), ("f32", 13)])
+def test_tile_passes_fma(S, oracle, dtype, m):
+    """Tile passes with FMA-contracted products (the fast mode): same state
+    as gate-by-gate application within rounding (FP64 1e-13, FP32 2e-6)."""
+    from qvm_amd.state import HipPureState
+    n = 15
+    gates = CI.layered(n, 5, seed=m) + CI.qft(n)[:60] + CI.haar_cz(n, 2, seed=m)
+    plan = P.plan_tiled(gates, n, m=m, low=4, merge_1q=True)
+    say(f"fma tiles {dtype} m={m} passes={len(plan)}")
+    st = HipPureState(n, dtype, fma_tiles=True)
+    P.execute_plan(st, plan)
+    ref = oracle.zero_state(n)
+    for U, q in gates:
+        oracle.apply_matrix(ref, U, G.nat_tuple(*q))
+    assert np.max(np.abs(st.amplitudes() - ref)) < (1e-13 if dtype == "f64" else 2e-6)
+    st.close()
+
+
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 @pytest.mark.parametrize("k", [3, 4, 5, 6])
 def test_fused_mfma_block(S, oracle, k, dtype):

@@ -278,3 +278,151 @@ def test_noise_helpers():
     assert NZ.perturb_measurement(0, 0.9, 0.2, 0.1, 0.8, 0.91) == 1
     assert NZ.perturb_measurement(1, 0.9, 0.2, 0.1, 0.8, 0.2) == 0
     assert NZ.perturb_measurement(1, 0.9, 0.2, 0.1, 0.8, 0.21) == 1
+
+
+# --- tile scheduler: the slot-space program equals the op list ---------------
+
+def _apply_vec(X, op, m, tb=None, f32=False):
+    """Apply one tile op (public kinds in local bits, or the scheduler's
+    slot-space kinds when tb is given) to a 2^m vector.  f32: the scheduled
+    FP32 form, whose operands are stored as floats."""
+    S = 1 << m
+    idx = np.arange(S)
+    mm = np.array([op.m[i] for i in range(32)])
+    if f32 and op.kind not in (N.TILE_REMAP, N.TILE_RSWAP, N.TILE_XREG):
+        mm = np.frombuffer(mm.tobytes(), dtype=np.float32)[:32].astype(np.float64)
+    cm = op.ctrl_mask
+    ok = (idx & cm) == cm
+    k = op.kind
+    if k == N.TILE_REMAP:
+        src = np.frombuffer(mm.tobytes()[:16], dtype=np.uint8)
+        old = np.zeros(S, dtype=np.int64)
+        for j in range(m):
+            old |= ((idx >> j) & 1) << int(src[j])
+        return X[old]
+    if k == N.TILE_RSWAP:
+        A, B = tb + op.q0, tb + op.q1
+        sw = idx.copy()
+        diff = ((idx >> A) & 1) != ((idx >> B) & 1)
+        sw[diff] ^= (1 << A) | (1 << B)
+        return X[sw]
+    X = X.copy()
+    if k == N.TILE_DENSE1:
+        if tb is not None:
+            assert op.q0 >= tb
+        u = mm[:8].view(np.complex128).reshape(2, 2)
+        b = 1 << op.q0
+        a = idx[((idx & b) == 0) & ok]
+        xa, xb = X[a].copy(), X[a | b].copy()
+        X[a], X[a | b] = u[0, 0] * xa + u[0, 1] * xb, u[1, 0] * xa + u[1, 1] * xb
+    elif k == N.TILE_DENSE2:
+        if tb is not None:
+            assert op.q0 == tb and op.q1 == tb + 1
+        u = mm.view(np.complex128).reshape(4, 4)
+        b0, b1 = 1 << op.q0, 1 << op.q1
+        a = idx[((idx & (b0 | b1)) == 0) & ok]
+        y = u @ np.stack([X[a], X[a | b0], X[a | b1], X[a | b0 | b1]])
+        X[a], X[a | b0], X[a | b1], X[a | b0 | b1] = y
+    elif k in (N.TILE_XREG, N.TILE_CPERM):
+        if tb is not None:
+            assert k == N.TILE_XREG and op.q0 >= tb
+        b = 1 << op.q0
+        sel = ((idx & b) == 0) & ok
+        if k == N.TILE_CPERM:
+            sel &= ((idx >> op.q1) & 1) == 1
+        a = idx[sel]
+        X[a], X[a | b] = X[a | b].copy(), X[a].copy()
+    elif k == N.TILE_SWAP:
+        if tb is not None:
+            assert {op.q0, op.q1} == {tb, tb + 1}
+        b0, b1 = 1 << op.q0, 1 << op.q1
+        a = idx[((idx & (b0 | b1)) == 0) & ok]
+        X[a | b0], X[a | b1] = X[a | b1].copy(), X[a | b0].copy()
+    elif k in (N.TILE_DIAG1, N.TILE_DIAG2):
+        d = mm[:8].view(np.complex128)
+        j = (idx >> op.q0) & 1
+        if k == N.TILE_DIAG2:
+            j = j | (((idx >> op.q1) & 1) << 1)
+        X = np.where(ok, X * d[j], X)
+    elif k == N.TILE_CPHASE:
+        need = cm | (1 << op.q0) | (1 << op.q1)
+        X = np.where((idx & need) == need, X * complex(mm[0], mm[1]), X)
+    else:
+        raise AssertionError(k)
+    return X
+
+
+def _emulate_slot_program(sched, m, tb, psi_tile, f32=False):
+    """Numpy model of the register tile kernel: X[s] is the amplitude held
+    in slot s (register s >> tb of thread s & (2^tb - 1)); ops are in slot
+    bits.  The final X must be in the identity layout."""
+    X = psi_tile.copy()
+    for op in sched:
+        X = _apply_vec(X, op, m, tb, f32)
+    return X
+
+
+def _op(kind, q0, q1, ctrl, values):
+    op = N.TileOp()
+    op.kind, op.q0, op.q1, op.ctrl_mask = kind, q0, q1, ctrl
+    v = np.zeros(16, dtype=np.complex128)
+    vals = np.asarray(values, dtype=np.complex128).reshape(-1)
+    v[:vals.size] = vals
+    for i, x in enumerate(v.view(np.float64)):
+        op.m[i] = x
+    return op
+
+
+def _random_tile_ops(m, count, rng):
+    ops = []
+    for _ in range(count):
+        kind = int(rng.choice([1, 1, 2, 3, 4, 5, 5, 6, 6, 7]))
+        q = [int(x) for x in rng.choice(m, 3, replace=False)]
+        ctrl = 0
+        if rng.random() < 0.2:
+            ctrl = 1 << q[2]
+        if kind in (N.TILE_DENSE1,):
+            U = np.linalg.qr(rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2)))[0]
+            ops.append(_op(kind, q[0], 0, ctrl, U.reshape(-1)))
+        elif kind == N.TILE_DENSE2:
+            U = np.linalg.qr(rng.normal(size=(4, 4)) + 1j * rng.normal(size=(4, 4)))[0]
+            ops.append(_op(kind, q[0], q[1], ctrl, U.reshape(-1)))
+        elif kind == N.TILE_DIAG1:
+            ops.append(_op(kind, q[0], 0, ctrl, np.exp(1j * rng.uniform(0, 6, 2))))
+        elif kind == N.TILE_DIAG2:
+            ops.append(_op(kind, q[0], q[1], ctrl, np.exp(1j * rng.uniform(0, 6, 4))))
+        elif kind == N.TILE_CPHASE:
+            ops.append(_op(kind, q[0], q[1], ctrl, np.array([np.exp(1j * rng.uniform(0, 6))])))
+        else:  # CPERM / SWAP
+            ops.append(_op(kind, q[0], q[1], ctrl if kind == N.TILE_CPERM or rng.random() < 0.5 else 0,
+                                 np.zeros(1)))
+    return ops
+
+
+@pytest.mark.parametrize("dtype,m", [("f64", 8), ("f64", 9), ("f64", 11), ("f64", 12), ("f32", 10), ("f32", 13)])
+def test_tile_schedule_matches_op_list(dtype, m):
+    rng = np.random.default_rng(m + (100 if dtype == "f32" else 0))
+    rb = min(m - 6, 4)
+    tb = m - rb
+    for trial in range(3):
+        ops = _random_tile_ops(m, 40, rng)
+        sched = N.tile_schedule(dtype, m, ops)
+        v = rng.normal(size=1 << m) + 1j * rng.normal(size=1 << m)
+        # reference: the op list applied to the tile as an m-qubit state
+        want = v.copy()
+        for op in ops:
+            want = _apply_vec(want, op, m)
+        if m <= 9:  # the vector model against the Kronecker model of the op semantics
+            ref = v.copy()
+            for op in ops:
+                ref = emulate_tile_op(op, list(range(m)), m, ref)
+            assert np.max(np.abs(ref - want)) < 1e-12
+        got = _emulate_slot_program(sched, m, tb, v, dtype == "f32")
+        assert np.max(np.abs(got - want)) < (1e-12 if dtype == "f64" else 1e-4)
+        nremap = sum(1 for o in sched if o.kind == N.TILE_REMAP)
+        assert nremap <= len(ops)
+    # a pass whose targets are all register bits needs no relayout
+    hi = list(range(tb, m))
+    ops = [_op(N.TILE_DENSE1, q, 0, 0, np.eye(2).reshape(-1)) for q in hi]
+    ops += [_op(N.TILE_DIAG1, 0, 0, 0, np.array([1, 1j]))]
+    assert all(o.kind != N.TILE_REMAP for o in N.tile_schedule(dtype, m, ops))
