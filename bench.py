@@ -298,7 +298,7 @@ def precision(args) -> int:
     log(f"precision: f64 plain {t64:.2f} s")
     out["results"]["f64_plain"] = {"s": t64, "amp_updates_per_s": upd / t64, "norm": ref.norm()}
     for mode in ("plain", "tiled"):
-        s32 = HipPureState(n, "f32")
+        s32 = HipPureState(n, "f32", fma_tiles=True)  # the FMA form of the tile passes
         dt = run(s32, gates, mode)
         nrm = s32.norm()
         ip = s32.inner_product(ref) / nrm
@@ -317,7 +317,7 @@ def precision(args) -> int:
     if nb:
         gb = C.haar_cz(nb, 20, seed=3)
         updb = C.amp_updates(gb, nb)
-        big = HipPureState(nb, "f32")
+        big = HipPureState(nb, "f32", fma_tiles=True)
         for mode in ("tiled", "plain"):
             dt = run(big, gb, mode)
             nrm = big.norm()
@@ -348,7 +348,7 @@ def batch(args) -> int:
     hcs = {}
     for dtype in ("f64", "f32"):
         t0 = time.perf_counter()
-        b, _, _ = qaoa_batch(inst, dtype, shots=1024, seed=9, use_graph=True)
+        b, _, _ = qaoa_batch(inst, dtype, shots=1024, seed=9, use_graph=True, fma_tiles=True)
         t_plan = time.perf_counter() - t0
         b.run()
         b.sync()
@@ -368,7 +368,7 @@ def batch(args) -> int:
         b.close()
     out["results"]["f32_max_abs_hc_diff_vs_f64"] = float(np.max(np.abs(hcs["f32"] - hcs["f64"])))
     # eager (uncaptured) FP64 run for the graph's launch-overhead saving
-    b, _, _ = qaoa_batch(inst, "f64", shots=1024, seed=9, use_graph=False)
+    b, _, _ = qaoa_batch(inst, "f64", shots=1024, seed=9, use_graph=False, fma_tiles=True)
     b.run()
     b.sync()
     t0 = time.perf_counter()
@@ -452,8 +452,8 @@ def learned(args) -> int:
         upd = C.amp_updates(gates, n)
         cands = CM.candidate_plans(gates, n, dtype)
         name, plan, pred = cm.choose(gates, n, dtype, cands)
-        a = HipPureState(n, dtype)
-        ref = HipPureState(n, dtype)
+        a = HipPureState(n, dtype, fma_tiles=True)
+        ref = HipPureState(n, dtype, fma_tiles=True)
         t_learned = timed(a, plan)
         t_k4 = timed(ref, cands["fused_k4"])
         t_plain = timed(ref, cands["plain"])  # ref now holds the plain result

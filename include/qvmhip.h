@@ -138,12 +138,14 @@ QVMHIP_API int qvmhip_apply_fused(qvmhip_state *s, const double *U, uint32_t k,
  * the same as qvmhip_apply_matrix, so an in-order pass is bit-identical. */
 QVMHIP_API int qvmhip_apply_tile_pass(qvmhip_state *s, uint32_t m, const uint32_t *tile_qubits,
                                       uint32_t nops, const qvmhip_tile_op *ops);
-/* The slot-space program a tile pass of width m runs for dtype (the op list
+/* The slot-space program a tile pass of width m runs for dtype and config
+ * flags (the op list
  * with register relayouts inserted; no counterpart in the reference -- it
  * exposes the tile scheduler for inspection and tests).  Writes up to cap ops
  * to out and the full count to *nout; QVMHIP_ERANGE when cap is too small. */
-QVMHIP_API int qvmhip_tile_schedule(int dtype, uint32_t m, uint32_t nops, const qvmhip_tile_op *ops,
-                                    qvmhip_tile_op *out, uint32_t cap, uint32_t *nout);
+QVMHIP_API int qvmhip_tile_schedule(int dtype, uint32_t flags, uint32_t m, uint32_t nops,
+                                    const qvmhip_tile_op *ops, qvmhip_tile_op *out, uint32_t cap,
+                                    uint32_t *nout);
 
 /* --- reductions --------------------------------------------------------- */
 /* wavefunction-excited-state-probability: src/wavefunction.lisp:62-70. */
@@ -272,6 +274,9 @@ QVMHIP_API int qvmhip_time_program(qvmhip_program *p, int reps, double *ms);
 typedef struct qvmhip_batch qvmhip_batch;
 QVMHIP_API int qvmhip_batch_create(uint32_t count, const uint32_t *nqubits, int dtype, qvmhip_batch **out);
 QVMHIP_API int qvmhip_batch_destroy(qvmhip_batch *b);
+/* QVMHIP_FLAG_FMA_TILES for the batch's tile passes (before any item is
+ * added); no counterpart in the reference. */
+QVMHIP_API int qvmhip_batch_set_flags(qvmhip_batch *b, uint32_t flags);
 QVMHIP_API int qvmhip_batch_add_tile_pass(qvmhip_batch *b, uint32_t inst, uint32_t m, const uint32_t *tile_qubits,
                                           uint32_t nops, const qvmhip_tile_op *ops);
 /* 1..3-qubit gate, same convention as qvmhip_apply_matrix. */

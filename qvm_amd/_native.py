@@ -84,7 +84,7 @@ SIGNATURES = {
     "qvmhip_apply_permutation": [_vp, _u32p, _u32, _u32p],
     "qvmhip_apply_fused": [_vp, _dp, _u32, _u32p],
     "qvmhip_apply_tile_pass": [_vp, _u32, _u32p, _u32, C.POINTER(TileOp)],
-    "qvmhip_tile_schedule": [C.c_int, _u32, _u32, C.POINTER(TileOp), C.POINTER(TileOp), _u32, _u32p],
+    "qvmhip_tile_schedule": [C.c_int, _u32, _u32, _u32, C.POINTER(TileOp), C.POINTER(TileOp), _u32, _u32p],
     "qvmhip_prob_excited": [_vp, _u32, _dp],
     "qvmhip_prob_ground": [_vp, _u32, _dp],
     "qvmhip_norm": [_vp, _dp],
@@ -117,6 +117,7 @@ SIGNATURES = {
     "qvmhip_profile_summary": [_vp, C.c_int, _u64p, _dp],
     "qvmhip_batch_create": [_u32, _u32p, C.c_int, C.POINTER(_vp)],
     "qvmhip_batch_destroy": [_vp],
+    "qvmhip_batch_set_flags": [_vp, _u32],
     "qvmhip_batch_add_tile_pass": [_vp, _u32, _u32, _u32p, _u32, C.POINTER(TileOp)],
     "qvmhip_batch_add_matrix": [_vp, _u32, _dp, _u32, _u32p],
     "qvmhip_batch_set_zz": [_vp, _u32, _u32p, _u32],
@@ -204,12 +205,13 @@ def cmat(U) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(U, dtype=np.complex128))
 
 
-def tile_schedule(dtype: str, m: int, ops) -> list:
+def tile_schedule(dtype: str, m: int, ops, fma: bool = False) -> list:
     """The slot-space program the engine runs for a tile pass (see
     qvmhip_tile_schedule)."""
     arr = (TileOp * max(1, len(ops)))(*ops)
     cap = 4 * len(ops) + 8
     out = (TileOp * cap)()
     n = C.c_uint32()
-    check(lib().qvmhip_tile_schedule(F64 if dtype == "f64" else F32, m, len(ops), arr, out, cap, C.byref(n)))
+    check(lib().qvmhip_tile_schedule(F64 if dtype == "f64" else F32, FLAG_FMA_TILES if fma else 0, m, len(ops),
+                                     arr, out, cap, C.byref(n)))
     return [out[i] for i in range(n.value)]

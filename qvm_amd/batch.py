@@ -27,7 +27,8 @@ def tile_width(n: int, dtype: str) -> int:
 
 class Batch:
     def __init__(self, circuits: Sequence[Sequence], nqubits: Sequence[int], dtype: str = "f64",
-                 edges: Sequence | None = None, shots: int = 0, uniforms=None, use_graph: bool = True):
+                 edges: Sequence | None = None, shots: int = 0, uniforms=None, use_graph: bool = True,
+                 fma_tiles: bool = False):
         """circuits[i]: list of (U, quil_qubits) on nqubits[i] qubits
         (every n >= 8); edges[i]: MaxCut edges for <H_C>; uniforms:
         count*shots draws in [0,1], instance-major."""
@@ -39,6 +40,8 @@ class Batch:
         nqa = N.u32arr(self.nq)
         N.check(N.lib().qvmhip_batch_create(self.count, N.u32ptr(nqa), N.F64 if dtype == "f64" else N.F32,
                                             C.byref(self._h)))
+        if fma_tiles:
+            N.check(N.lib().qvmhip_batch_set_flags(self._h, N.FLAG_FMA_TILES))
         self.items = 0
         for i, gates in enumerate(circuits):
             n = self.nq[i]
@@ -97,12 +100,13 @@ class Batch:
             pass
 
 
-def qaoa_batch(instances, dtype: str = "f64", shots: int = 1024, seed: int = 9, use_graph: bool = True):
+def qaoa_batch(instances, dtype: str = "f64", shots: int = 1024, seed: int = 9, use_graph: bool = True,
+               fma_tiles: bool = False):
     """Batch for (n, edges, gammas, betas) instances (circuits.qaoa_instances)."""
     circuits = [CI.qaoa_maxcut(n, e, g, b) for n, e, g, b in instances]
     u = CI.uniforms(seed, len(instances) * shots) if shots else None
     return Batch(circuits, [n for n, _, _, _ in instances], dtype, edges=[e for _, e, _, _ in instances],
-                 shots=shots, uniforms=u, use_graph=use_graph), circuits, u
+                 shots=shots, uniforms=u, use_graph=use_graph, fma_tiles=fma_tiles), circuits, u
 
 
 def expected_cut(nedges: int, hc: float) -> float:

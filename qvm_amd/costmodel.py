@@ -7,7 +7,8 @@ k = 2..6, tiled with m = 10..12 -- by summing the predictions, and runs the
 cheapest; the fusion width and the qubits named as tile qubits (the qubit
 permutation into LDS) are what the model chooses.  The network is trained and
 evaluated by the engine's own HIP kernels (qvm_amd/csrc/mlp.hip) on timings
-the engine records with HIP events (``collect``).
+the engine records with HIP events (``collect``) on states whose tile passes
+use the FMA form (the fast mode the planner chooses for).
 
 The target is log(ms per GB of state): launches of any n share one scale, and
 the model's prediction for n is exp(y) * state_GB.
@@ -192,7 +193,7 @@ def collect(ns: Sequence[int], dtypes: Sequence[str], per_config: int, seed: int
     recs = []
     for n in ns:
         for dt in dtypes:
-            st = HipPureState(n, dt)
+            st = HipPureState(n, dt, fma_tiles=True)
             for it in random_launches(n, rng, per_config):
                 ms = _time_items(st, [it])
                 recs.append({"n": n, "dtype": dt, "kind": it.kind, "width": len(it.qubits),

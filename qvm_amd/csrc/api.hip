@@ -264,16 +264,17 @@ int qvmhip_apply_tile_pass(qvmhip_state *s, uint32_t m, const uint32_t *tile_qub
     return launch_tile_pass(s, m, tile_qubits, nops, ops, 0);
 }
 
-int qvmhip_tile_schedule(int dtype, uint32_t m, uint32_t nops, const qvmhip_tile_op *ops, qvmhip_tile_op *out,
-                         uint32_t cap, uint32_t *nout) {
+int qvmhip_tile_schedule(int dtype, uint32_t flags, uint32_t m, uint32_t nops, const qvmhip_tile_op *ops,
+                         qvmhip_tile_op *out, uint32_t cap, uint32_t *nout) {
     if (!nout || (nops && !ops) || (cap && !out)) return fail(QVMHIP_EINVAL, "null argument");
     if (dtype != QVMHIP_F64 && dtype != QVMHIP_F32) return fail(QVMHIP_EINVAL, "unknown dtype");
+    if (flags & ~QVMHIP_FLAG_FMA_TILES) return fail(QVMHIP_EINVAL, "unknown flags");
     if (m < 8 || m > 13 || (m == 13 && dtype == QVMHIP_F64))
         return fail(QVMHIP_EINVAL, "tile size m must be 8..12 (F64) or 8..13 (F32)");
     int rc = check_tile_ops(m, nops, ops);
     if (rc) return rc;
     std::vector<qvmhip_tile_op> sched;
-    schedule_tile_ops(dtype, m, ops, nops, sched);
+    schedule_tile_ops(dtype, (flags & QVMHIP_FLAG_FMA_TILES) != 0, m, ops, nops, sched);
     *nout = (uint32_t)sched.size();
     if (sched.size() > cap) return fail(QVMHIP_ERANGE, "output capacity too small");
     std::copy(sched.begin(), sched.end(), out);

@@ -69,6 +69,7 @@ __global__ void k_batch_ones(vec2_t<T> *__restrict__ buf, const uint64_t *__rest
 struct qvmhip_batch {
     uint32_t count = 0;
     int dtype = QVMHIP_F64;
+    uint32_t flags = 0;  // QVMHIP_FLAG_* (FMA-contracted tile passes)
     size_t amp_bytes = 16;
     std::vector<uint32_t> nq;
     std::vector<uint64_t> offset;  // amplitude offset of each state
@@ -144,7 +145,7 @@ int enqueue(qvmhip_batch *b) {
     for (const BLaunch &L : b->launches) {
         int rc;
         if (L.kind == kItemTile)
-            rc = launch_tile_batch(b->stream, b->dtype, L.m, b->buf, (const char *)b->entries_dev + L.ent_off, L.nent,
+            rc = launch_tile_batch(b->stream, b->dtype, (b->flags & QVMHIP_FLAG_FMA_TILES) != 0, L.m, b->buf, (const char *)b->entries_dev + L.ent_off, L.nent,
                                    L.blocks);
         else
             rc = launch_gate(&b->views[L.inst], L.item->U.data(), L.item->k, L.item->qubits, 0);
@@ -232,6 +233,15 @@ int qvmhip_batch_create(uint32_t count, const uint32_t *nqubits, int dtype, qvmh
     return QVMHIP_OK;
 }
 
+int qvmhip_batch_set_flags(qvmhip_batch *b, uint32_t flags) {
+    if (!b) return fail(QVMHIP_EINVAL, "null batch");
+    if (flags & ~QVMHIP_FLAG_FMA_TILES) return fail(QVMHIP_EINVAL, "unknown batch flags");
+    for (const auto &p : b->prog)
+        if (!p.empty()) return fail(QVMHIP_EINVAL, "batch flags must be set before items are added");
+    b->flags = flags;
+    return QVMHIP_OK;
+}
+
 int qvmhip_batch_add_tile_pass(qvmhip_batch *b, uint32_t inst, uint32_t m, const uint32_t *tile_qubits,
                                uint32_t nops, const qvmhip_tile_op *ops) {
     int rc = check_inst(b, inst);
@@ -252,7 +262,7 @@ int qvmhip_batch_add_tile_pass(qvmhip_batch *b, uint32_t inst, uint32_t m, const
     it.k = m;
     std::copy(tile_qubits, tile_qubits + m, it.qubits);
     std::vector<qvmhip_tile_op> sched;
-    schedule_tile_ops(b->dtype, m, ops, nops, sched);
+    schedule_tile_ops(b->dtype, (b->flags & QVMHIP_FLAG_FMA_TILES) != 0, m, ops, nops, sched);
     it.nops = (uint32_t)sched.size();
     it.op_off = push(b->blob, sched.data(), sizeof(qvmhip_tile_op) * sched.size());
     b->prog[inst].push_back(std::move(it));
@@ -344,7 +354,7 @@ int qvmhip_batch_finalize(qvmhip_batch *b, int use_graph) {
                 const BItem &it = b->prog[i][L];
                 std::vector<char> ent(esz);
                 uint64_t nt = 0;
-                build_tile_entry(ent.data(), b->dtype, b->nq[i], it.k, it.qubits, b->offset[i], blocks,
+                build_tile_entry(ent.data(), b->dtype, (b->flags & QVMHIP_FLAG_FMA_TILES) != 0, b->nq[i], it.k, it.qubits, b->offset[i], blocks,
                                  (const qvmhip_tile_op *)((const char *)b->blob_dev + it.op_off), it.nops, &nt);
                 b->entries.insert(b->entries.end(), ent.begin(), ent.end());
                 blocks += nt;

@@ -142,7 +142,7 @@ int qvmhip_program_add_tile_pass(qvmhip_program *p, uint32_t m, const uint32_t *
     it.k = m;
     std::copy(tile_qubits, tile_qubits + m, it.qubits);
     std::vector<qvmhip_tile_op> sched;
-    schedule_tile_ops(p->s->dtype, m, ops, nops, sched);
+    schedule_tile_ops(p->s->dtype, (p->s->flags & QVMHIP_FLAG_FMA_TILES) != 0, m, ops, nops, sched);
     it.nops = (uint32_t)sched.size();
     it.bytes = sizeof(qvmhip_tile_op) * sched.size();
     it.host_off = push_blob(p, sched.data(), it.bytes);

@@ -184,13 +184,14 @@ int sample_to_device(qvmhip_state *s, const double *u_dev, uint64_t nshots, uint
 // Tile passes: validation of a public op list (tile-local bit positions) and
 // its translation into the slot-space program the tile kernels run.
 int check_tile_ops(uint32_t m, uint32_t nops, const qvmhip_tile_op *ops);
-void schedule_tile_ops(int dtype, uint32_t m, const qvmhip_tile_op *in, uint32_t nin,
+void schedule_tile_ops(int dtype, bool fma, uint32_t m, const qvmhip_tile_op *in, uint32_t nin,
                        std::vector<qvmhip_tile_op> &out);
 size_t tile_entry_bytes();
-void build_tile_entry(void *entry, int dtype, uint32_t n, uint32_t m, const uint32_t *tq, uint64_t amp_offset,
-                      uint64_t block_start, const qvmhip_tile_op *ops_dev, uint32_t nops, uint64_t *ntiles);
-int launch_tile_batch(hipStream_t st, int dtype, uint32_t m, void *base, const void *entries_dev, uint32_t nent,
-                      uint64_t total_blocks);
+void build_tile_entry(void *entry, int dtype, bool fma, uint32_t n, uint32_t m, const uint32_t *tq,
+                      uint64_t amp_offset, uint64_t block_start, const qvmhip_tile_op *ops_dev, uint32_t nops,
+                      uint64_t *ntiles);
+int launch_tile_batch(hipStream_t st, int dtype, bool fma, uint32_t m, void *base, const void *entries_dev,
+                      uint32_t nent, uint64_t total_blocks);
 }  // namespace qvmhip
 
 #define QH_CHECK(expr)                                                                    \

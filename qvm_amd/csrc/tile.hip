@@ -40,11 +40,11 @@ constexpr uint32_t kTileXReg = 16;   // X on register bit (q0 - TB) where ctrl_m
 constexpr uint32_t kTileRSwap = 17;  // exchange register bits q0 and q1 (both < RB)
 constexpr uint32_t kTileRemap = 18;  // relayout through LDS; new slot bit j <- old bit src[j]
 
-// Register bits per thread: 4 (16 amplitudes; 32 registers would spill),
-// fewer for small tiles so a workgroup keeps at least 64 threads.
-__host__ __device__ constexpr int tile_reg_bits(int m, bool f64) {
-    (void)f64;
-    return (m - 6) < 4 ? (m - 6) : 4;
+// Register bits per thread (measured on MI355X): 3 (8 amplitudes) for FP64
+// and for exact FP32, 4 (16) for FP32 with FMA contraction; fewer for small
+// tiles so a workgroup keeps at least 64 threads.
+__host__ __device__ constexpr int tile_reg_bits(int m, bool f64, bool fma) {
+    return (m - 6) < ((!f64 && fma) ? 4 : 3) ? (m - 6) : ((!f64 && fma) ? 4 : 3);
 }
 
 struct TileArgs {
@@ -485,7 +485,7 @@ struct TileEntry {
     uint32_t pad;
 };
 
-template <typename T, int M, int RB>
+template <typename T, int M, int RB, bool FMA>
 __global__ __launch_bounds__(1 << (M - RB)) void k_tile_batch(vec2_t<T> *__restrict__ base,
                                                              const TileEntry *__restrict__ ent, uint32_t nent,
                                                              uint64_t total_blocks) {
@@ -503,7 +503,7 @@ __global__ __launch_bounds__(1 << (M - RB)) void k_tile_batch(vec2_t<T> *__restr
         const uint64_t toff = thread_offset<M - RB>(e.args, threadIdx.x);
         vec2_t<T> x[1 << RB];
         tile_load<T, M, RB>(base + e.amp_offset, e.args, b - e.block_start, toff, x);
-        tile_ops<T, M, RB, false>(x, e.ops, e.nops, sa);
+        tile_ops<T, M, RB, FMA>(x, e.ops, e.nops, sa);
         tile_store<T, M, RB>(base + e.amp_offset, e.args, b - e.block_start, toff, x);
     }
 }
@@ -577,11 +577,11 @@ qvmhip_tile_op remap_op(const Layout &from, const int *new_s2l) {
 // Turn a pass's op list (tile-local bit positions) into the slot-space
 // program the kernel runs.  `pin` places the targets of 2-qubit dense ops and
 // controlled SWAPs on register bits 0 and 1.
-void schedule_tile_ops(int dtype, uint32_t m, const qvmhip_tile_op *in, uint32_t nin,
+void schedule_tile_ops(int dtype, bool fma, uint32_t m, const qvmhip_tile_op *in, uint32_t nin,
                        std::vector<qvmhip_tile_op> &out) {
     Layout L;
     L.m = (int)m;
-    L.rb = tile_reg_bits((int)m, dtype == QVMHIP_F64);
+    L.rb = tile_reg_bits((int)m, dtype == QVMHIP_F64, fma);
     L.tb = L.m - L.rb;
     for (int i = 0; i < L.m; ++i) L.l2s[i] = L.s2l[i] = i;
     out.clear();
@@ -744,37 +744,39 @@ void make_tile_args(uint32_t n, uint32_t m, const uint32_t *tq, int tb, TileArgs
     }
 }
 
-template <typename T, int M>
+template <typename T, int M, bool FMA>
 int run_tile(qvmhip_state *s, const uint32_t *tq, uint32_t m, const qvmhip_tile_op *ops_dev,
              uint32_t nops) {
-    constexpr int RB = tile_reg_bits(M, sizeof(T) == 8);
+    constexpr int RB = tile_reg_bits(M, sizeof(T) == 8, FMA);
     constexpr int NT = 1 << (M - RB);
     TileArgs a;
     make_tile_args(s->n, m, tq, M - RB, a);
     prof_begin(s, QVMHIP_K_TILE);
     const uint32_t grid = (uint32_t)std::min<uint64_t>(a.g.ngroups, kMaxGrid);
-    if (s->flags & QVMHIP_FLAG_FMA_TILES)
-        hipLaunchKernelGGL((k_tile<T, M, RB, true>), dim3(grid), dim3(NT), 0, s->stream, (vec2_t<T> *)s->amps, a,
-                           ops_dev, nops);
-    else
-        hipLaunchKernelGGL((k_tile<T, M, RB, false>), dim3(grid), dim3(NT), 0, s->stream, (vec2_t<T> *)s->amps, a,
-                           ops_dev, nops);
+    hipLaunchKernelGGL((k_tile<T, M, RB, FMA>), dim3(grid), dim3(NT), 0, s->stream, (vec2_t<T> *)s->amps, a,
+                       ops_dev, nops);
     prof_end(s);
     QH_LAUNCH_CHECK();
     return QVMHIP_OK;
+}
+
+template <typename T, int M>
+int run_tile_f(qvmhip_state *s, const uint32_t *tq, uint32_t m, const qvmhip_tile_op *ops_dev, uint32_t nops) {
+    return (s->flags & QVMHIP_FLAG_FMA_TILES) ? run_tile<T, M, true>(s, tq, m, ops_dev, nops)
+                                              : run_tile<T, M, false>(s, tq, m, ops_dev, nops);
 }
 
 template <typename T>
 int run_tile_m(qvmhip_state *s, uint32_t m, const uint32_t *tq, const qvmhip_tile_op *ops_dev,
                uint32_t nops) {
     switch (m) {
-        case 8: return run_tile<T, 8>(s, tq, m, ops_dev, nops);
-        case 9: return run_tile<T, 9>(s, tq, m, ops_dev, nops);
-        case 10: return run_tile<T, 10>(s, tq, m, ops_dev, nops);
-        case 11: return run_tile<T, 11>(s, tq, m, ops_dev, nops);
-        case 12: return run_tile<T, 12>(s, tq, m, ops_dev, nops);
+        case 8: return run_tile_f<T, 8>(s, tq, m, ops_dev, nops);
+        case 9: return run_tile_f<T, 9>(s, tq, m, ops_dev, nops);
+        case 10: return run_tile_f<T, 10>(s, tq, m, ops_dev, nops);
+        case 11: return run_tile_f<T, 11>(s, tq, m, ops_dev, nops);
+        case 12: return run_tile_f<T, 12>(s, tq, m, ops_dev, nops);
         case 13:
-            if constexpr (sizeof(T) == 4) return run_tile<T, 13>(s, tq, m, ops_dev, nops);
+            if constexpr (sizeof(T) == 4) return run_tile_f<T, 13>(s, tq, m, ops_dev, nops);
             break;
     }
     return fail(QVMHIP_EINVAL, "tile size m must be 8..12 (F64) or 8..13 (F32)");
@@ -788,7 +790,7 @@ int launch_tile_pass(qvmhip_state *s, uint32_t m, const uint32_t *tile_qubits, u
     qvmhip_tile_op *tmp = nullptr;
     if (!ops_on_device) {
         std::vector<qvmhip_tile_op> sched;
-        schedule_tile_ops(s->dtype, m, ops, nops, sched);
+        schedule_tile_ops(s->dtype, (s->flags & QVMHIP_FLAG_FMA_TILES) != 0, m, ops, nops, sched);
         nops = (uint32_t)sched.size();
         // Operands go through the matrix slot (64 KiB = 240 ops); longer lists
         // get a temporary buffer released after the stream drains.  The slot
@@ -816,11 +818,12 @@ int launch_tile_pass(qvmhip_state *s, uint32_t m, const uint32_t *tile_qubits, u
 
 size_t tile_entry_bytes() { return sizeof(TileEntry); }
 
-void build_tile_entry(void *entry, int dtype, uint32_t n, uint32_t m, const uint32_t *tq, uint64_t amp_offset,
-                      uint64_t block_start, const qvmhip_tile_op *ops_dev, uint32_t nops, uint64_t *ntiles) {
+void build_tile_entry(void *entry, int dtype, bool fma, uint32_t n, uint32_t m, const uint32_t *tq,
+                      uint64_t amp_offset, uint64_t block_start, const qvmhip_tile_op *ops_dev, uint32_t nops,
+                      uint64_t *ntiles) {
     TileEntry e;
     std::memset(&e, 0, sizeof(e));
-    make_tile_args(n, m, tq, (int)m - tile_reg_bits((int)m, dtype == QVMHIP_F64), e.args);
+    make_tile_args(n, m, tq, (int)m - tile_reg_bits((int)m, dtype == QVMHIP_F64, fma), e.args);
     e.amp_offset = amp_offset;
     e.block_start = block_start;
     e.ops = ops_dev;
@@ -830,25 +833,25 @@ void build_tile_entry(void *entry, int dtype, uint32_t n, uint32_t m, const uint
 }
 
 namespace {
-template <typename T, int M>
+template <typename T, int M, bool FMA>
 void tile_batch_go(hipStream_t st, void *base, const void *ent, uint32_t nent, uint64_t total) {
-    constexpr int RB = tile_reg_bits(M, sizeof(T) == 8);
+    constexpr int RB = tile_reg_bits(M, sizeof(T) == 8, FMA);
     constexpr int NT = 1 << (M - RB);
     const uint32_t grid = (uint32_t)std::min<uint64_t>(total, kMaxGrid);
-    hipLaunchKernelGGL((k_tile_batch<T, M, RB>), dim3(grid), dim3(NT), 0, st, (vec2_t<T> *)base,
+    hipLaunchKernelGGL((k_tile_batch<T, M, RB, FMA>), dim3(grid), dim3(NT), 0, st, (vec2_t<T> *)base,
                        (const TileEntry *)ent, nent, total);
 }
-template <typename T>
+template <typename T, bool FMA>
 int tile_batch_m(hipStream_t st, uint32_t m, void *base, const void *ent, uint32_t nent, uint64_t total) {
     switch (m) {
-        case 8: tile_batch_go<T, 8>(st, base, ent, nent, total); break;
-        case 9: tile_batch_go<T, 9>(st, base, ent, nent, total); break;
-        case 10: tile_batch_go<T, 10>(st, base, ent, nent, total); break;
-        case 11: tile_batch_go<T, 11>(st, base, ent, nent, total); break;
-        case 12: tile_batch_go<T, 12>(st, base, ent, nent, total); break;
+        case 8: tile_batch_go<T, 8, FMA>(st, base, ent, nent, total); break;
+        case 9: tile_batch_go<T, 9, FMA>(st, base, ent, nent, total); break;
+        case 10: tile_batch_go<T, 10, FMA>(st, base, ent, nent, total); break;
+        case 11: tile_batch_go<T, 11, FMA>(st, base, ent, nent, total); break;
+        case 12: tile_batch_go<T, 12, FMA>(st, base, ent, nent, total); break;
         case 13:
             if constexpr (sizeof(T) == 4) {
-                tile_batch_go<T, 13>(st, base, ent, nent, total);
+                tile_batch_go<T, 13, FMA>(st, base, ent, nent, total);
                 break;
             }
             return fail(QVMHIP_EINVAL, "tile size 13 is FP32 only");
@@ -860,10 +863,13 @@ int tile_batch_m(hipStream_t st, uint32_t m, void *base, const void *ent, uint32
 }
 }  // namespace
 
-int launch_tile_batch(hipStream_t st, int dtype, uint32_t m, void *base, const void *entries_dev, uint32_t nent,
-                      uint64_t total_blocks) {
-    return dtype == QVMHIP_F64 ? tile_batch_m<double>(st, m, base, entries_dev, nent, total_blocks)
-                               : tile_batch_m<float>(st, m, base, entries_dev, nent, total_blocks);
+int launch_tile_batch(hipStream_t st, int dtype, bool fma, uint32_t m, void *base, const void *entries_dev,
+                      uint32_t nent, uint64_t total_blocks) {
+    if (fma)
+        return dtype == QVMHIP_F64 ? tile_batch_m<double, true>(st, m, base, entries_dev, nent, total_blocks)
+                                   : tile_batch_m<float, true>(st, m, base, entries_dev, nent, total_blocks);
+    return dtype == QVMHIP_F64 ? tile_batch_m<double, false>(st, m, base, entries_dev, nent, total_blocks)
+                               : tile_batch_m<float, false>(st, m, base, entries_dev, nent, total_blocks);
 }
 
 }  // namespace qvmhip

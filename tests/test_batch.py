@@ -22,11 +22,12 @@ def zz_expectation(psi, edges):
     return sum(float(np.sum(p * (1 - 2 * (((idx >> a) ^ (idx >> b)) & 1)))) for a, b in edges)
 
 
-def test_qaoa_batch_matches_oracle(oracle):
+@pytest.mark.parametrize("fma", [False, True])
+def test_qaoa_batch_matches_oracle(oracle, fma):
     from qvm_amd.batch import qaoa_batch
     inst = CI.qaoa_instances(10, seed=4, nmin=10, nmax=16)
     shots = 512
-    b, circuits, u = qaoa_batch(inst, "f64", shots=shots, seed=3)
+    b, circuits, u = qaoa_batch(inst, "f64", shots=shots, seed=3, fma_tiles=fma)
     b.run()
     hc, samples = b.results()
     b.run()  # graph replay: identical results
@@ -54,7 +55,7 @@ def test_qaoa_batch_f32_and_eager(oracle):
     b64, circuits, _ = qaoa_batch(inst, "f64", shots=0, use_graph=False)
     b64.run()
     h64, _ = b64.results()
-    b32, _, _ = qaoa_batch(inst, "f32", shots=0)
+    b32, _, _ = qaoa_batch(inst, "f32", shots=0, fma_tiles=True)
     b32.run()
     h32, _ = b32.results()
     assert np.max(np.abs(h64 - h32)) < 1e-4

@@ -94,7 +94,7 @@ def test_learned_planner_beats_fixed_width():
     totals = {"learned": 0.0, "fixed_k4": 0.0, "plain": 0.0}
     for i, dtype in enumerate(["f64", "f32", "f64"]):
         gates = CI.random_mix(n, 12, seed=100 + i)
-        st = HipPureState(n, dtype)
+        st = HipPureState(n, dtype, fma_tiles=True)
         cands = CM.candidate_plans(gates, n, dtype)
         name, plan, costs = cm.choose(gates, n, dtype, cands)
         measured = {}

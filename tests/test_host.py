@@ -399,14 +399,15 @@ def _random_tile_ops(m, count, rng):
     return ops
 
 
-@pytest.mark.parametrize("dtype,m", [("f64", 8), ("f64", 9), ("f64", 11), ("f64", 12), ("f32", 10), ("f32", 13)])
-def test_tile_schedule_matches_op_list(dtype, m):
+@pytest.mark.parametrize("dtype,m,fma", [("f64", 8, False), ("f64", 9, False), ("f64", 11, False), ("f64", 12, True),
+                                         ("f32", 10, False), ("f32", 12, True), ("f32", 13, True)])
+def test_tile_schedule_matches_op_list(dtype, m, fma):
     rng = np.random.default_rng(m + (100 if dtype == "f32" else 0))
-    rb = min(m - 6, 4)
+    rb = min(m - 6, 4 if (dtype == "f32" and fma) else 3)
     tb = m - rb
     for trial in range(3):
         ops = _random_tile_ops(m, 40, rng)
-        sched = N.tile_schedule(dtype, m, ops)
+        sched = N.tile_schedule(dtype, m, ops, fma)
         v = rng.normal(size=1 << m) + 1j * rng.normal(size=1 << m)
         # reference: the op list applied to the tile as an m-qubit state
         want = v.copy()
@@ -425,4 +426,4 @@ def test_tile_schedule_matches_op_list(dtype, m):
     hi = list(range(tb, m))
     ops = [_op(N.TILE_DENSE1, q, 0, 0, np.eye(2).reshape(-1)) for q in hi]
     ops += [_op(N.TILE_DIAG1, 0, 0, 0, np.array([1, 1j]))]
-    assert all(o.kind != N.TILE_REMAP for o in N.tile_schedule(dtype, m, ops))
+    assert all(o.kind != N.TILE_REMAP for o in N.tile_schedule(dtype, m, ops, fma))
