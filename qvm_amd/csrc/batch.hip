@@ -32,12 +32,17 @@ struct BItem {
     std::vector<double> U;     // gate matrix (k <= 3)
 };
 
+// Tiles per work block of the batched tile kernel: enough to amortise the
+// entry lookup, few enough to spread small states over many workgroups.
+constexpr uint32_t kTilesPerBlock = 4;
+
 struct BLaunch {
     int kind;
     uint32_t m;         // tile width
     size_t ent_off;     // bytes into the entry buffer
     uint32_t nent;
-    uint64_t blocks;
+    uint64_t blocks;    // work blocks
+    size_t map_off;     // first element of this launch's block -> entry map
     uint32_t inst;      // gate launches
     const BItem *item;  // gate launches
 };
@@ -96,6 +101,8 @@ struct qvmhip_batch {
     std::vector<BLaunch> launches;
     std::vector<char> entries;
     void *entries_dev = nullptr;
+    std::vector<uint32_t> blkmap;  // per tile launch: work block -> entry (launch-local index)
+    uint32_t *blkmap_dev = nullptr;
     bool finalized = false, use_graph = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -109,7 +116,7 @@ void free_batch(qvmhip_batch *b) {
     if (b->graph) hipGraphDestroy(b->graph);
     for (void *p : {(void *)b->buf, (void *)b->red, (void *)b->chunk_sum, (void *)b->chunk_pre, (void *)b->scan_tmp,
                     b->mat, b->blob_dev, (void *)b->edges_dev, (void *)b->exp_dev, (void *)b->u_dev,
-                    (void *)b->out_dev, (void *)b->starts_dev, b->entries_dev})
+                    (void *)b->out_dev, (void *)b->starts_dev, b->entries_dev, (void *)b->blkmap_dev})
         if (p) hipFree(p);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
@@ -145,8 +152,9 @@ int enqueue(qvmhip_batch *b) {
     for (const BLaunch &L : b->launches) {
         int rc;
         if (L.kind == kItemTile)
-            rc = launch_tile_batch(b->stream, b->dtype, (b->flags & QVMHIP_FLAG_FMA_TILES) != 0, L.m, b->buf, (const char *)b->entries_dev + L.ent_off, L.nent,
-                                   L.blocks);
+            rc = launch_tile_batch(b->stream, b->dtype, (b->flags & QVMHIP_FLAG_FMA_TILES) != 0, L.m, b->buf,
+                                   (const char *)b->entries_dev + L.ent_off, b->blkmap_dev + L.map_off, L.blocks,
+                                   kTilesPerBlock);
         else
             rc = launch_gate(&b->views[L.inst], L.item->U.data(), L.item->k, L.item->qubits, 0);
         if (rc) return rc;
@@ -345,19 +353,24 @@ int qvmhip_batch_finalize(qvmhip_batch *b, int use_graph) {
             if (it.kind == kItemTile)
                 by_m[it.k].push_back(i);
             else
-                b->launches.push_back({kItemGate, 0, 0, 0, 0, i, &it});
+                b->launches.push_back({kItemGate, 0, 0, 0, 0, 0, i, &it});
         }
         for (auto &kv : by_m) {
-            BLaunch bl{kItemTile, kv.first, b->entries.size(), (uint32_t)kv.second.size(), 0, 0, nullptr};
+            BLaunch bl{kItemTile, kv.first, b->entries.size(), (uint32_t)kv.second.size(), 0, b->blkmap.size(), 0,
+                       nullptr};
             uint64_t blocks = 0;
+            uint32_t local = 0;
             for (uint32_t i : kv.second) {
                 const BItem &it = b->prog[i][L];
                 std::vector<char> ent(esz);
-                uint64_t nt = 0;
-                build_tile_entry(ent.data(), b->dtype, (b->flags & QVMHIP_FLAG_FMA_TILES) != 0, b->nq[i], it.k, it.qubits, b->offset[i], blocks,
-                                 (const qvmhip_tile_op *)((const char *)b->blob_dev + it.op_off), it.nops, &nt);
+                const uint64_t nb = build_tile_entry(
+                    ent.data(), b->dtype, (b->flags & QVMHIP_FLAG_FMA_TILES) != 0, b->nq[i], it.k, it.qubits,
+                    b->offset[i], blocks, (const qvmhip_tile_op *)((const char *)b->blob_dev + it.op_off), it.nops,
+                    kTilesPerBlock);
                 b->entries.insert(b->entries.end(), ent.begin(), ent.end());
-                blocks += nt;
+                b->blkmap.insert(b->blkmap.end(), (size_t)nb, local);
+                blocks += nb;
+                ++local;
             }
             bl.blocks = blocks;
             b->launches.push_back(bl);
@@ -366,6 +379,9 @@ int qvmhip_batch_finalize(qvmhip_batch *b, int use_graph) {
     if (!b->entries.empty()) {
         QH_CHECK(hipMalloc(&b->entries_dev, b->entries.size()));
         QH_CHECK(hipMemcpy(b->entries_dev, b->entries.data(), b->entries.size(), hipMemcpyHostToDevice));
+        QH_CHECK(hipMalloc((void **)&b->blkmap_dev, b->blkmap.size() * sizeof(uint32_t)));
+        QH_CHECK(hipMemcpy(b->blkmap_dev, b->blkmap.data(), b->blkmap.size() * sizeof(uint32_t),
+                           hipMemcpyHostToDevice));
     }
     b->use_graph = use_graph != 0;
     if (b->use_graph) {

@@ -187,11 +187,13 @@ int check_tile_ops(uint32_t m, uint32_t nops, const qvmhip_tile_op *ops);
 void schedule_tile_ops(int dtype, bool fma, uint32_t m, const qvmhip_tile_op *in, uint32_t nin,
                        std::vector<qvmhip_tile_op> &out);
 size_t tile_entry_bytes();
-void build_tile_entry(void *entry, int dtype, bool fma, uint32_t n, uint32_t m, const uint32_t *tq,
-                      uint64_t amp_offset, uint64_t block_start, const qvmhip_tile_op *ops_dev, uint32_t nops,
-                      uint64_t *ntiles);
+// Fills one batch tile entry; returns its number of work blocks of
+// tiles_per_block tiles.
+uint64_t build_tile_entry(void *entry, int dtype, bool fma, uint32_t n, uint32_t m, const uint32_t *tq,
+                          uint64_t amp_offset, uint64_t block_start, const qvmhip_tile_op *ops_dev, uint32_t nops,
+                          uint32_t tiles_per_block);
 int launch_tile_batch(hipStream_t st, int dtype, bool fma, uint32_t m, void *base, const void *entries_dev,
-                      uint32_t nent, uint64_t total_blocks);
+                      const uint32_t *blk_ent_dev, uint64_t total_blocks, uint32_t tiles_per_block);
 }  // namespace qvmhip
 
 #define QH_CHECK(expr)                                                                    \

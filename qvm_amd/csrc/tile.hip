@@ -474,12 +474,14 @@ __global__ __launch_bounds__(1 << (M - RB)) void k_tile(vec2_t<T> *__restrict__ 
     }
 }
 
-// Batched form: many states of one buffer, one entry per state.  Entry e
-// owns blocks [block_start[e], block_start[e] + ntiles[e]) of the launch.
+// Batched form: many states of one buffer, one entry per state.  Work block
+// b of a launch belongs to entry blk_ent[b] (a host-built map, one load
+// instead of a search) and covers tiles [(b - block_start) * CH, ... + CH) of
+// that entry's state.
 struct TileEntry {
     TileArgs args;
     uint64_t amp_offset;   // first amplitude of the state in the batch buffer
-    uint64_t block_start;  // first block of the launch that works on this entry
+    uint64_t block_start;  // first work block of the launch that works on this entry
     const qvmhip_tile_op *ops;
     uint32_t nops;
     uint32_t pad;
@@ -487,24 +489,23 @@ struct TileEntry {
 
 template <typename T, int M, int RB, bool FMA>
 __global__ __launch_bounds__(1 << (M - RB)) void k_tile_batch(vec2_t<T> *__restrict__ base,
-                                                             const TileEntry *__restrict__ ent, uint32_t nent,
-                                                             uint64_t total_blocks) {
-    __shared__ vec2_t<T> sa[(1 << M) + (1 << RB)];
+                                                             const TileEntry *__restrict__ ent,
+                                                             const uint32_t *__restrict__ blk_ent,
+                                                             uint64_t total_blocks, uint32_t tiles_per_block) {
+    using V = vec2_t<T>;
+    __shared__ V sa[(1 << M) + (1 << RB)];
     for (uint64_t b = blockIdx.x; b < total_blocks; b += gridDim.x) {
-        uint32_t lo = 0, hi = nent;  // last entry with block_start <= b
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) / 2;
-            if (ent[mid].block_start <= b)
-                lo = mid;
-            else
-                hi = mid;
-        }
-        const TileEntry &e = ent[lo];
+        const TileEntry &e = ent[blk_ent[b]];
         const uint64_t toff = thread_offset<M - RB>(e.args, threadIdx.x);
-        vec2_t<T> x[1 << RB];
-        tile_load<T, M, RB>(base + e.amp_offset, e.args, b - e.block_start, toff, x);
-        tile_ops<T, M, RB, FMA>(x, e.ops, e.nops, sa);
-        tile_store<T, M, RB>(base + e.amp_offset, e.args, b - e.block_start, toff, x);
+        const uint64_t first = (b - e.block_start) * tiles_per_block;
+        const uint64_t last = first + tiles_per_block < e.args.g.ngroups ? first + tiles_per_block : e.args.g.ngroups;
+        V *psi = base + e.amp_offset;
+        for (uint64_t tile = first; tile < last; ++tile) {
+            V x[1 << RB];
+            tile_load<T, M, RB>(psi, e.args, tile, toff, x);
+            tile_ops<T, M, RB, FMA>(x, e.ops, e.nops, sa);
+            tile_store<T, M, RB>(psi, e.args, tile, toff, x);
+        }
     }
 }
 
@@ -818,9 +819,9 @@ int launch_tile_pass(qvmhip_state *s, uint32_t m, const uint32_t *tile_qubits, u
 
 size_t tile_entry_bytes() { return sizeof(TileEntry); }
 
-void build_tile_entry(void *entry, int dtype, bool fma, uint32_t n, uint32_t m, const uint32_t *tq,
-                      uint64_t amp_offset, uint64_t block_start, const qvmhip_tile_op *ops_dev, uint32_t nops,
-                      uint64_t *ntiles) {
+uint64_t build_tile_entry(void *entry, int dtype, bool fma, uint32_t n, uint32_t m, const uint32_t *tq,
+                          uint64_t amp_offset, uint64_t block_start, const qvmhip_tile_op *ops_dev, uint32_t nops,
+                          uint32_t tiles_per_block) {
     TileEntry e;
     std::memset(&e, 0, sizeof(e));
     make_tile_args(n, m, tq, (int)m - tile_reg_bits((int)m, dtype == QVMHIP_F64, fma), e.args);
@@ -828,30 +829,32 @@ void build_tile_entry(void *entry, int dtype, bool fma, uint32_t n, uint32_t m, 
     e.block_start = block_start;
     e.ops = ops_dev;
     e.nops = nops;
-    *ntiles = e.args.g.ngroups;
     std::memcpy(entry, &e, sizeof(e));
+    return (e.args.g.ngroups + tiles_per_block - 1) / tiles_per_block;
 }
 
 namespace {
 template <typename T, int M, bool FMA>
-void tile_batch_go(hipStream_t st, void *base, const void *ent, uint32_t nent, uint64_t total) {
+void tile_batch_go(hipStream_t st, void *base, const void *ent, const uint32_t *blk_ent, uint64_t total,
+                   uint32_t tpb) {
     constexpr int RB = tile_reg_bits(M, sizeof(T) == 8, FMA);
     constexpr int NT = 1 << (M - RB);
     const uint32_t grid = (uint32_t)std::min<uint64_t>(total, kMaxGrid);
     hipLaunchKernelGGL((k_tile_batch<T, M, RB, FMA>), dim3(grid), dim3(NT), 0, st, (vec2_t<T> *)base,
-                       (const TileEntry *)ent, nent, total);
+                       (const TileEntry *)ent, blk_ent, total, tpb);
 }
 template <typename T, bool FMA>
-int tile_batch_m(hipStream_t st, uint32_t m, void *base, const void *ent, uint32_t nent, uint64_t total) {
+int tile_batch_m(hipStream_t st, uint32_t m, void *base, const void *ent, const uint32_t *blk_ent, uint64_t total,
+                 uint32_t tpb) {
     switch (m) {
-        case 8: tile_batch_go<T, 8, FMA>(st, base, ent, nent, total); break;
-        case 9: tile_batch_go<T, 9, FMA>(st, base, ent, nent, total); break;
-        case 10: tile_batch_go<T, 10, FMA>(st, base, ent, nent, total); break;
-        case 11: tile_batch_go<T, 11, FMA>(st, base, ent, nent, total); break;
-        case 12: tile_batch_go<T, 12, FMA>(st, base, ent, nent, total); break;
+        case 8: tile_batch_go<T, 8, FMA>(st, base, ent, blk_ent, total, tpb); break;
+        case 9: tile_batch_go<T, 9, FMA>(st, base, ent, blk_ent, total, tpb); break;
+        case 10: tile_batch_go<T, 10, FMA>(st, base, ent, blk_ent, total, tpb); break;
+        case 11: tile_batch_go<T, 11, FMA>(st, base, ent, blk_ent, total, tpb); break;
+        case 12: tile_batch_go<T, 12, FMA>(st, base, ent, blk_ent, total, tpb); break;
         case 13:
             if constexpr (sizeof(T) == 4) {
-                tile_batch_go<T, 13, FMA>(st, base, ent, nent, total);
+                tile_batch_go<T, 13, FMA>(st, base, ent, blk_ent, total, tpb);
                 break;
             }
             return fail(QVMHIP_EINVAL, "tile size 13 is FP32 only");
@@ -864,12 +867,14 @@ int tile_batch_m(hipStream_t st, uint32_t m, void *base, const void *ent, uint32
 }  // namespace
 
 int launch_tile_batch(hipStream_t st, int dtype, bool fma, uint32_t m, void *base, const void *entries_dev,
-                      uint32_t nent, uint64_t total_blocks) {
+                      const uint32_t *blk_ent_dev, uint64_t total_blocks, uint32_t tiles_per_block) {
     if (fma)
-        return dtype == QVMHIP_F64 ? tile_batch_m<double, true>(st, m, base, entries_dev, nent, total_blocks)
-                                   : tile_batch_m<float, true>(st, m, base, entries_dev, nent, total_blocks);
-    return dtype == QVMHIP_F64 ? tile_batch_m<double, false>(st, m, base, entries_dev, nent, total_blocks)
-                               : tile_batch_m<float, false>(st, m, base, entries_dev, nent, total_blocks);
+        return dtype == QVMHIP_F64
+                   ? tile_batch_m<double, true>(st, m, base, entries_dev, blk_ent_dev, total_blocks, tiles_per_block)
+                   : tile_batch_m<float, true>(st, m, base, entries_dev, blk_ent_dev, total_blocks, tiles_per_block);
+    return dtype == QVMHIP_F64
+               ? tile_batch_m<double, false>(st, m, base, entries_dev, blk_ent_dev, total_blocks, tiles_per_block)
+               : tile_batch_m<float, false>(st, m, base, entries_dev, blk_ent_dev, total_blocks, tiles_per_block);
 }
 
 }  // namespace qvmhip

@@ -27,6 +27,7 @@ def main(argv=None) -> int:
                              "tile"])
     ap.add_argument("--reps", type=int, default=8)
     ap.add_argument("--fma", action="store_true", help="tile passes with FMA-contracted products")
+    ap.add_argument("--tile-m", type=int, default=0, help="tile width (default: the planner's)")
     a = ap.parse_args(argv)
     st = HipPureState(a.n, a.dtype, fma_tiles=a.fma)
     rng = np.random.default_rng(0)
@@ -47,8 +48,8 @@ def main(argv=None) -> int:
         elif a.kernel == "tile":
             from . import circuits as C
             from . import planner as P
-            plan = P.plan_tiled(C.layered(a.n, 1, seed=r), a.n, P.default_tile_m(a.dtype), P.default_low(a.dtype),
-                                merge_1q=a.fma)
+            plan = P.plan_tiled(C.layered(a.n, 1, seed=r), a.n, a.tile_m or P.default_tile_m(a.dtype),
+                                P.default_low(a.dtype), merge_1q=a.fma)
             P.execute_plan(st, plan)
     st.sync()
     other = HipPureState(a.n, a.dtype)

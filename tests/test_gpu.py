@@ -336,6 +336,90 @@ def test_tile_passes_vs_plain(S, oracle, m, reorder):
     st.close()
 
 
+def _random_tile_ops(m, count, rng):
+    """Random tile ops of every public kind, some controlled (local bits)."""
+    ops = []
+    for _ in range(count):
+        kind = int(rng.choice([1, 1, 2, 3, 4, 5, 5, 6, 6, 7]))
+        q = [int(x) for x in rng.choice(m, 3, replace=False)]
+        ctrl = (1 << q[2]) if rng.random() < 0.25 else 0
+        op = N.TileOp()
+        op.kind, op.q0, op.ctrl_mask = kind, q[0], ctrl
+        op.q1 = q[1] if kind not in (N.TILE_DENSE1, N.TILE_DIAG1) else 0
+        if kind == N.TILE_DENSE1:
+            vals = rand_unitary(1, rng).reshape(-1)
+        elif kind == N.TILE_DENSE2:
+            vals = rand_unitary(2, rng).reshape(-1)
+        elif kind == N.TILE_DIAG1:
+            vals = np.exp(1j * rng.uniform(0, 6, 2))
+        elif kind == N.TILE_DIAG2:
+            vals = np.exp(1j * rng.uniform(0, 6, 4))
+        elif kind == N.TILE_CPHASE:
+            vals = np.exp(1j * rng.uniform(0, 6, 1))
+        else:
+            vals = np.zeros(1)
+            if kind == N.TILE_SWAP and rng.random() < 0.5:
+                op.ctrl_mask = 0
+        v = np.zeros(16, dtype=np.complex128)
+        v[:vals.size] = vals
+        for k, x in enumerate(v.view(np.float64)):
+            op.m[k] = x
+        ops.append(op)
+    return ops
+
+
+def _tile_op_as_gate(op, tq):
+    """(U, Quil-order qubits) of a tile op on tile qubits tq."""
+    mm = np.array([op.m[i] for i in range(32)]).view(np.complex128)
+    g = lambda b: tq[b]  # noqa: E731
+    if op.kind == N.TILE_DENSE1:
+        U, qs = mm[:4].reshape(2, 2), [g(op.q0)]
+    elif op.kind == N.TILE_DIAG1:
+        U, qs = np.diag(mm[:2]), [g(op.q0)]
+    elif op.kind == N.TILE_DENSE2:
+        U, qs = mm[:16].reshape(4, 4), [g(op.q1), g(op.q0)]
+    elif op.kind == N.TILE_DIAG2:
+        U, qs = np.diag(mm[:4]), [g(op.q1), g(op.q0)]
+    elif op.kind == N.TILE_CPERM:
+        U, qs = G.gate_matrix("CNOT"), [g(op.q1), g(op.q0)]
+    elif op.kind == N.TILE_SWAP:
+        U, qs = G.gate_matrix("SWAP"), [g(op.q1), g(op.q0)]
+    else:
+        U, qs = np.diag([1, 1, 1, mm[0]]), [g(op.q1), g(op.q0)]
+    for c in [b for b in range(32) if (op.ctrl_mask >> b) & 1]:
+        U, qs = G.controlled(U), [g(c)] + qs
+    return U, qs
+
+
+@pytest.mark.parametrize("dtype,m,fma", [("f64", 8, False), ("f64", 10, False), ("f64", 12, False),
+                                         ("f64", 11, True), ("f32", 9, False), ("f32", 13, True)])
+def test_tile_pass_every_op_kind(S, oracle, dtype, m, fma):
+    """Random passes of every op kind (controlled dense 1q/2q, diagonals on
+    thread and register bits, CNOT, controlled and plain SWAP, CPHASE) through
+    qvmhip_apply_tile_pass: exact FP64 passes equal the oracle bit for bit."""
+    from qvm_amd.state import HipPureState
+    rng = np.random.default_rng(m * 7 + (1 if fma else 0))
+    n = 15
+    tq = sorted([0, 1, 2] + [int(x) for x in rng.choice(np.arange(3, n), m - 3, replace=False)])
+    psi = rand_state(n, rng)
+    st = HipPureState(n, dtype, fma_tiles=fma)
+    st.write_amplitudes(psi)
+    ref = psi.copy()
+    for rep in range(3):
+        ops = _random_tile_ops(m, 50, rng)
+        say(f"tile ops {dtype} m={m} fma={fma} rep={rep}")
+        st.apply_tile_pass(tq, ops)
+        for op in ops:
+            U, qs = _tile_op_as_gate(op, tq)
+            oracle.apply_matrix(ref, U, G.nat_tuple(*qs))
+    got = st.amplitudes()
+    if dtype == "f64" and not fma:
+        np.testing.assert_array_equal(got, ref)
+    else:
+        assert np.max(np.abs(got - ref)) < (1e-13 if dtype == "f64" else 2e-6)
+    st.close()
+
+
 def test_tile_pass_f32(S, oracle):
     n = 14
     gates = CI.layered(n, 4, seed=6)
