@@ -205,6 +205,14 @@ QVMHIP_API int qvmhip_write_amplitudes(qvmhip_state *s, uint64_t start, uint64_t
                                        const double *in);
 /* copy-wavefunction (src/wavefunction.lisp:93-109): dst <- src, same n/dtype. */
 QVMHIP_API int qvmhip_copy(qvmhip_state *dst, const qvmhip_state *src);
+/* dst += src elementwise, same n/dtype: the running sum of the Kraus-list
+ * evolution of a density matrix (map-into sum #'+, src/apply-gate.lisp:76-96). */
+QVMHIP_API int qvmhip_accumulate(qvmhip_state *dst, const qvmhip_state *src);
+/* For a 2*nq-qubit state holding vec(rho) row-major (rho[i][j] at i*2^nq + j):
+ * out[i] = Re rho[i][i], i < 2^nq.  density-matrix-state-measurement-
+ * probabilities (src/state-representation.lisp:268-286) and the diagonal of
+ * get-excited-state-probability on density states (src/measurement.lisp:77-85). */
+QVMHIP_API int qvmhip_density_diagonal(qvmhip_state *s, uint32_t nq, double *out);
 /* Wire formats of the server's octet-stream replies: amplitudes
  * [start, start+count) as big-endian IEEE doubles re, im (16 octets each,
  * write-complex-double-float-as-binary, app/src/utilities.lisp:60-75, sent by

@@ -99,6 +99,8 @@ SIGNATURES = {
     "qvmhip_sample_measure_all": [_vp, _dp, _u64, _u64p],
     "qvmhip_sample": [_vp, _dp, _u64, _u64p],
     "qvmhip_read_amplitudes": [_vp, _u64, _u64, _dp],
+    "qvmhip_accumulate": [_vp, _vp],
+    "qvmhip_density_diagonal": [_vp, _u32, _dp],
     "qvmhip_read_wavefunction_be": [_vp, _u64, _u64, C.c_void_p],
     "qvmhip_read_probabilities_be": [_vp, _u64, _u64, C.c_void_p],
     "qvmhip_write_amplitudes": [_vp, _u64, _u64, _dp],

@@ -522,6 +522,19 @@ int qvmhip_write_amplitudes(qvmhip_state *s, uint64_t start, uint64_t count, con
     return QVMHIP_OK;
 }
 
+int qvmhip_accumulate(qvmhip_state *dst, const qvmhip_state *src) {
+    if (!dst || !src) return fail(QVMHIP_EINVAL, "null state");
+    if (dst->n != src->n || dst->dtype != src->dtype) return fail(QVMHIP_EINVAL, "states differ in size or dtype");
+    if (dst != src) QH_CHECK(hipStreamSynchronize(src->stream));
+    return accumulate(dst, src);
+}
+
+int qvmhip_density_diagonal(qvmhip_state *s, uint32_t nq, double *out) {
+    if (!s || !out) return fail(QVMHIP_EINVAL, "null argument");
+    if (2 * nq != s->n) return fail(QVMHIP_EINVAL, "a density state of nq qubits has 2*nq qubits");
+    return density_diagonal(s, nq, out);
+}
+
 int qvmhip_copy(qvmhip_state *dst, const qvmhip_state *src) {
     if (!dst || !src) return fail(QVMHIP_EINVAL, "null state");
     if (dst->n != src->n || dst->dtype != src->dtype) return fail(QVMHIP_EINVAL, "states differ in size or dtype");

@@ -173,6 +173,8 @@ int inner_product(qvmhip_state *a, qvmhip_state *b, double *re, double *im);
 int max_abs_diff(qvmhip_state *a, qvmhip_state *b, double *out);
 int tv_distance(qvmhip_state *a, qvmhip_state *b, double *out);
 int probabilities(qvmhip_state *s, uint64_t start, uint64_t count, double *host_out);
+int accumulate(qvmhip_state *dst, const qvmhip_state *src);
+int density_diagonal(qvmhip_state *s, uint32_t nq, double *host_out);
 // export modes of export_range: 16-byte (re, im) doubles or 8-byte probabilities
 enum { kExportAmps = 0, kExportAmpsBE = 1, kExportProbs = 2, kExportProbsBE = 3 };
 int export_range(qvmhip_state *s, uint64_t start, uint64_t count, int mode, void *host_out);

@@ -425,6 +425,30 @@ __global__ __launch_bounds__(kBlock) void k_narrow(const double2 *__restrict__ i
     }
 }
 
+// dst += src, elementwise (the running sum of a Kraus-list density evolution).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_accumulate(vec2_t<T> *__restrict__ dst, const vec2_t<T> *__restrict__ src,
+                                                       uint64_t len) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < len; i += stride) {
+        vec2_t<T> a = dst[i];
+        const vec2_t<T> b = src[i];
+        a.x = a.x + b.x;
+        a.y = a.y + b.y;
+        dst[i] = a;
+    }
+}
+
+// out[i] = Re psi[i * (dim + 1)], i < dim: the diagonal of a density matrix
+// stored row-major as a vector of dim^2 amplitudes.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_diag_gather(const vec2_t<T> *__restrict__ psi, uint64_t dim,
+                                                        double *__restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < dim; i += stride)
+        out[i] = (double)psi[i * (dim + 1)].x;
+}
+
 // Host export of a range as 64-bit words: amplitudes as (re, im) doubles or
 // probabilities re*re + im*im, in native or big-endian byte order (the
 // server's octet-stream replies, app/src/utilities.lisp:40-87).
@@ -786,6 +810,37 @@ int export_range(qvmhip_state *s, uint64_t start, uint64_t count, int mode, void
 
 int probabilities(qvmhip_state *s, uint64_t start, uint64_t count, double *host_out) {
     return export_range(s, start, count, kExportProbs, host_out);
+}
+
+int accumulate(qvmhip_state *dst, const qvmhip_state *src) {
+    const uint32_t grid = sweep_grid(dst->len);
+    if (dst->dtype == QVMHIP_F64)
+        hipLaunchKernelGGL(k_accumulate<double>, dim3(grid), dim3(kBlock), 0, dst->stream, (double2 *)dst->amps,
+                           (const double2 *)src->amps, dst->len);
+    else
+        hipLaunchKernelGGL(k_accumulate<float>, dim3(grid), dim3(kBlock), 0, dst->stream, (float2 *)dst->amps,
+                           (const float2 *)src->amps, dst->len);
+    QH_LAUNCH_CHECK();
+    return QVMHIP_OK;
+}
+
+int density_diagonal(qvmhip_state *s, uint32_t nq, double *host_out) {
+    const uint64_t dim = uint64_t(1) << nq;
+    double *dev = nullptr;
+    QH_CHECK(hipMalloc((void **)&dev, dim * sizeof(double)));
+    const uint32_t grid = sweep_grid(dim);
+    if (s->dtype == QVMHIP_F64)
+        hipLaunchKernelGGL(k_diag_gather<double>, dim3(grid), dim3(kBlock), 0, s->stream, (const double2 *)s->amps,
+                           dim, dev);
+    else
+        hipLaunchKernelGGL(k_diag_gather<float>, dim3(grid), dim3(kBlock), 0, s->stream, (const float2 *)s->amps,
+                           dim, dev);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(host_out, dev, dim * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    hipFree(dev);
+    if (e != hipSuccess) return fail(QVMHIP_EDEVICE, std::string("density diagonal: ") + hipGetErrorString(e));
+    return QVMHIP_OK;
 }
 
 int convert_f32_to_f64(qvmhip_state *s, uint64_t start, uint64_t count, double *dev_out) {

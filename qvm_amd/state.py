@@ -246,6 +246,17 @@ class HipPureState:
         """copy-wavefunction (src/wavefunction.lisp:93-109)."""
         N.check(N.lib().qvmhip_copy(self._h, other._h))
 
+    def accumulate(self, other: "HipPureState") -> None:
+        """self += other, elementwise."""
+        N.check(N.lib().qvmhip_accumulate(self._h, other._h))
+
+    def density_diagonal(self, nq: int) -> np.ndarray:
+        """Re rho[i][i] of the density matrix this 2*nq-qubit state holds
+        as vec(rho), row-major."""
+        out = np.empty(1 << nq, dtype=np.float64)
+        N.check(N.lib().qvmhip_density_diagonal(self._h, int(nq), N.dptr(out)))
+        return out
+
     def swap_contents(self, other: "HipPureState") -> None:
         """Exchange the device buffers of two states of one size and dtype
         (the rotatef of the state and trial amplitude vectors,
