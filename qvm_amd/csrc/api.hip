@@ -258,7 +258,7 @@ int qvmhip_apply_tile_pass(qvmhip_state *s, uint32_t m, const uint32_t *tile_qub
         if (tile_qubits[i] >= s->n) return fail(QVMHIP_EINVAL, "tile qubit out of range");
         if (i && tile_qubits[i] <= tile_qubits[i - 1]) return fail(QVMHIP_EINVAL, "tile qubits must ascend");
     }
-    int rc = check_tile_ops(m, nops, ops);
+    int rc = check_tile_ops(s->dtype, m, nops, ops);
     if (rc) return rc;
     if (nops == 0) return QVMHIP_OK;
     return launch_tile_pass(s, m, tile_qubits, nops, ops, 0);
@@ -269,9 +269,7 @@ int qvmhip_tile_schedule(int dtype, uint32_t flags, uint32_t m, uint32_t nops, c
     if (!nout || (nops && !ops) || (cap && !out)) return fail(QVMHIP_EINVAL, "null argument");
     if (dtype != QVMHIP_F64 && dtype != QVMHIP_F32) return fail(QVMHIP_EINVAL, "unknown dtype");
     if (flags & ~QVMHIP_FLAG_FMA_TILES) return fail(QVMHIP_EINVAL, "unknown flags");
-    if (m < 8 || m > 13 || (m == 13 && dtype == QVMHIP_F64))
-        return fail(QVMHIP_EINVAL, "tile size m must be 8..12 (F64) or 8..13 (F32)");
-    int rc = check_tile_ops(m, nops, ops);
+    int rc = check_tile_ops(dtype, m, nops, ops);
     if (rc) return rc;
     std::vector<qvmhip_tile_op> sched;
     schedule_tile_ops(dtype, (flags & QVMHIP_FLAG_FMA_TILES) != 0, m, ops, nops, sched);

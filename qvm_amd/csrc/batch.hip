@@ -262,7 +262,7 @@ int qvmhip_batch_add_tile_pass(qvmhip_batch *b, uint32_t inst, uint32_t m, const
         if (tile_qubits[i] >= n) return fail(QVMHIP_EINVAL, "tile qubit out of range");
         if (i && tile_qubits[i] <= tile_qubits[i - 1]) return fail(QVMHIP_EINVAL, "tile qubits must ascend");
     }
-    rc = check_tile_ops(m, nops, ops);
+    rc = check_tile_ops(b->dtype, m, nops, ops);
     if (rc) return rc;
     if (!nops) return QVMHIP_OK;
     BItem it;

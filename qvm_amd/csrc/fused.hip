@@ -208,8 +208,21 @@ int run_fused(qvmhip_state *s, const void *mat_dev, const uint32_t *qubits) {
             if ((a >> b) & 1) o |= uint64_t(1) << pos[b];
         off.o[a] = o;
     }
+    // Every wave grid-strides over equal shares of column groups, so the grid
+    // is exactly the workgroups that fit on the device at once: a second,
+    // partial round would run at a fraction of the occupancy.
+    static int per_cu = 0;
+    if (per_cu == 0) {
+        int nb = 0;
+        per_cu = (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fused<T, K>, 256, 0) == hipSuccess && nb > 0)
+                     ? nb
+                     : 2;
+    }
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess || ncu <= 0)
+        ncu = 256;
     const uint64_t ncg = (g.ngroups + 15) / 16;
-    const uint32_t grid = (uint32_t)std::min<uint64_t>((ncg + 3) / 4, 256 * 8);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((ncg + 3) / 4, (uint64_t)per_cu * (uint64_t)ncu);
     prof_begin(s, QVMHIP_K_FUSED);
     hipLaunchKernelGGL((k_fused<T, K>), dim3(grid), dim3(256), 0, s->stream, (vec2_t<T> *)s->amps,
                        (const double *)mat_dev, g, off);

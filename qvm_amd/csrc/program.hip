@@ -134,7 +134,7 @@ int qvmhip_program_add_tile_pass(qvmhip_program *p, uint32_t m, const uint32_t *
         if (tile_qubits[i] >= p->s->n) return fail(QVMHIP_EINVAL, "tile qubit out of range");
         if (i && tile_qubits[i] <= tile_qubits[i - 1]) return fail(QVMHIP_EINVAL, "tile qubits must ascend");
     }
-    int rc = check_tile_ops(m, nops, ops);
+    int rc = check_tile_ops(p->s->dtype, m, nops, ops);
     if (rc) return rc;
     if (nops == 0) return QVMHIP_OK;
     Item it;

@@ -185,7 +185,7 @@ int expectation_zz_to_device(qvmhip_state *s, const uint32_t *edges_dev, uint32_
 int sample_to_device(qvmhip_state *s, const double *u_dev, uint64_t nshots, uint64_t *out_dev, int rule_ge);
 // Tile passes: validation of a public op list (tile-local bit positions) and
 // its translation into the slot-space program the tile kernels run.
-int check_tile_ops(uint32_t m, uint32_t nops, const qvmhip_tile_op *ops);
+int check_tile_ops(int dtype, uint32_t m, uint32_t nops, const qvmhip_tile_op *ops);
 void schedule_tile_ops(int dtype, bool fma, uint32_t m, const qvmhip_tile_op *in, uint32_t nin,
                        std::vector<qvmhip_tile_op> &out);
 size_t tile_entry_bytes();

@@ -755,7 +755,8 @@ int export_range(qvmhip_state *s, uint64_t start, uint64_t count, int mode, void
     };
     for (int h = 0; h < 2; ++h) {
         hipError_t e = hipEventCreateWithFlags(&ev[h], hipEventDisableTiming);
-        if (e == hipSuccess && !direct && count > (uint64_t)h * per) e = hipMalloc((void **)&dev[h], per * eb);
+        if (e == hipSuccess && !direct && count > (uint64_t)h * per)
+            e = hipMalloc((void **)&dev[h], std::min<uint64_t>(per, count) * eb);
         if (e != hipSuccess) {
             cleanup();
             return fail(e == hipErrorOutOfMemory ? QVMHIP_ENOMEM : QVMHIP_EDEVICE,

@@ -512,7 +512,9 @@ __global__ __launch_bounds__(1 << (M - RB)) void k_tile_batch(vec2_t<T> *__restr
 // ---------------------------------------------------------------------------
 // Host side.
 
-int check_tile_ops(uint32_t m, uint32_t nops, const qvmhip_tile_op *ops) {
+int check_tile_ops(int dtype, uint32_t m, uint32_t nops, const qvmhip_tile_op *ops) {
+    if (m < 8 || m > 13 || (m == 13 && dtype == QVMHIP_F64))
+        return fail(QVMHIP_EINVAL, "tile size m must be 8..12 (F64) or 8..13 (F32)");
     for (uint32_t o = 0; o < nops; ++o) {
         const qvmhip_tile_op &op = ops[o];
         if (op.kind < QVMHIP_TILE_DENSE1 || op.kind > QVMHIP_TILE_CPHASE)

@@ -231,6 +231,15 @@ def test_error_paths_without_a_device():
     assert L.qvmhip_apply_matrix(None, None, 1, None) == N.EINVAL
     assert L.qvmhip_measure(None, 0, 0.5, 0, None, None) == N.EINVAL
     assert L.qvmhip_read_wavefunction_be(None, 0, 1, None) == N.EINVAL
+    op = N.TileOp()
+    op.kind, op.q0 = N.TILE_DENSE1, 0
+    for dtype, m in (("f64", 5), ("f64", 13), ("f32", 14)):
+        with pytest.raises(N.QVMHipError):
+            N.tile_schedule(dtype, m, [op])
+    op.q0 = 12
+    with pytest.raises(N.QVMHipError):
+        N.tile_schedule("f64", 12, [op])  # local bit outside the tile
+    assert L.qvmhip_batch_set_flags(None, 0) == N.EINVAL
     assert L.qvmhip_total_probability(None, None) == N.EINVAL
 
 
