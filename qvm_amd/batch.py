@@ -21,14 +21,14 @@ from . import gates as G
 from . import planner as P
 
 
-def tile_width(n: int, dtype: str) -> int:
-    return min(n, P.default_tile_m(dtype))
+def tile_width(n: int, dtype: str, cap: int = 0) -> int:
+    return min(n, cap or P.default_tile_m(dtype))
 
 
 class Batch:
     def __init__(self, circuits: Sequence[Sequence], nqubits: Sequence[int], dtype: str = "f64",
                  edges: Sequence | None = None, shots: int = 0, uniforms=None, use_graph: bool = True,
-                 fma_tiles: bool = False):
+                 fma_tiles: bool = False, tile_m: int = 0):
         """circuits[i]: list of (U, quil_qubits) on nqubits[i] qubits
         (every n >= 8); edges[i]: MaxCut edges for <H_C>; uniforms:
         count*shots draws in [0,1], instance-major."""
@@ -47,7 +47,7 @@ class Batch:
             n = self.nq[i]
             if n < 8:
                 raise ValueError("batch instances need at least 8 qubits (tile passes)")
-            m = tile_width(n, dtype)
+            m = tile_width(n, dtype, tile_m)
             plan = P.plan_tiled(list(gates), n, m, P.default_low(dtype) if m > 8 else 3)
             for it in plan:
                 if it.kind == "tile":
@@ -101,12 +101,12 @@ class Batch:
 
 
 def qaoa_batch(instances, dtype: str = "f64", shots: int = 1024, seed: int = 9, use_graph: bool = True,
-               fma_tiles: bool = False):
+               fma_tiles: bool = False, tile_m: int = 0):
     """Batch for (n, edges, gammas, betas) instances (circuits.qaoa_instances)."""
     circuits = [CI.qaoa_maxcut(n, e, g, b) for n, e, g, b in instances]
     u = CI.uniforms(seed, len(instances) * shots) if shots else None
     return Batch(circuits, [n for n, _, _, _ in instances], dtype, edges=[e for _, e, _, _ in instances],
-                 shots=shots, uniforms=u, use_graph=use_graph, fma_tiles=fma_tiles), circuits, u
+                 shots=shots, uniforms=u, use_graph=use_graph, fma_tiles=fma_tiles, tile_m=tile_m), circuits, u
 
 
 def expected_cut(nedges: int, hc: float) -> float:
