@@ -227,7 +227,9 @@ QVMHIP_API int qvmhip_read_probabilities_be(qvmhip_state *s, uint64_t start, uin
 /* --- compiled programs (compile-loaded-program / compile-instruction) ---- */
 /* A program records gates, tile passes and fused blocks against one state,
  * uploads their operands once, and replays them -- optionally as one captured
- * hipGraph (src/qvm.lisp:166-175, src/compile-gate.lisp:461-526). */
+ * hipGraph (src/qvm.lisp:166-175, src/compile-gate.lisp:461-526).
+ * Destroying the state first detaches its programs: they then refuse to
+ * run (QVMHIP_EINVAL) and may still be destroyed. */
 QVMHIP_API int qvmhip_program_create(qvmhip_state *s, qvmhip_program **out);
 QVMHIP_API int qvmhip_program_add_matrix(qvmhip_program *p, const double *U, uint32_t k,
                                          const uint32_t *qubits);
@@ -268,8 +270,8 @@ QVMHIP_API int qvmhip_profile_enable(qvmhip_state *s, int on);
 /* Waits for the handle, then reports the number of recorded launches of
  * `kind` and their summed duration in milliseconds. */
 QVMHIP_API int qvmhip_profile_summary(qvmhip_state *s, int kind, uint64_t *count, double *total_ms);
-/* Time `reps` launches of the last program (or a single gate when p is
- * NULL) with HIP events on the handle's stream; *ms = mean per launch. */
+/* Time `reps` launches of a finalized program with HIP events on its
+ * state's stream; *ms = mean per launch. */
 QVMHIP_API int qvmhip_time_program(qvmhip_program *p, int reps, double *ms);
 
 /* --- batches of small states (no single counterpart in the reference) ---

@@ -180,6 +180,7 @@ int qvmhip_destroy(qvmhip_state *s) {
     if (!s) return QVMHIP_OK;
     hipSetDevice(s->device);
     hipStreamSynchronize(s->stream);
+    for (qvmhip_program *p : s->programs) program_detach(p);
     free_state(s);
     return QVMHIP_OK;
 }

@@ -78,17 +78,23 @@ int dense_width_of(const double *U, uint32_t k) {
 
 }  // namespace
 
+constexpr const char *kStateGone = "the program's state was destroyed";
+
+void qvmhip::program_detach(qvmhip_program *p) { p->s = nullptr; }
+
 extern "C" {
 
 int qvmhip_program_create(qvmhip_state *s, qvmhip_program **out) {
     if (!s || !out) return fail(QVMHIP_EINVAL, "null argument");
     *out = new qvmhip_program();
     (*out)->s = s;
+    s->programs.push_back(*out);
     return QVMHIP_OK;
 }
 
 int qvmhip_program_add_matrix(qvmhip_program *p, const double *U, uint32_t k, const uint32_t *qubits) {
     if (!p || !U || !qubits) return fail(QVMHIP_EINVAL, "null argument");
+    if (!p->s) return fail(QVMHIP_EINVAL, kStateGone);
     if (p->finalized) return fail(QVMHIP_EINVAL, "program already finalized");
     if (k < 1 || k > 6 || k > p->s->n) return fail(QVMHIP_EINVAL, "gate arity must be 1..6");
     for (uint32_t i = 0; i < k; ++i) {
@@ -108,6 +114,7 @@ int qvmhip_program_add_matrix(qvmhip_program *p, const double *U, uint32_t k, co
 
 int qvmhip_program_add_fused(qvmhip_program *p, const double *U, uint32_t k, const uint32_t *qubits) {
     if (!p || !U || !qubits) return fail(QVMHIP_EINVAL, "null argument");
+    if (!p->s) return fail(QVMHIP_EINVAL, kStateGone);
     if (p->finalized) return fail(QVMHIP_EINVAL, "program already finalized");
     if (k < 3 || k > 6 || k > p->s->n) return fail(QVMHIP_EINVAL, "fused blocks are 3..6 qubits");
     for (uint32_t i = 0; i < k; ++i) {
@@ -128,6 +135,7 @@ int qvmhip_program_add_fused(qvmhip_program *p, const double *U, uint32_t k, con
 int qvmhip_program_add_tile_pass(qvmhip_program *p, uint32_t m, const uint32_t *tile_qubits, uint32_t nops,
                                  const qvmhip_tile_op *ops) {
     if (!p || !tile_qubits || (nops && !ops)) return fail(QVMHIP_EINVAL, "null argument");
+    if (!p->s) return fail(QVMHIP_EINVAL, kStateGone);
     if (p->finalized) return fail(QVMHIP_EINVAL, "program already finalized");
     if (m > p->s->n || m > 16) return fail(QVMHIP_EINVAL, "tile wider than the state");
     for (uint32_t i = 0; i < m; ++i) {
@@ -153,6 +161,7 @@ int qvmhip_program_add_tile_pass(qvmhip_program *p, uint32_t m, const uint32_t *
 int qvmhip_program_finalize(qvmhip_program *p, int use_graph) {
     if (!p) return fail(QVMHIP_EINVAL, "null program");
     if (p->finalized) return fail(QVMHIP_EINVAL, "program already finalized");
+    if (!p->s) return fail(QVMHIP_EINVAL, kStateGone);
     qvmhip_state *s = p->s;
     if (!p->blob.empty()) {
         QH_CHECK(hipMalloc(&p->dev, p->blob.size()));
@@ -184,6 +193,7 @@ int qvmhip_program_finalize(qvmhip_program *p, int use_graph) {
 int qvmhip_program_run(qvmhip_program *p) {
     if (!p) return fail(QVMHIP_EINVAL, "null program");
     if (!p->finalized) return fail(QVMHIP_EINVAL, "program not finalized");
+    if (!p->s) return fail(QVMHIP_EINVAL, kStateGone);
     if (p->use_graph) {
         QH_CHECK(hipGraphLaunch(p->exec, p->s->stream));
         return QVMHIP_OK;
@@ -193,7 +203,11 @@ int qvmhip_program_run(qvmhip_program *p) {
 
 int qvmhip_program_destroy(qvmhip_program *p) {
     if (!p) return QVMHIP_OK;
-    if (p->s) hipStreamSynchronize(p->s->stream);
+    if (p->s) {
+        hipStreamSynchronize(p->s->stream);
+        auto &v = p->s->programs;
+        v.erase(std::remove(v.begin(), v.end(), p), v.end());
+    }
     if (p->exec) hipGraphExecDestroy(p->exec);
     if (p->graph) hipGraphDestroy(p->graph);
     if (p->dev) hipFree(p->dev);
@@ -203,6 +217,7 @@ int qvmhip_program_destroy(qvmhip_program *p) {
 
 int qvmhip_time_program(qvmhip_program *p, int reps, double *ms) {
     if (!p || !ms || reps < 1) return fail(QVMHIP_EINVAL, "bad argument");
+    if (!p->s) return fail(QVMHIP_EINVAL, kStateGone);
     qvmhip_state *s = p->s;
     hipEvent_t a, b;
     QH_CHECK(hipEventCreate(&a));

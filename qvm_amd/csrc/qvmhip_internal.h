@@ -126,6 +126,10 @@ struct qvmhip_state {
     bool profiling = false;
     std::vector<ProfRec> prof;
     std::vector<hipEvent_t> event_pool;
+    // programs recorded against this state: destroying the state detaches
+    // them (their state pointer becomes null), so a program destroyed later
+    // frees only its own graph and operands
+    std::vector<qvmhip_program *> programs;
 };
 
 namespace qvmhip {
@@ -157,6 +161,7 @@ int launch_gate(qvmhip_state *s, const double *U, uint32_t k, const uint32_t *qu
 int launch_diagonal(qvmhip_state *s, const double *d, uint32_t k, const uint32_t *qubits);
 int launch_permutation(qvmhip_state *s, const uint32_t *perm, uint32_t k, const uint32_t *qubits);
 int launch_fill_zero_state(qvmhip_state *s);
+void program_detach(qvmhip_program *p);  // the state is being destroyed
 int launch_scale(qvmhip_state *s, double factor);
 int reduce_probability(qvmhip_state *s, int mode, uint32_t q, double *out_dev);
 int reduce_norm2(qvmhip_state *s, double *out_dev);

@@ -35,7 +35,7 @@ import numpy as np
 from . import gates as G
 from . import quil as Q
 from .noise import check_kraus_ops, check_povm, perturb_measurement
-from .qvm import QVMError, _UniformStream
+from .qvm import QVMError, _UniformStream, transition_classical
 from .state import HipPureState
 
 
@@ -133,8 +133,11 @@ class DensityQVM:
             return self.pc + 1
         if isinstance(ins, Q.Halt):
             return None
-        if isinstance(ins, Q.Nop):
+        if isinstance(ins, (Q.Nop, Q.Label)):
             return self.pc + 1
+        r = transition_classical(self, ins)
+        if r is not None:
+            return r
         raise QVMError(f"unsupported instruction {ins!r}")
 
     # -- evolution --------------------------------------------------------

@@ -4,7 +4,11 @@ The reference parses Quil with cl-quil (not vendored; call sites
 src/classical-memory-mixin.lisp:39,144).  This module reads the subset that
 the pure-state path executes: DECLARE, gate applications with parameter
 expressions and the DAGGER / CONTROLLED / FORKED modifiers, DEFGATE (matrix
-and permutation forms), MEASURE, RESET, HALT, NOP and PRAGMA (ignored).
+and permutation forms), MEASURE, RESET, HALT, NOP, PRAGMA (ignored), LABEL and
+the jumps JUMP / JUMP-WHEN / JUMP-UNLESS (targets resolved to instruction
+indices when the program is read, as label patching does), and the classical
+instructions MOVE, EXCHANGE, CONVERT, NEG, NOT, ADD, SUB, MUL, DIV, AND, IOR,
+XOR, EQ, GT, GE, LT, LE (src/transition-classical-instructions.lisp).
 Parameter expressions may name classical memory (``theta[0]``) and are
 evaluated when the instruction executes, as ``force-parameter`` does
 (src/transition.lisp:142-153).
@@ -198,6 +202,30 @@ class Nop:
 
 
 @dataclass
+class Label:
+    name: str
+
+
+@dataclass
+class Jump:
+    """JUMP / JUMP-WHEN / JUMP-UNLESS: target is the label's instruction index
+    after parsing; cond is (name, index) of the bit for the conditional forms."""
+    label: str
+    when: Optional[bool] = None
+    cond: Optional[Tuple[str, int]] = None
+    target: int = -1
+
+
+@dataclass
+class Classical:
+    """A classical memory instruction: op, destination (name, index), and
+    operands that are (name, index) memory references or numbers."""
+    op: str
+    dest: Tuple[str, int]
+    args: List = field(default_factory=list)
+
+
+@dataclass
 class Declare:
     name: str
     type: str
@@ -332,9 +360,157 @@ def parse_quil(text: str) -> ParsedProgram:
         elif word in ("NOP", "PRAGMA", "WAIT"):
             if word == "NOP":
                 code.append(Nop())
+        elif word == "LABEL":
+            code.append(Label(_label_name(s.split(None, 1)[1] if len(s.split()) > 1 else "")))
+        elif word in ("JUMP", "JUMP-WHEN", "JUMP-UNLESS"):
+            parts = s.split()
+            if word == "JUMP":
+                if len(parts) != 2:
+                    raise QuilParseError(f"bad JUMP: {s!r}")
+                code.append(Jump(_label_name(parts[1])))
+            else:
+                if len(parts) != 3:
+                    raise QuilParseError(f"bad {word}: {s!r}")
+                code.append(Jump(_label_name(parts[1]), word == "JUMP-WHEN", _mref(parts[2])))
+        elif word in _CLASSICAL:
+            nargs = _CLASSICAL[word]
+            # a two-operand instruction's source may be a constant expression
+            # with spaces in it, e.g. MOVE alpha[0] (pi / 2.0)
+            parts = s.split(None, nargs)
+            if len(parts) != nargs + 1 or (nargs == 3 and len(parts[3].split()) != 1):
+                raise QuilParseError(f"{word} takes {nargs} operand(s): {s!r}")
+            code.append(Classical(word, _mref(parts[1]), [_operand(x) for x in parts[2:]]))
         else:
             code.append(_parse_gate_line(s))
+    # label patching: jumps name instruction indices
+    where = {ins.name: k for k, ins in enumerate(code) if isinstance(ins, Label)}
+    for ins in code:
+        if isinstance(ins, Jump):
+            if ins.label not in where:
+                raise QuilParseError(f"jump to undefined label @{ins.label}")
+            ins.target = where[ins.label]
     return ParsedProgram(decls, defs, code)
+
+
+# operand counts of the classical instructions (destination included)
+_CLASSICAL = {"MOVE": 2, "EXCHANGE": 2, "CONVERT": 2, "NEG": 1, "NOT": 1, "ADD": 2, "SUB": 2, "MUL": 2, "DIV": 2, "AND": 2, "IOR": 2,
+              "XOR": 2, "EQ": 3, "GT": 3, "GE": 3, "LT": 3, "LE": 3}
+
+
+def _label_name(tok: str) -> str:
+    tok = tok.strip()
+    if not tok.startswith("@") or len(tok) < 2:
+        raise QuilParseError(f"bad label {tok!r}")
+    return tok[1:]
+
+
+def _mref(tok: str) -> Tuple[str, int]:
+    m = re.match(r"([A-Za-z_]\w*)(\[(\d+)\])?$", tok)
+    if not m:
+        raise QuilParseError(f"bad memory reference {tok!r}")
+    return (m.group(1), int(m.group(3) or 0))
+
+
+def _operand(tok: str):
+    """A classical operand: an integer or real literal, a memory reference,
+    or a constant expression (pi / 2.0), evaluated now."""
+    tok = tok.strip()
+    try:
+        return int(tok)
+    except ValueError:
+        pass
+    try:
+        return float(tok)
+    except ValueError:
+        pass
+    if re.match(r"[A-Za-z_]\w*(\[\d+\])?$", tok) and tok.lower() != "pi":
+        return _mref(tok)
+    try:
+        v = complex(parse_expression(tok)({}, {}))
+    except (QuilParseError, KeyError, TypeError, ValueError, ZeroDivisionError):
+        raise QuilParseError(f"bad classical operand {tok!r}")
+    if v.imag != 0:
+        raise QuilParseError(f"classical operand {tok!r} is not real")
+    return v.real
+
+
+def execute_classical(ins: Classical, memory: dict, types: Optional[Dict[str, str]] = None) -> None:
+    """One classical instruction on the classical memory
+    (src/transition-classical-instructions.lisp).  ``types`` maps a memory
+    name to its declared type (BIT, OCTET, INTEGER, REAL); NOT and CONVERT
+    depend on it (NOT of a BIT is 1 - b, of an OCTET the low 8 bits of the
+    complement, of an INTEGER the 64-bit complement, lines 32-51; CONVERT to
+    INTEGER rounds half to even, to BIT tests for non-zero, lines 149-191).
+    INTEGER arithmetic wraps to signed 64 bits like ensure-s64; INTEGER DIV
+    floors."""
+    def get(x):
+        if isinstance(x, tuple):
+            if x[0] not in memory:
+                raise QuilParseError(f"undeclared memory {x[0]}")
+            return memory[x[0]][x[1]]
+        return x
+    name, idx = ins.dest
+    if name not in memory:
+        raise QuilParseError(f"undeclared memory {name}")
+    cell = memory[name]
+    if not (0 <= idx < cell.size):
+        raise QuilParseError(f"memory reference {name}[{idx}] out of range")
+    ty = (types or {}).get(name, "REAL" if cell.dtype.kind == "f" else "INTEGER")
+    a = [get(x) for x in ins.args]
+    op = ins.op
+
+    def wrap(v):
+        if ty == "REAL":
+            return float(v)
+        v = int(v) & 0xFFFFFFFFFFFFFFFF
+        return v - (1 << 64) if v >> 63 else v
+
+    if op == "MOVE":
+        cell[idx] = wrap(a[0])
+    elif op == "EXCHANGE":
+        other = a[0]
+        oname, oidx = ins.args[0]
+        memory[oname][oidx] = cell[idx]
+        cell[idx] = other
+    elif op == "CONVERT":
+        v = a[0]
+        if ty == "REAL":
+            cell[idx] = float(v)
+        elif ty == "BIT":
+            cell[idx] = int(v != 0)
+        else:
+            cell[idx] = wrap(round(v) if isinstance(v, (float, np.floating)) else int(v))
+    elif op == "NEG":
+        cell[idx] = wrap(-cell[idx])
+    elif op == "NOT":
+        if ty == "BIT":
+            cell[idx] = 1 - int(cell[idx])
+        elif ty == "OCTET":
+            cell[idx] = ~int(cell[idx]) & 0xFF
+        else:
+            cell[idx] = wrap(~int(cell[idx]))
+    elif op in ("ADD", "SUB", "MUL", "DIV"):
+        x, y = (float(cell[idx]), float(a[0])) if ty == "REAL" else (int(cell[idx]), int(a[0]))
+        if op == "ADD":
+            r = x + y
+        elif op == "SUB":
+            r = x - y
+        elif op == "MUL":
+            r = x * y
+        elif ty == "REAL":
+            r = x / y
+        else:
+            if y == 0:
+                raise QuilParseError("integer division by zero")
+            r = x // y  # floor division (line 245)
+        cell[idx] = wrap(r)
+    elif op in ("AND", "IOR", "XOR"):
+        x, y = int(cell[idx]), int(a[0])
+        cell[idx] = wrap(x & y if op == "AND" else (x | y if op == "IOR" else x ^ y))
+    else:  # comparisons: dest <- left OP right
+        left, right = a
+        cell[idx] = int({"EQ": left == right, "GT": left > right, "GE": left >= right,
+                         "LT": left < right, "LE": left <= right}[op])
 
 
 def resolve_gate_matrix(ins: GateApplication, defs: Dict[str, GateDefinition], memory: dict) -> np.ndarray:

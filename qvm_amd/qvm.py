@@ -84,6 +84,87 @@ def compile_measure_chains(code, n: int) -> list:
     return code
 
 
+def transition_classical(qvm, ins) -> Optional[int]:
+    """Jumps and classical instructions on a machine with ``pc``, ``memory``
+    and ``program`` (src/transition.lisp:98-115: JUMP sets the pc to the
+    label, JUMP-WHEN when the bit is 1, JUMP-UNLESS when it is 0;
+    src/transition-classical-instructions.lisp).  Returns the next pc, or
+    None when ``ins`` is neither."""
+    if isinstance(ins, Q.Jump):
+        if ins.when is None:
+            return ins.target
+        name, idx = ins.cond
+        if name not in qvm.memory:
+            raise QVMError(f"undeclared memory {name}")
+        taken = int(qvm.memory[name][idx]) == 1 if ins.when else int(qvm.memory[name][idx]) == 0
+        return ins.target if taken else qvm.pc + 1
+    if isinstance(ins, Q.Classical):
+        types = {d.name: d.type for d in qvm.program.declarations.values()} if qvm.program else None
+        Q.execute_classical(ins, qvm.memory, types)
+        return qvm.pc + 1
+    return None
+
+
+def segment_program(code, gate, compile_run) -> list:
+    """Cut an instruction vector into steps: each maximal run of
+    instructions for which ``gate(ins)`` returns a (matrix, qubits) pair
+    becomes one ``compile_run(list_of_pairs)`` step; NOPs are dropped; every
+    other instruction is its own step.  Labels are steps of their own, so a
+    run never spans one and a jump always lands on a step boundary; jump
+    targets (instruction indices of labels) are renumbered to step indices."""
+    steps: list = []
+    run: list = []
+    label_step = {}
+    jumps = []
+
+    def flush():
+        if run:
+            steps.append(compile_run(list(run)))
+            run.clear()
+
+    for i, ins in enumerate(code):
+        g = gate(ins)
+        if g is not None:
+            run.append(g)
+        elif isinstance(ins, Q.Nop):
+            continue
+        else:
+            flush()
+            if isinstance(ins, Q.Label):
+                label_step[i] = len(steps)
+            elif isinstance(ins, Q.Jump):
+                jumps.append(len(steps))
+            steps.append(ins)
+    flush()
+    for j in jumps:
+        J = steps[j]
+        if J.target not in label_step:
+            raise QVMError(f"jump to @{J.label} does not land on a label")
+        steps[j] = Q.Jump(J.label, J.when, J.cond, label_step[J.target])
+    return steps
+
+
+class _MemoryProbe(dict):
+    """Stands in for classical memory while a gate's parameters are
+    evaluated, recording whether any of them reads it."""
+
+    def __init__(self, memory: dict):
+        super().__init__()
+        self.memory = memory
+        self.read = False
+
+    def __getitem__(self, name):
+        self.read = True
+        return self.memory[name]
+
+
+def _reads_memory(ins: "Q.GateApplication", memory: dict) -> bool:
+    probe = _MemoryProbe(memory)
+    for p in ins.params:
+        p({}, probe)
+    return probe.read
+
+
 class _UniformStream:
     def __init__(self, seed=None):
         self._g = np.random.default_rng(seed)
@@ -122,8 +203,13 @@ class PureStateQVM:
             dt = np.float64 if d.type == "REAL" else np.int64
             self.memory[d.name] = np.zeros(d.length, dtype=dt)
         self.pc = 0
-        self._compiled = None
+        self._drop_compiled()
         return self
+
+    def _drop_compiled(self) -> None:
+        if self._compiled is not None:
+            self._compiled.close()
+            self._compiled = None
 
     def memory_ref(self, name: str, index: int = 0):
         return self.memory[name][index]
@@ -185,8 +271,11 @@ class PureStateQVM:
             return self.pc + 1
         if isinstance(ins, Q.Halt):
             return None
-        if isinstance(ins, Q.Nop):
+        if isinstance(ins, (Q.Nop, Q.Label)):
             return self.pc + 1
+        r = transition_classical(self, ins)
+        if r is not None:
+            return r
         raise QVMError(f"unsupported instruction {ins!r}")
 
     # -- measurement -----------------------------------------------------
@@ -228,39 +317,33 @@ class PureStateQVM:
                                measure_chains: bool = True, **kw) -> "CompiledProgram":
         """compile-loaded-program (src/qvm.lisp:166-175).  Measure chains over
         all qubits become one measure-all (*compile-measure-chains*); every
-        maximal run of gate applications is planned into engine launches
-        (plain: one launch per gate; tiled: LDS tile passes; fused: dense
-        MFMA blocks; auto: the plan the trained launch-cost model prices
-        lowest) and captured; the other instructions run through
-        ``transition``.  Gate parameters are evaluated against classical
-        memory when the program is compiled."""
+        maximal run of gate applications with constant parameters is planned
+        into engine launches (plain: one launch per gate; tiled: LDS tile
+        passes; fused: dense MFMA blocks; auto: the plan the trained
+        launch-cost model prices lowest) and captured.  Gates whose
+        parameters read classical memory stay interpreted, as
+        compile-instruction leaves them (src/compile-gate.lisp:484-489), and
+        so do the other instructions.  Labels end a run, so a jump always
+        lands on a step boundary; jump targets are renumbered to steps."""
         from . import planner
         if self.program is None:
             raise QVMError("no program loaded")
         code = self.program.executable_code
         if measure_chains:
             code = compile_measure_chains(code, self.number_of_qubits)
-        steps = []
-        run: list = []
 
-        def flush():
-            if run:
-                steps.append(planner.compile_gates(self.state, list(run), mode=mode, use_graph=use_graph, **kw))
-                run.clear()
+        def gate(ins):
+            if not isinstance(ins, Q.GateApplication) or _reads_memory(ins, self.memory):
+                return None
+            U = Q.resolve_gate_matrix(ins, self.program.gate_definitions, self.memory)
+            if G.num_qubits_of(U) != len(ins.qubits):
+                raise QVMError(f"Gate {ins.name} takes {G.num_qubits_of(U)} qubit(s), "
+                               f"applied to {len(ins.qubits)}")
+            return (U, list(ins.qubits))
 
-        for ins in code:
-            if isinstance(ins, Q.GateApplication):
-                U = Q.resolve_gate_matrix(ins, self.program.gate_definitions, self.memory)
-                if G.num_qubits_of(U) != len(ins.qubits):
-                    raise QVMError(f"Gate {ins.name} takes {G.num_qubits_of(U)} qubit(s), "
-                                   f"applied to {len(ins.qubits)}")
-                run.append((U, list(ins.qubits)))
-            elif isinstance(ins, Q.Nop):
-                continue
-            else:
-                flush()
-                steps.append(ins)
-        flush()
+        steps = segment_program(code, gate, lambda run: planner.compile_gates(
+            self.state, run, mode=mode, use_graph=use_graph, **kw))
+        self._drop_compiled()
         self._compiled = CompiledProgram.from_steps(steps, mode)
         return self._compiled
 
@@ -275,6 +358,8 @@ class PureStateQVM:
         return self.state.probabilities_octets()
 
     def close(self) -> None:
+        """Release the compiled program's graphs and operands, then the state."""
+        self._drop_compiled()
         self.state.close()
 
 
@@ -316,14 +401,20 @@ class CompiledProgram:
         if self.program is not None:
             self.program.run()
             return
-        for st in self.steps:
+        pc = 0
+        while pc < len(self.steps):
+            st = self.steps[pc]
             if isinstance(st, CompiledProgram):
                 st.run()
-            else:
-                if qvm is None:
-                    raise QVMError("this compiled program has non-gate instructions; run it through its QVM")
-                if qvm.transition(st) is None:
-                    return  # HALT
+                pc += 1
+                continue
+            if qvm is None:
+                raise QVMError("this compiled program has non-gate instructions; run it through its QVM")
+            qvm.pc = pc  # transition returns the next step: pc + 1, or a jump's step
+            nxt = qvm.transition(st)
+            if nxt is None:
+                return  # HALT
+            pc = nxt
 
     def close(self) -> None:
         if self.program is not None:
