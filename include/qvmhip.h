@@ -54,8 +54,19 @@ enum {
     QVMHIP_TILE_DIAG2 = 4,  /* diag(m[0..3]) on (q0, q1)                       */
     QVMHIP_TILE_CPERM = 5,  /* swap |..c=1,t=0..> <-> |..c=1,t=1..>: q0 = t, q1 = c (CNOT) */
     QVMHIP_TILE_SWAP = 6,   /* swap local qubits q0, q1                        */
-    QVMHIP_TILE_CPHASE = 7  /* multiply amplitudes with q0 = q1 = 1 by m[0]   */
+    QVMHIP_TILE_CPHASE = 7, /* multiply amplitudes with q0 = q1 = 1 by m[0]   */
+    QVMHIP_TILE_GDIAG = 8   /* diagonal that also reads qubits outside the tile (below) */
 };
+
+/* QVMHIP_TILE_GDIAG: a diagonal gate on local qubit q0 and one or two state
+ * qubits g0, g1 that are NOT tile qubits.  Outside the tile those qubits are
+ * constant over a block, so the block's base index selects the entries.
+ * q1 packs them: g0 = q1 & 0xff, g1 = (q1 >> 8) & 0xff, ng = q1 >> 16
+ * (1 or 2).  m holds 2^(1+ng) complex entries, index = b(q0) + 2 b(g0) +
+ * 4 b(g1).  ctrl_mask must be 0.  A 1- or 2-qubit diagonal whose qubits are
+ * all outside the tile uses any local q0 with each entry repeated.  It lets
+ * a pass apply diagonal gates (RZ, CZ, CPHASE, RZZ) on any qubits. */
+#define QVMHIP_GDIAG_PACK(g0, g1, ng) ((uint32_t)(g0) | ((uint32_t)(g1) << 8) | ((uint32_t)(ng) << 16))
 
 /* One gate of a tile pass.  q0/q1 are LOCAL qubit positions in the tile
  * (0 .. m-1), ctrl_mask is a mask of local positions that must be 1 for the

@@ -33,6 +33,7 @@ TILE_DIAG2 = 4
 TILE_CPERM = 5
 TILE_SWAP = 6
 TILE_CPHASE = 7
+TILE_GDIAG = 8
 # slot-space kinds emitted by the tile scheduler (qvmhip_tile_schedule)
 TILE_XREG = 16
 TILE_RSWAP = 17

@@ -260,6 +260,7 @@ int qvmhip_apply_tile_pass(qvmhip_state *s, uint32_t m, const uint32_t *tile_qub
         if (i && tile_qubits[i] <= tile_qubits[i - 1]) return fail(QVMHIP_EINVAL, "tile qubits must ascend");
     }
     int rc = check_tile_ops(s->dtype, m, nops, ops);
+    if (!rc) rc = check_tile_globals(s->n, m, tile_qubits, nops, ops);
     if (rc) return rc;
     if (nops == 0) return QVMHIP_OK;
     return launch_tile_pass(s, m, tile_qubits, nops, ops, 0);

@@ -154,7 +154,7 @@ int enqueue(qvmhip_batch *b) {
         if (L.kind == kItemTile)
             rc = launch_tile_batch(b->stream, b->dtype, (b->flags & QVMHIP_FLAG_FMA_TILES) != 0, L.m, b->buf,
                                    (const char *)b->entries_dev + L.ent_off, b->blkmap_dev + L.map_off, L.blocks,
-                                   kTilesPerBlock);
+                                   kTilesPerBlock, b->blob_dev);
         else
             rc = launch_gate(&b->views[L.inst], L.item->U.data(), L.item->k, L.item->qubits, 0);
         if (rc) return rc;
@@ -263,6 +263,7 @@ int qvmhip_batch_add_tile_pass(qvmhip_batch *b, uint32_t inst, uint32_t m, const
         if (i && tile_qubits[i] <= tile_qubits[i - 1]) return fail(QVMHIP_EINVAL, "tile qubits must ascend");
     }
     rc = check_tile_ops(b->dtype, m, nops, ops);
+    if (!rc) rc = check_tile_globals(n, m, tile_qubits, nops, ops);
     if (rc) return rc;
     if (!nops) return QVMHIP_OK;
     BItem it;
@@ -365,7 +366,7 @@ int qvmhip_batch_finalize(qvmhip_batch *b, int use_graph) {
                 std::vector<char> ent(esz);
                 const uint64_t nb = build_tile_entry(
                     ent.data(), b->dtype, (b->flags & QVMHIP_FLAG_FMA_TILES) != 0, b->nq[i], it.k, it.qubits,
-                    b->offset[i], blocks, (const qvmhip_tile_op *)((const char *)b->blob_dev + it.op_off), it.nops,
+                    b->offset[i], blocks, it.op_off, it.nops,
                     kTilesPerBlock);
                 b->entries.insert(b->entries.end(), ent.begin(), ent.end());
                 b->blkmap.insert(b->blkmap.end(), (size_t)nb, local);

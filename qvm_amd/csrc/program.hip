@@ -143,6 +143,7 @@ int qvmhip_program_add_tile_pass(qvmhip_program *p, uint32_t m, const uint32_t *
         if (i && tile_qubits[i] <= tile_qubits[i - 1]) return fail(QVMHIP_EINVAL, "tile qubits must ascend");
     }
     int rc = check_tile_ops(p->s->dtype, m, nops, ops);
+    if (!rc) rc = check_tile_globals(p->s->n, m, tile_qubits, nops, ops);
     if (rc) return rc;
     if (nops == 0) return QVMHIP_OK;
     Item it;

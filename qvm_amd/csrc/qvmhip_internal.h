@@ -191,16 +191,20 @@ int sample_to_device(qvmhip_state *s, const double *u_dev, uint64_t nshots, uint
 // Tile passes: validation of a public op list (tile-local bit positions) and
 // its translation into the slot-space program the tile kernels run.
 int check_tile_ops(int dtype, uint32_t m, uint32_t nops, const qvmhip_tile_op *ops);
+// GDIAG outside qubits: in range, not tile qubits, distinct
+int check_tile_globals(uint32_t n, uint32_t m, const uint32_t *tq, uint32_t nops, const qvmhip_tile_op *ops);
 void schedule_tile_ops(int dtype, bool fma, uint32_t m, const qvmhip_tile_op *in, uint32_t nin,
                        std::vector<qvmhip_tile_op> &out);
 size_t tile_entry_bytes();
 // Fills one batch tile entry; returns its number of work blocks of
 // tiles_per_block tiles.
 uint64_t build_tile_entry(void *entry, int dtype, bool fma, uint32_t n, uint32_t m, const uint32_t *tq,
-                          uint64_t amp_offset, uint64_t block_start, const qvmhip_tile_op *ops_dev, uint32_t nops,
+                          uint64_t amp_offset, uint64_t block_start, uint64_t op_off, uint32_t nops,
                           uint32_t tiles_per_block);
+// op_buf: the device buffer the entries' op offsets point into
 int launch_tile_batch(hipStream_t st, int dtype, bool fma, uint32_t m, void *base, const void *entries_dev,
-                      const uint32_t *blk_ent_dev, uint64_t total_blocks, uint32_t tiles_per_block);
+                      const uint32_t *blk_ent_dev, uint64_t total_blocks, uint32_t tiles_per_block,
+                      const void *op_buf);
 }  // namespace qvmhip
 
 #define QH_CHECK(expr)                                                                    \

@@ -336,6 +336,36 @@ __device__ __forceinline__ void reg_diag(vec2_t<T> (&x)[R], const qvmhip_tile_op
     }
 }
 
+// Diagonal on local slot bit q0 whose entries also depend on state qubits
+// outside the tile: `sel` (their bits in the block's base index, uniform)
+// picks the entry pair (2 sel, 2 sel + 1), and q0 picks within it.
+template <typename T, int R, int TB, bool FMA>
+__device__ __forceinline__ void reg_gdiag(vec2_t<T> (&x)[R], const qvmhip_tile_op &op, uint32_t sel, uint32_t t) {
+    using V = vec2_t<T>;
+    const T r0 = mval<T>(op, 4 * sel), i0 = mval<T>(op, 4 * sel + 1);
+    const T r1 = mval<T>(op, 4 * sel + 2), i1 = mval<T>(op, 4 * sel + 3);
+    const uint32_t q0 = op.q0;
+    if (q0 >= TB) {
+        const uint32_t rs = q0 - TB;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const bool one = (((uint32_t)r >> rs) & 1u) != 0;  // uniform per register
+            V y;
+            cmul<T, FMA>(x[r].x, x[r].y, one ? r1 : r0, one ? i1 : i0, y.x, y.y);
+            x[r] = y;
+        }
+        return;
+    }
+    const bool one = ((t >> q0) & 1u) != 0;
+    const T dr = one ? r1 : r0, di = one ? i1 : i0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        V y;
+        cmul<T, FMA>(x[r].x, x[r].y, dr, di, y.x, y.y);
+        x[r] = y;
+    }
+}
+
 // LDS address of slot s: register row r = s >> TB of NT slots is padded by
 // one element, so a thread's store of register r is t + r*(NT+1) (a fixed
 // offset from t), and a lane bit that a REMAP takes from register bit b of the
@@ -375,17 +405,17 @@ __device__ __forceinline__ void reg_remap(vec2_t<T> (&x)[R], vec2_t<T> *sa, cons
 // One tile: load the 2^M amplitudes of tile `tile` into registers, run the
 // scheduled op list, store.  Shared by the single-state and batched kernels.
 template <typename T, int M, int RB>
-__device__ __forceinline__ void tile_load(const vec2_t<T> *__restrict__ psi, const TileArgs &a, uint64_t tile,
+__device__ __forceinline__ void tile_load(const vec2_t<T> *__restrict__ psi, const TileArgs &a, uint64_t gb,
                                           uint64_t toff, vec2_t<T> (&x)[1 << RB]) {
-    const vec2_t<T> *p = psi + group_base(a.g, tile) + toff;
+    const vec2_t<T> *p = psi + gb + toff;
 #pragma unroll
     for (int r = 0; r < (1 << RB); ++r) x[r] = p[a.roff[r]];
 }
 
 template <typename T, int M, int RB>
-__device__ __forceinline__ void tile_store(vec2_t<T> *__restrict__ psi, const TileArgs &a, uint64_t tile,
+__device__ __forceinline__ void tile_store(vec2_t<T> *__restrict__ psi, const TileArgs &a, uint64_t gb,
                                            uint64_t toff, const vec2_t<T> (&x)[1 << RB]) {
-    vec2_t<T> *p = psi + group_base(a.g, tile) + toff;
+    vec2_t<T> *p = psi + gb + toff;
     // recompute the store addresses instead of keeping them live across
     // the op loop
     asm volatile("" : "+v"(p));
@@ -396,7 +426,7 @@ __device__ __forceinline__ void tile_store(vec2_t<T> *__restrict__ psi, const Ti
 // Run a scheduled op list on the block held in x.
 template <typename T, int M, int RB, bool FMA>
 __device__ inline void tile_ops(vec2_t<T> (&x)[1 << RB], const qvmhip_tile_op *__restrict__ ops, uint32_t nops,
-                                vec2_t<T> *sa) {
+                                vec2_t<T> *sa, uint64_t gb) {
     constexpr int R = 1 << RB, TB = M - RB;
     const uint32_t t = threadIdx.x;
     for (uint32_t oi = 0; oi < nops; ++oi) {
@@ -447,6 +477,13 @@ __device__ inline void tile_ops(vec2_t<T> (&x)[1 << RB], const qvmhip_tile_op *_
             case kTileRemap:
                 reg_remap<T, R, TB>(x, sa, op, t);
                 break;
+            case QVMHIP_TILE_GDIAG: {
+                const uint32_t d = op.q1;
+                uint32_t sel = (uint32_t)(gb >> (d & 63u)) & 1u;
+                if ((d >> 16) == 2u) sel |= ((uint32_t)(gb >> ((d >> 8) & 63u)) & 1u) << 1;
+                reg_gdiag<T, R, TB, FMA>(x, op, sel, t);
+                break;
+            }
             default:  // DIAG1, DIAG2, CPHASE
                 if (ctrl)
                     reg_diag<T, R, TB, true, FMA>(x, op, t, lane_ok, rcm);
@@ -468,9 +505,10 @@ __global__ __launch_bounds__(1 << (M - RB)) void k_tile(vec2_t<T> *__restrict__ 
     // many resident workgroups overlap each other's loads, ops and stores.
     for (uint64_t tile = blockIdx.x; tile < a.g.ngroups; tile += gridDim.x) {
         V x[1 << RB];
-        tile_load<T, M, RB>(psi, a, tile, toff, x);
-        tile_ops<T, M, RB, FMA>(x, ops, nops, sa);
-        tile_store<T, M, RB>(psi, a, tile, toff, x);
+        const uint64_t gb = group_base(a.g, tile);
+        tile_load<T, M, RB>(psi, a, gb, toff, x);
+        tile_ops<T, M, RB, FMA>(x, ops, nops, sa, gb);
+        tile_store<T, M, RB>(psi, a, gb, toff, x);
     }
 }
 
@@ -478,11 +516,16 @@ __global__ __launch_bounds__(1 << (M - RB)) void k_tile(vec2_t<T> *__restrict__ 
 // b of a launch belongs to entry blk_ent[b] (a host-built map, one load
 // instead of a search) and covers tiles [(b - block_start) * CH, ... + CH) of
 // that entry's state.
+// The op lists of all entries live in one device buffer passed to the kernel
+// as a restrict argument; an entry holds its byte offset there.  (A pointer
+// read from memory would not be known to be unaliased with the state, and
+// the op loads would become per-lane vector loads with every op field in
+// vector registers instead of scalar loads into SGPRs.)
 struct TileEntry {
     TileArgs args;
     uint64_t amp_offset;   // first amplitude of the state in the batch buffer
     uint64_t block_start;  // first work block of the launch that works on this entry
-    const qvmhip_tile_op *ops;
+    uint64_t op_off;       // byte offset of the entry's scheduled ops in the op buffer
     uint32_t nops;
     uint32_t pad;
 };
@@ -491,7 +534,8 @@ template <typename T, int M, int RB, bool FMA>
 __global__ __launch_bounds__(1 << (M - RB)) void k_tile_batch(vec2_t<T> *__restrict__ base,
                                                              const TileEntry *__restrict__ ent,
                                                              const uint32_t *__restrict__ blk_ent,
-                                                             uint64_t total_blocks, uint32_t tiles_per_block) {
+                                                             uint64_t total_blocks, uint32_t tiles_per_block,
+                                                             const char *__restrict__ op_buf) {
     using V = vec2_t<T>;
     __shared__ V sa[(1 << M) + (1 << RB)];
     for (uint64_t b = blockIdx.x; b < total_blocks; b += gridDim.x) {
@@ -502,9 +546,10 @@ __global__ __launch_bounds__(1 << (M - RB)) void k_tile_batch(vec2_t<T> *__restr
         V *psi = base + e.amp_offset;
         for (uint64_t tile = first; tile < last; ++tile) {
             V x[1 << RB];
-            tile_load<T, M, RB>(psi, e.args, tile, toff, x);
-            tile_ops<T, M, RB, FMA>(x, e.ops, e.nops, sa);
-            tile_store<T, M, RB>(psi, e.args, tile, toff, x);
+            const uint64_t gb = group_base(e.args.g, tile);
+            tile_load<T, M, RB>(psi, e.args, gb, toff, x);
+            tile_ops<T, M, RB, FMA>(x, (const qvmhip_tile_op *)(op_buf + e.op_off), e.nops, sa, gb);
+            tile_store<T, M, RB>(psi, e.args, gb, toff, x);
         }
     }
 }
@@ -517,14 +562,37 @@ int check_tile_ops(int dtype, uint32_t m, uint32_t nops, const qvmhip_tile_op *o
         return fail(QVMHIP_EINVAL, "tile size m must be 8..12 (F64) or 8..13 (F32)");
     for (uint32_t o = 0; o < nops; ++o) {
         const qvmhip_tile_op &op = ops[o];
-        if (op.kind < QVMHIP_TILE_DENSE1 || op.kind > QVMHIP_TILE_CPHASE)
+        if (op.kind < QVMHIP_TILE_DENSE1 || op.kind > QVMHIP_TILE_GDIAG)
             return fail(QVMHIP_EINVAL, "unknown tile op kind");
+        if (op.kind == QVMHIP_TILE_GDIAG) {
+            const uint32_t ng = op.q1 >> 16;
+            if (op.q0 >= m) return fail(QVMHIP_EINVAL, "tile op qubit out of range");
+            if (ng < 1 || ng > 2) return fail(QVMHIP_EINVAL, "GDIAG reads one or two outside qubits");
+            if (op.ctrl_mask) return fail(QVMHIP_EINVAL, "GDIAG takes no controls");
+            continue;
+        }
         const bool two = op.kind != QVMHIP_TILE_DENSE1 && op.kind != QVMHIP_TILE_DIAG1;
         if (op.q0 >= m || (two && (op.q1 >= m || op.q1 == op.q0)))
             return fail(QVMHIP_EINVAL, "tile op qubit out of range");
         if (m < 32 && (op.ctrl_mask >> m)) return fail(QVMHIP_EINVAL, "tile op control outside the tile");
         if ((op.ctrl_mask >> op.q0) & 1u || (two && ((op.ctrl_mask >> op.q1) & 1u)))
             return fail(QVMHIP_EINVAL, "tile op control overlaps its targets");
+    }
+    return QVMHIP_OK;
+}
+
+int check_tile_globals(uint32_t n, uint32_t m, const uint32_t *tq, uint32_t nops, const qvmhip_tile_op *ops) {
+    for (uint32_t o = 0; o < nops; ++o) {
+        const qvmhip_tile_op &op = ops[o];
+        if (op.kind != QVMHIP_TILE_GDIAG) continue;
+        const uint32_t ng = op.q1 >> 16;
+        const uint32_t g[2] = {op.q1 & 0xffu, (op.q1 >> 8) & 0xffu};
+        for (uint32_t k = 0; k < ng; ++k) {
+            if (g[k] >= n) return fail(QVMHIP_EINVAL, "GDIAG qubit out of range");
+            for (uint32_t i = 0; i < m; ++i)
+                if (tq[i] == g[k]) return fail(QVMHIP_EINVAL, "GDIAG outside qubit is a tile qubit");
+        }
+        if (ng == 2 && g[0] == g[1]) return fail(QVMHIP_EINVAL, "GDIAG outside qubits repeat");
     }
     return QVMHIP_OK;
 }
@@ -700,8 +768,9 @@ void schedule_tile_ops(int dtype, bool fma, uint32_t m, const qvmhip_tile_op *in
         qvmhip_tile_op o = op;
         o.ctrl_mask = L.slot_mask(op.ctrl_mask);
         o.q0 = (uint32_t)L.l2s[op.q0];
-        const bool two = op.kind != QVMHIP_TILE_DENSE1 && op.kind != QVMHIP_TILE_DIAG1;
-        o.q1 = two ? (uint32_t)L.l2s[op.q1] : 0u;
+        const bool two = op.kind != QVMHIP_TILE_DENSE1 && op.kind != QVMHIP_TILE_DIAG1 &&
+                         op.kind != QVMHIP_TILE_GDIAG;
+        if (op.kind != QVMHIP_TILE_GDIAG) o.q1 = two ? (uint32_t)L.l2s[op.q1] : 0u;  // GDIAG: outside qubits
         if (op.kind == QVMHIP_TILE_CPERM) {  // X on the target where the control is 1
             o.kind = kTileXReg;
             o.ctrl_mask |= 1u << L.l2s[op.q1];
@@ -822,14 +891,14 @@ int launch_tile_pass(qvmhip_state *s, uint32_t m, const uint32_t *tile_qubits, u
 size_t tile_entry_bytes() { return sizeof(TileEntry); }
 
 uint64_t build_tile_entry(void *entry, int dtype, bool fma, uint32_t n, uint32_t m, const uint32_t *tq,
-                          uint64_t amp_offset, uint64_t block_start, const qvmhip_tile_op *ops_dev, uint32_t nops,
+                          uint64_t amp_offset, uint64_t block_start, uint64_t op_off, uint32_t nops,
                           uint32_t tiles_per_block) {
     TileEntry e;
     std::memset(&e, 0, sizeof(e));
     make_tile_args(n, m, tq, (int)m - tile_reg_bits((int)m, dtype == QVMHIP_F64, fma), e.args);
     e.amp_offset = amp_offset;
     e.block_start = block_start;
-    e.ops = ops_dev;
+    e.op_off = op_off;
     e.nops = nops;
     std::memcpy(entry, &e, sizeof(e));
     return (e.args.g.ngroups + tiles_per_block - 1) / tiles_per_block;
@@ -838,25 +907,25 @@ uint64_t build_tile_entry(void *entry, int dtype, bool fma, uint32_t n, uint32_t
 namespace {
 template <typename T, int M, bool FMA>
 void tile_batch_go(hipStream_t st, void *base, const void *ent, const uint32_t *blk_ent, uint64_t total,
-                   uint32_t tpb) {
+                   uint32_t tpb, const void *op_buf) {
     constexpr int RB = tile_reg_bits(M, sizeof(T) == 8, FMA);
     constexpr int NT = 1 << (M - RB);
     const uint32_t grid = (uint32_t)std::min<uint64_t>(total, kMaxGrid);
     hipLaunchKernelGGL((k_tile_batch<T, M, RB, FMA>), dim3(grid), dim3(NT), 0, st, (vec2_t<T> *)base,
-                       (const TileEntry *)ent, blk_ent, total, tpb);
+                       (const TileEntry *)ent, blk_ent, total, tpb, (const char *)op_buf);
 }
 template <typename T, bool FMA>
 int tile_batch_m(hipStream_t st, uint32_t m, void *base, const void *ent, const uint32_t *blk_ent, uint64_t total,
-                 uint32_t tpb) {
+                 uint32_t tpb, const void *op_buf) {
     switch (m) {
-        case 8: tile_batch_go<T, 8, FMA>(st, base, ent, blk_ent, total, tpb); break;
-        case 9: tile_batch_go<T, 9, FMA>(st, base, ent, blk_ent, total, tpb); break;
-        case 10: tile_batch_go<T, 10, FMA>(st, base, ent, blk_ent, total, tpb); break;
-        case 11: tile_batch_go<T, 11, FMA>(st, base, ent, blk_ent, total, tpb); break;
-        case 12: tile_batch_go<T, 12, FMA>(st, base, ent, blk_ent, total, tpb); break;
+        case 8: tile_batch_go<T, 8, FMA>(st, base, ent, blk_ent, total, tpb, op_buf); break;
+        case 9: tile_batch_go<T, 9, FMA>(st, base, ent, blk_ent, total, tpb, op_buf); break;
+        case 10: tile_batch_go<T, 10, FMA>(st, base, ent, blk_ent, total, tpb, op_buf); break;
+        case 11: tile_batch_go<T, 11, FMA>(st, base, ent, blk_ent, total, tpb, op_buf); break;
+        case 12: tile_batch_go<T, 12, FMA>(st, base, ent, blk_ent, total, tpb, op_buf); break;
         case 13:
             if constexpr (sizeof(T) == 4) {
-                tile_batch_go<T, 13, FMA>(st, base, ent, blk_ent, total, tpb);
+                tile_batch_go<T, 13, FMA>(st, base, ent, blk_ent, total, tpb, op_buf);
                 break;
             }
             return fail(QVMHIP_EINVAL, "tile size 13 is FP32 only");
@@ -869,14 +938,17 @@ int tile_batch_m(hipStream_t st, uint32_t m, void *base, const void *ent, const 
 }  // namespace
 
 int launch_tile_batch(hipStream_t st, int dtype, bool fma, uint32_t m, void *base, const void *entries_dev,
-                      const uint32_t *blk_ent_dev, uint64_t total_blocks, uint32_t tiles_per_block) {
+                      const uint32_t *blk_ent_dev, uint64_t total_blocks, uint32_t tiles_per_block,
+                      const void *op_buf) {
     if (fma)
-        return dtype == QVMHIP_F64
-                   ? tile_batch_m<double, true>(st, m, base, entries_dev, blk_ent_dev, total_blocks, tiles_per_block)
-                   : tile_batch_m<float, true>(st, m, base, entries_dev, blk_ent_dev, total_blocks, tiles_per_block);
-    return dtype == QVMHIP_F64
-               ? tile_batch_m<double, false>(st, m, base, entries_dev, blk_ent_dev, total_blocks, tiles_per_block)
-               : tile_batch_m<float, false>(st, m, base, entries_dev, blk_ent_dev, total_blocks, tiles_per_block);
+        return dtype == QVMHIP_F64 ? tile_batch_m<double, true>(st, m, base, entries_dev, blk_ent_dev, total_blocks,
+                                                                tiles_per_block, op_buf)
+                                   : tile_batch_m<float, true>(st, m, base, entries_dev, blk_ent_dev, total_blocks,
+                                                               tiles_per_block, op_buf);
+    return dtype == QVMHIP_F64 ? tile_batch_m<double, false>(st, m, base, entries_dev, blk_ent_dev, total_blocks,
+                                                             tiles_per_block, op_buf)
+                               : tile_batch_m<float, false>(st, m, base, entries_dev, blk_ent_dev, total_blocks,
+                                                            tiles_per_block, op_buf);
 }
 
 }  // namespace qvmhip

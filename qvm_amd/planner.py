@@ -71,9 +71,11 @@ class _Ready:
 
 
 def group_in_order(gates_qubits: List[Tuple[int, ...]], width: int, seed_qubits=(),
-                   standalone=lambda i: False) -> List[Tuple[str, List[int], List[int]]]:
+                   standalone=lambda i: False, anywhere=lambda i: False) -> List[Tuple[str, List[int], List[int]]]:
     """Groups of consecutive gates (original order kept, so the grouped
-    execution performs exactly the plain path's floating-point operations)."""
+    execution performs exactly the plain path's floating-point operations).
+    ``anywhere(i)``: gate i joins the current group whatever its qubits (a
+    diagonal the tile kernel applies from the block's base index)."""
     out = []
     T: set = set(seed_qubits)
     members: List[int] = []
@@ -83,6 +85,9 @@ def group_in_order(gates_qubits: List[Tuple[int, ...]], width: int, seed_qubits=
                 out.append(("group", sorted(T), members))
             out.append(("single", list(qs), [i]))
             T, members = set(seed_qubits), []
+            continue
+        if anywhere(i):
+            members.append(i)
             continue
         if len(T | set(qs)) > width:
             out.append(("group", sorted(T), members))
@@ -95,7 +100,7 @@ def group_in_order(gates_qubits: List[Tuple[int, ...]], width: int, seed_qubits=
 
 
 def group_gates(gates_qubits: List[Tuple[int, ...]], nq: int, width: int, seed_qubits=(),
-                standalone=lambda i: False) -> List[Tuple[str, List[int], List[int]]]:
+                standalone=lambda i: False, anywhere=lambda i: False) -> List[Tuple[str, List[int], List[int]]]:
     """Greedy dependency-aware grouping.
 
     Returns a list of (kind, qubit set, gate indices in application order);
@@ -103,7 +108,8 @@ def group_gates(gates_qubits: List[Tuple[int, ...]], nq: int, width: int, seed_q
     groups, applied on its own).  Gates on disjoint qubits may move past each
     other: the unitary is unchanged, the rounding of the result is not
     (floating-point gate applications on disjoint qubits do not commute
-    bit-for-bit).
+    bit-for-bit).  A gate with ``anywhere(i)`` joins a group as soon as it is
+    ready, without adding its qubits to the group's set.
     """
     R = _Ready(gates_qubits, nq)
     out = []
@@ -122,7 +128,7 @@ def group_gates(gates_qubits: List[Tuple[int, ...]], nq: int, width: int, seed_q
             for i in R.ready():
                 if standalone(i):
                     continue
-                if set(gates_qubits[i]) <= T:
+                if anywhere(i) or set(gates_qubits[i]) <= T:
                     R.take(i)
                     members.append(i)
                     progressed = True
@@ -158,12 +164,44 @@ def _is_diag(U):
     return np.count_nonzero(U - np.diag(np.diag(U))) == 0
 
 
+def _gdiag_op(U: np.ndarray, nat: Sequence[int], local) -> N.TileOp:
+    """A diagonal gate with qubits outside the tile as a GDIAG op: local
+    qubit q0 (any local position when the gate has none inside) and the
+    outside qubits g0, g1; entry index b(q0) + 2 b(g0) + 4 b(g1)."""
+    d = np.diag(U)
+    inside = [q for q in nat if q in local]
+    outside = [q for q in nat if q not in local]
+    if len(inside) > 1 or not 1 <= len(outside) <= 2:
+        raise ValueError("GDIAG covers one tile qubit and one or two outside qubits")
+    q0 = local[inside[0]] if inside else 0
+    ng = len(outside)
+    vals = np.zeros(1 << (1 + ng), dtype=np.complex128)
+    for e in range(vals.size):
+        j = 0  # the gate's diagonal index: bit b <-> nat[b]
+        for b, q in enumerate(nat):
+            bit = (e & 1) if q not in outside else (e >> (1 + outside.index(q))) & 1
+            j |= bit << b
+        vals[e] = d[j]
+    op = N.TileOp()
+    op.kind = N.TILE_GDIAG
+    op.q0 = q0
+    op.q1 = outside[0] | ((outside[1] << 8) if ng == 2 else 0) | (ng << 16)
+    for i, x in enumerate(vals.view(np.float64)):
+        op.m[i] = x
+    return op
+
+
 def tile_op(U: np.ndarray, quil_qubits: Sequence[int], local) -> N.TileOp:
-    """Translate one 1- or 2-qubit gate into a tile op on local positions."""
+    """Translate one 1- or 2-qubit gate into a tile op on local positions
+    (a diagonal gate may also name qubits outside the tile: GDIAG)."""
     op = N.TileOp()
     m = np.zeros(32)
     nat = G.nat_tuple(*quil_qubits)
     k = len(nat)
+    if any(q not in local for q in nat):
+        if not _is_diag(U):
+            raise ValueError("only diagonal gates may name qubits outside the tile")
+        return _gdiag_op(U, nat, local)
     if k == 1:
         op.q0 = local[nat[0]]
         if _is_diag(U):
@@ -302,10 +340,15 @@ def plan_tiled(gates: List[Gate], nq: int, m: int, low: int = 4, merge_1q: bool 
     m = min(m, nq)
     low = min(low, m)
     qs = [tuple(q) for _, q in gates]
+    # 1- and 2-qubit diagonals run in any pass (GDIAG reads the qubits
+    # outside the tile from the block's base index)
+    diag = [len(q) <= 2 and _is_diag(U) for U, q in gates]
     if reorder:
-        groups = group_gates(qs, nq, m, seed_qubits=range(low), standalone=lambda i: len(qs[i]) > 2)
+        groups = group_gates(qs, nq, m, seed_qubits=range(low), standalone=lambda i: len(qs[i]) > 2,
+                             anywhere=lambda i: diag[i])
     else:
-        groups = group_in_order(qs, m, seed_qubits=range(low), standalone=lambda i: len(qs[i]) > 2)
+        groups = group_in_order(qs, m, seed_qubits=range(low), standalone=lambda i: len(qs[i]) > 2,
+                                anywhere=lambda i: diag[i])
     plan = []
     for kind, T, members in groups:
         if kind == "single":

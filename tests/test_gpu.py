@@ -336,17 +336,25 @@ def test_tile_passes_vs_plain(S, oracle, m, reorder):
     st.close()
 
 
-def _random_tile_ops(m, count, rng):
-    """Random tile ops of every public kind, some controlled (local bits)."""
+def _random_tile_ops(m, count, rng, outside=()):
+    """Random tile ops of every public kind, some controlled (local bits);
+    GDIAG ops on one or two of the state qubits ``outside`` the tile."""
     ops = []
+    kinds = [1, 1, 2, 3, 4, 5, 5, 6, 6, 7] + ([8, 8] if len(outside) else [])
     for _ in range(count):
-        kind = int(rng.choice([1, 1, 2, 3, 4, 5, 5, 6, 6, 7]))
+        kind = int(rng.choice(kinds))
         q = [int(x) for x in rng.choice(m, 3, replace=False)]
         ctrl = (1 << q[2]) if rng.random() < 0.25 else 0
         op = N.TileOp()
         op.kind, op.q0, op.ctrl_mask = kind, q[0], ctrl
         op.q1 = q[1] if kind not in (N.TILE_DENSE1, N.TILE_DIAG1) else 0
-        if kind == N.TILE_DENSE1:
+        if kind == N.TILE_GDIAG:
+            ng = 1 if len(outside) == 1 else int(rng.integers(1, 3))
+            g = [int(x) for x in rng.choice(list(outside), ng, replace=False)] + [0]
+            op.ctrl_mask = 0
+            op.q1 = g[0] | ((g[1] << 8) if ng == 2 else 0) | (ng << 16)
+            vals = np.exp(1j * rng.uniform(0, 6, 2 << ng))
+        elif kind == N.TILE_DENSE1:
             vals = rand_unitary(1, rng).reshape(-1)
         elif kind == N.TILE_DENSE2:
             vals = rand_unitary(2, rng).reshape(-1)
@@ -384,6 +392,10 @@ def _tile_op_as_gate(op, tq):
         U, qs = G.gate_matrix("CNOT"), [g(op.q1), g(op.q0)]
     elif op.kind == N.TILE_SWAP:
         U, qs = G.gate_matrix("SWAP"), [g(op.q1), g(op.q0)]
+    elif op.kind == N.TILE_GDIAG:
+        ng = op.q1 >> 16
+        outside = [op.q1 & 0xFF, (op.q1 >> 8) & 0xFF][:ng]
+        U, qs = np.diag(mm[:1 << (1 + ng)]), list(reversed([g(op.q0)] + outside))
     else:
         U, qs = np.diag([1, 1, 1, mm[0]]), [g(op.q1), g(op.q0)]
     for c in [b for b in range(32) if (op.ctrl_mask >> b) & 1]:
@@ -395,7 +407,8 @@ def _tile_op_as_gate(op, tq):
                                          ("f64", 11, True), ("f32", 9, False), ("f32", 13, True)])
 def test_tile_pass_every_op_kind(S, oracle, dtype, m, fma):
     """Random passes of every op kind (controlled dense 1q/2q, diagonals on
-    thread and register bits, CNOT, controlled and plain SWAP, CPHASE) through
+    thread and register bits, CNOT, controlled and plain SWAP, CPHASE,
+    diagonals on qubits outside the tile) through
     qvmhip_apply_tile_pass: exact FP64 passes equal the oracle bit for bit."""
     from qvm_amd.state import HipPureState
     rng = np.random.default_rng(m * 7 + (1 if fma else 0))
@@ -406,7 +419,7 @@ def test_tile_pass_every_op_kind(S, oracle, dtype, m, fma):
     st.write_amplitudes(psi)
     ref = psi.copy()
     for rep in range(3):
-        ops = _random_tile_ops(m, 50, rng)
+        ops = _random_tile_ops(m, 50, rng, outside=[q for q in range(n) if q not in tq])
         say(f"tile ops {dtype} m={m} fma={fma} rep={rep}")
         st.apply_tile_pass(tq, ops)
         for op in ops:

@@ -48,6 +48,12 @@ def emulate_tile_op(op, tq, n, psi):
         U, qs = G.gate_matrix("SWAP"), [g(op.q1), g(op.q0)]
     elif op.kind == N.TILE_CPHASE:
         U, qs = np.diag([1, 1, 1, m[0]]), [g(op.q1), g(op.q0)]
+    elif op.kind == N.TILE_GDIAG:
+        # diagonal over (tile qubit q0, outside qubits g0[, g1]): entry
+        # index b(q0) + 2 b(g0) + 4 b(g1); Quil order is the reverse
+        ng = op.q1 >> 16
+        outside = [op.q1 & 0xFF, (op.q1 >> 8) & 0xFF][:ng]
+        U, qs = np.diag(m[:1 << (1 + ng)]), list(reversed([g(op.q0)] + outside))
     else:
         raise AssertionError(op.kind)
     for c in ctrl:
@@ -291,10 +297,11 @@ def test_noise_helpers():
 
 # --- tile scheduler: the slot-space program equals the op list ---------------
 
-def _apply_vec(X, op, m, tb=None, f32=False):
+def _apply_vec(X, op, m, tb=None, f32=False, gbits=None):
     """Apply one tile op (public kinds in local bits, or the scheduler's
     slot-space kinds when tb is given) to a 2^m vector.  f32: the scheduled
-    FP32 form, whose operands are stored as floats."""
+    FP32 form, whose operands are stored as floats.  gbits: the values of
+    the qubits outside the tile (GDIAG), state qubit -> bit."""
     S = 1 << m
     idx = np.arange(S)
     mm = np.array([op.m[i] for i in range(32)])
@@ -356,18 +363,24 @@ def _apply_vec(X, op, m, tb=None, f32=False):
     elif k == N.TILE_CPHASE:
         need = cm | (1 << op.q0) | (1 << op.q1)
         X = np.where((idx & need) == need, X * complex(mm[0], mm[1]), X)
+    elif k == N.TILE_GDIAG:
+        gb = gbits or {}
+        ng = op.q1 >> 16
+        sel = gb.get(op.q1 & 0xFF, 0) | ((gb.get((op.q1 >> 8) & 0xFF, 0) << 1) if ng == 2 else 0)
+        d = mm[:16].view(np.complex128)
+        X = X * d[2 * sel + ((idx >> op.q0) & 1)]
     else:
         raise AssertionError(k)
     return X
 
 
-def _emulate_slot_program(sched, m, tb, psi_tile, f32=False):
+def _emulate_slot_program(sched, m, tb, psi_tile, f32=False, gbits=None):
     """Numpy model of the register tile kernel: X[s] is the amplitude held
     in slot s (register s >> tb of thread s & (2^tb - 1)); ops are in slot
     bits.  The final X must be in the identity layout."""
     X = psi_tile.copy()
     for op in sched:
-        X = _apply_vec(X, op, m, tb, f32)
+        X = _apply_vec(X, op, m, tb, f32, gbits)
     return X
 
 
@@ -382,15 +395,23 @@ def _op(kind, q0, q1, ctrl, values):
     return op
 
 
-def _random_tile_ops(m, count, rng):
+def _random_tile_ops(m, count, rng, outside=()):
+    """Random ops of every public kind on an m-qubit tile; GDIAG ops (one
+    or two of the state qubits ``outside`` the tile) when outside is given."""
     ops = []
+    kinds = [1, 1, 2, 3, 4, 5, 5, 6, 6, 7] + ([8, 8] if len(outside) else [])
     for _ in range(count):
-        kind = int(rng.choice([1, 1, 2, 3, 4, 5, 5, 6, 6, 7]))
+        kind = int(rng.choice(kinds))
         q = [int(x) for x in rng.choice(m, 3, replace=False)]
         ctrl = 0
         if rng.random() < 0.2:
             ctrl = 1 << q[2]
-        if kind in (N.TILE_DENSE1,):
+        if kind == N.TILE_GDIAG:
+            ng = 1 if len(outside) == 1 else int(rng.integers(1, 3))
+            g = [int(x) for x in rng.choice(list(outside), ng, replace=False)] + [0]
+            ops.append(_op(kind, q[0], g[0] | (g[1] << 8 if ng == 2 else 0) | (ng << 16), 0,
+                           np.exp(1j * rng.uniform(0, 6, 2 << ng))))
+        elif kind in (N.TILE_DENSE1,):
             U = np.linalg.qr(rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2)))[0]
             ops.append(_op(kind, q[0], 0, ctrl, U.reshape(-1)))
         elif kind == N.TILE_DENSE2:
@@ -415,19 +436,24 @@ def test_tile_schedule_matches_op_list(dtype, m, fma):
     rb = min(m - 6, 4 if (dtype == "f32" and fma) else 3)
     tb = m - rb
     for trial in range(3):
-        ops = _random_tile_ops(m, 40, rng)
+        # qubits m and m + 1 stand for state qubits outside the tile, with
+        # fixed values over the block
+        ops = _random_tile_ops(m, 40, rng, outside=(m, m + 1))
+        gbits = {m: int(rng.integers(2)), m + 1: int(rng.integers(2))}
         sched = N.tile_schedule(dtype, m, ops, fma)
         v = rng.normal(size=1 << m) + 1j * rng.normal(size=1 << m)
         # reference: the op list applied to the tile as an m-qubit state
         want = v.copy()
         for op in ops:
-            want = _apply_vec(want, op, m)
+            want = _apply_vec(want, op, m, gbits=gbits)
         if m <= 9:  # the vector model against the Kronecker model of the op semantics
-            ref = v.copy()
+            base = (gbits[m] << m) | (gbits[m + 1] << (m + 1))
+            ref = np.zeros(1 << (m + 2), dtype=np.complex128)
+            ref[base:base + (1 << m)] = v
             for op in ops:
-                ref = emulate_tile_op(op, list(range(m)), m, ref)
-            assert np.max(np.abs(ref - want)) < 1e-12
-        got = _emulate_slot_program(sched, m, tb, v, dtype == "f32")
+                ref = emulate_tile_op(op, list(range(m)), m + 2, ref)
+            assert np.max(np.abs(ref[base:base + (1 << m)] - want)) < 1e-12
+        got = _emulate_slot_program(sched, m, tb, v, dtype == "f32", gbits)
         assert np.max(np.abs(got - want)) < (1e-12 if dtype == "f64" else 1e-4)
         nremap = sum(1 for o in sched if o.kind == N.TILE_REMAP)
         assert nremap <= len(ops)
