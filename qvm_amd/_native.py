@@ -38,6 +38,7 @@ TILE_GDIAG = 8
 TILE_XREG = 16
 TILE_RSWAP = 17
 TILE_REMAP = 18
+TILE_RDIAG = 19
 
 
 class QVMHipError(RuntimeError):

@@ -98,6 +98,9 @@ struct qvmhip_batch {
     double *u_dev = nullptr;
     uint64_t *out_dev = nullptr;
     uint64_t *starts_dev = nullptr;
+    // finishing pass (<H_C> and shots of every instance in four launches)
+    BatchFinishArgs fin{};
+    std::vector<void *> fin_dev;  // its device tables, freed with the batch
     std::vector<BLaunch> launches;
     std::vector<char> entries;
     void *entries_dev = nullptr;
@@ -117,6 +120,8 @@ void free_batch(qvmhip_batch *b) {
     for (void *p : {(void *)b->buf, (void *)b->red, (void *)b->chunk_sum, (void *)b->chunk_pre, (void *)b->scan_tmp,
                     b->mat, b->blob_dev, (void *)b->edges_dev, (void *)b->exp_dev, (void *)b->u_dev,
                     (void *)b->out_dev, (void *)b->starts_dev, b->entries_dev, (void *)b->blkmap_dev})
+        if (p) hipFree(p);
+    for (void *p : b->fin_dev)
         if (p) hipFree(p);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
@@ -159,17 +164,65 @@ int enqueue(qvmhip_batch *b) {
             rc = launch_gate(&b->views[L.inst], L.item->U.data(), L.item->k, L.item->qubits, 0);
         if (rc) return rc;
     }
+    if (b->fin.zz_part || b->fin.shots) return launch_batch_finish(b->stream, b->dtype, b->buf, b->fin);
+    return QVMHIP_OK;
+}
+
+// Device tables of the finishing pass; called once by finalize.
+int build_finish(qvmhip_batch *b) {
+    BatchFinishArgs &f = b->fin;
+    f.count = b->count;
+    std::vector<uint64_t> len(b->count), chunk_base(b->count + 1, 0), blk_base(b->count + 1, 0);
+    std::vector<uint32_t> blk_inst, edge_off(b->count), nedges(b->count);
+    bool any_edges = false;
     for (uint32_t i = 0; i < b->count; ++i) {
-        if (!b->edges[i].empty()) {
-            int rc = expectation_zz_to_device(&b->views[i], b->edges_dev + b->edge_off[i] / sizeof(uint32_t),
-                                              (uint32_t)(b->edges[i].size() / 2), b->exp_dev + i);
-            if (rc) return rc;
-        }
-        if (b->shots) {
-            int rc = sample_to_device(&b->views[i], b->u_dev + (size_t)i * b->shots, b->shots,
-                                      b->out_dev + (size_t)i * b->shots, 1);
-            if (rc) return rc;
-        }
+        len[i] = uint64_t(1) << b->nq[i];
+        chunk_base[i + 1] = chunk_base[i] + std::max<uint64_t>(1, len[i] >> kChunkLog2);
+        const uint64_t nb = batch_finish_blocks(len[i]);
+        blk_base[i + 1] = blk_base[i] + nb;
+        blk_inst.insert(blk_inst.end(), (size_t)nb, i);
+        edge_off[i] = (uint32_t)(b->edge_off[i] / sizeof(uint32_t));
+        nedges[i] = (uint32_t)(b->edges[i].size() / 2);
+        any_edges = any_edges || nedges[i] != 0;
+    }
+    f.blocks = blk_base[b->count];
+    if (!any_edges && !b->shots) return QVMHIP_OK;
+    auto up = [&](const void *host, size_t bytes, void **dev) -> int {
+        QH_CHECK(hipMalloc(dev, std::max<size_t>(bytes, 8)));
+        b->fin_dev.push_back(*dev);
+        if (host) QH_CHECK(hipMemcpy(*dev, host, bytes, hipMemcpyHostToDevice));
+        return QVMHIP_OK;
+    };
+    void *d;
+    int rc;
+    if ((rc = up(b->offset.data(), 8 * b->count, &d))) return rc;
+    f.inst_off = (const uint64_t *)d;
+    if ((rc = up(len.data(), 8 * b->count, &d))) return rc;
+    f.inst_len = (const uint64_t *)d;
+    if ((rc = up(chunk_base.data(), 8 * chunk_base.size(), &d))) return rc;
+    f.chunk_base = (const uint64_t *)d;
+    if ((rc = up(blk_base.data(), 8 * blk_base.size(), &d))) return rc;
+    f.blk_base = (const uint64_t *)d;
+    if ((rc = up(blk_inst.data(), 4 * blk_inst.size(), &d))) return rc;
+    f.blk_inst = (const uint32_t *)d;
+    if ((rc = up(edge_off.data(), 4 * edge_off.size(), &d))) return rc;
+    f.edge_off = (const uint32_t *)d;
+    if ((rc = up(nedges.data(), 4 * nedges.size(), &d))) return rc;
+    f.nedges = (const uint32_t *)d;
+    f.edges = b->edges_dev;
+    if (any_edges) {
+        if ((rc = up(nullptr, 8 * f.blocks, &d))) return rc;
+        f.zz_part = (double *)d;
+        f.exp_out = b->exp_dev;
+    }
+    if (b->shots) {
+        if ((rc = up(nullptr, 8 * chunk_base[b->count], &d))) return rc;
+        f.chunk_sum = (double *)d;
+        if ((rc = up(nullptr, 8 * chunk_base[b->count], &d))) return rc;
+        f.chunk_pre = (double *)d;
+        f.u = b->u_dev;
+        f.shots = b->shots;
+        f.shot_out = b->out_dev;
     }
     return QVMHIP_OK;
 }
@@ -340,6 +393,10 @@ int qvmhip_batch_finalize(qvmhip_batch *b, int use_graph) {
         QH_CHECK(hipMalloc((void **)&b->u_dev, b->u_host.size() * sizeof(double)));
         QH_CHECK(hipMemcpy(b->u_dev, b->u_host.data(), b->u_host.size() * sizeof(double), hipMemcpyHostToDevice));
         QH_CHECK(hipMalloc((void **)&b->out_dev, b->u_host.size() * sizeof(uint64_t)));
+    }
+    {
+        int rc = build_finish(b);
+        if (rc) return rc;
     }
     // levels: the L-th item of every instance; tile passes grouped by width
     size_t depth = 0;

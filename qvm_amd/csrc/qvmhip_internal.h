@@ -187,6 +187,24 @@ int convert_f32_to_f64(qvmhip_state *s, uint64_t start, uint64_t count, double *
 int convert_f64_to_f32(qvmhip_state *s, uint64_t start, uint64_t count, const double *dev_in);
 // Capturable (kernel launches only) forms used by batches and programs.
 int expectation_zz_to_device(qvmhip_state *s, const uint32_t *edges_dev, uint32_t nedges, double *out_dev);
+// Device tables of a batch's finishing pass (all instances in one set of
+// launches): per instance its amplitude offset and length, the prefix over
+// instances of sampling chunks (count + 1 entries) and of work blocks
+// (count + 1), its edge list (offset into `edges`, count); per work block
+// its instance.  zz_part / exp_out are null without MaxCut edges, the
+// chunk arrays / u / shot_out without shots.
+struct BatchFinishArgs {
+    uint32_t count;
+    uint64_t blocks;
+    const uint64_t *inst_off, *inst_len, *chunk_base, *blk_base;
+    const uint32_t *blk_inst, *edges, *edge_off, *nedges;
+    double *zz_part, *exp_out, *chunk_sum, *chunk_pre;
+    const double *u;
+    uint64_t shots;
+    uint64_t *shot_out;
+};
+uint64_t batch_finish_blocks(uint64_t len);  // work blocks of one instance
+int launch_batch_finish(hipStream_t st, int dtype, const void *buf, const BatchFinishArgs &a);
 int sample_to_device(qvmhip_state *s, const double *u_dev, uint64_t nshots, uint64_t *out_dev, int rule_ge);
 // Tile passes: validation of a public op list (tile-local bit positions) and
 // its translation into the slot-space program the tile kernels run.

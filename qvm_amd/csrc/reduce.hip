@@ -332,6 +332,50 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const double *__restrict_
     }
 }
 
+// The index a draw p selects (rule_ge: first index with cdf >= p, else
+// first with cdf > p; len - 1 when none): the last chunk whose exclusive
+// prefix is below p by binary search, then the lanes' in-chunk prefix.
+// Called by a whole wavefront with the same p; the result is valid in
+// every lane.
+template <typename T>
+__device__ inline uint64_t sample_one(const vec2_t<T> *__restrict__ psi, uint64_t len, uint64_t nchunks,
+                                      const double *__restrict__ pre, double p, int rule_ge, int lane) {
+    uint64_t lo = 0, hi = nchunks;  // invariant: answer in [lo, hi)
+    while (hi - lo > 1) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        const double v = pre[mid];
+        const bool ok = rule_ge ? (v < p) : (v <= p);
+        if (ok)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    for (uint64_t c = lo; c < nchunks; ++c) {
+        double pr[4];
+        chunk_lane_probs<T>(psi, len, c, lane, pr);
+        const double local = ((pr[0] + pr[1]) + pr[2]) + pr[3];
+        const double incl = wave_inclusive_scan(local, lane);
+        const double up = __shfl_up(incl, 1, kWave);
+        double run = lane ? up : 0.0;  // exclusive lane offset
+        int hit = -1;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            run += pr[k];
+            const double C = pre[c] + run;
+            const bool ok = rule_ge ? (C >= p) : (C > p);
+            const uint64_t idx = (c << kChunkLog2) + (uint64_t)lane * 4 + k;
+            if (ok && hit < 0 && idx < len) hit = k;
+        }
+        const unsigned long long mask = __ballot(hit >= 0);
+        if (mask) {
+            const int first = __ffsll((long long)mask) - 1;
+            const int k = __shfl(hit, first, kWave);
+            return (c << kChunkLog2) + (uint64_t)first * 4 + (uint64_t)k;
+        }
+    }
+    return len - 1;
+}
+
 // One wavefront per draw.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_sample_search(const vec2_t<T> *__restrict__ psi,
@@ -344,45 +388,122 @@ __global__ __launch_bounds__(kBlock) void k_sample_search(const vec2_t<T> *__res
     const uint64_t sstep = (uint64_t)gridDim.x * (kBlock / kWave);
     for (uint64_t shot = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; shot < nshots;
          shot += sstep) {
-        const double p = u[shot];
-        // last chunk c with pre[c] <= p (rule >) or pre[c] < p (rule >=); 0 if none
-        uint64_t lo = 0, hi = nchunks;  // invariant: answer in [lo, hi)
-        while (hi - lo > 1) {
-            const uint64_t mid = lo + (hi - lo) / 2;
-            const double v = pre[mid];
-            const bool ok = rule_ge ? (v < p) : (v <= p);
-            if (ok)
-                lo = mid;
-            else
-                hi = mid;
-        }
-        uint64_t result = len - 1;
-        bool found = false;
-        for (uint64_t c = lo; c < nchunks && !found; ++c) {
-            double pr[4];
-            chunk_lane_probs<T>(psi, len, c, lane, pr);
-            const double local = ((pr[0] + pr[1]) + pr[2]) + pr[3];
-            const double incl = wave_inclusive_scan(local, lane);
-            const double up = __shfl_up(incl, 1, kWave);
-            double run = lane ? up : 0.0;  // exclusive lane offset
-            int hit = -1;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                run += pr[k];
-                const double C = pre[c] + run;
-                const bool ok = rule_ge ? (C >= p) : (C > p);
-                const uint64_t idx = (c << kChunkLog2) + (uint64_t)lane * 4 + k;
-                if (ok && hit < 0 && idx < len) hit = k;
-            }
-            const unsigned long long mask = __ballot(hit >= 0);
-            if (mask) {
-                const int first = __ffsll((long long)mask) - 1;
-                const int k = __shfl(hit, first, kWave);
-                result = (c << kChunkLog2) + (uint64_t)first * 4 + (uint64_t)k;
-                found = true;
-            }
-        }
+        const uint64_t result = sample_one<T>(psi, len, nchunks, pre, u[shot], rule_ge, lane);
         if (lane == 0) out[shot] = result;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched finish: every instance of a batch buffer at once.  A work block
+// covers kFinishAmps amplitudes (16 sampling chunks) of one instance: each
+// wave builds the chunk sums of four chunks and, reading the same
+// probabilities, accumulates sum_x |psi_x|^2 (|E| - 2 cut(x)), the MaxCut
+// <H_C> partial of the block.
+
+constexpr uint64_t kFinishAmps = uint64_t(kBlock) * 16;
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bfinish_partial(const vec2_t<T> *__restrict__ buf,
+                                                            const BatchFinishArgs a) {
+    __shared__ double sm[kBlock / kWave];
+    __shared__ uint32_t se[512];
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    for (uint64_t b = blockIdx.x; b < a.blocks; b += gridDim.x) {
+        const uint32_t i = a.blk_inst[b];
+        const uint64_t lb = b - a.blk_base[i];
+        const uint32_t ne = a.nedges[i];
+        __syncthreads();  // the previous block's edge and partial reads are done
+        for (uint32_t k = threadIdx.x; k < 2 * ne; k += kBlock) se[k] = a.edges[a.edge_off[i] + k];
+        __syncthreads();
+        const vec2_t<T> *psi = buf + a.inst_off[i];
+        const uint64_t len = a.inst_len[i];
+        const uint64_t nch = a.chunk_base[i + 1] - a.chunk_base[i];
+        double acc = 0.0;
+#pragma unroll 1
+        for (int k4 = 0; k4 < 4; ++k4) {
+            const uint64_t c = lb * 16 + (uint64_t)wid * 4 + k4;
+            if (c >= nch) break;  // uniform per wave
+            double p[4];
+            chunk_lane_probs<T>(psi, len, c, lane, p);
+            if (a.chunk_sum) {
+                const double incl = wave_inclusive_scan(((p[0] + p[1]) + p[2]) + p[3], lane);
+                if (lane == kWave - 1) a.chunk_sum[a.chunk_base[i] + c] = incl;
+            }
+            if (ne) {
+                const uint64_t x0 = (c << kChunkLog2) + (uint64_t)lane * 4;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint64_t x = x0 + k;
+                    int cut = 0;
+                    for (uint32_t e = 0; e < ne; ++e) cut += (int)(((x >> se[2 * e]) ^ (x >> se[2 * e + 1])) & 1u);
+                    acc += p[k] * (double)((int)ne - 2 * cut);
+                }
+            }
+        }
+        if (a.zz_part) {
+            const double t = block_sum(acc, sm);
+            if (threadIdx.x == 0) a.zz_part[b] = t;
+        }
+    }
+}
+
+// One block per instance: its <H_C> from its block partials, in order.
+__global__ __launch_bounds__(kBlock) void k_bfinish_zz(const BatchFinishArgs a) {
+    __shared__ double sm[kBlock / kWave];
+    const uint32_t i = blockIdx.x;
+    if (a.nedges[i] == 0) return;  // uniform: the whole block leaves
+    double acc = 0.0;
+    for (uint64_t j = a.blk_base[i] + threadIdx.x; j < a.blk_base[i + 1]; j += kBlock) acc += a.zz_part[j];
+    const double t = block_sum(acc, sm);
+    if (threadIdx.x == 0) a.exp_out[i] = t;
+}
+
+// One block per instance: exclusive prefix of its chunk sums, 1024 at a
+// time with a running carry.
+__global__ __launch_bounds__(kBlock) void k_bfinish_scan(const BatchFinishArgs a) {
+    __shared__ double sm[kBlock / kWave];
+    const uint32_t i = blockIdx.x;
+    const uint64_t base = a.chunk_base[i], n = a.chunk_base[i + 1] - base;
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    double carry = 0.0;
+    for (uint64_t off = 0; off < n; off += 4 * kBlock) {
+        const uint64_t j = off + (uint64_t)threadIdx.x * 4;
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = (j + u < n) ? a.chunk_sum[base + j + u] : 0.0;
+        const double incl = wave_inclusive_scan(((v[0] + v[1]) + v[2]) + v[3], lane);
+        const double up = __shfl_up(incl, 1, kWave);
+        if (lane == kWave - 1) sm[wid] = incl;
+        __syncthreads();
+        double woff = 0.0, total = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) {
+            if (w < wid) woff += sm[w];
+            total += sm[w];
+        }
+        double run = carry + (woff + (lane ? up : 0.0));
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (j + u < n) a.chunk_pre[base + j + u] = run;
+            run += v[u];
+        }
+        carry += total;
+        __syncthreads();  // sm is rewritten by the next round
+    }
+}
+
+// One wavefront per (instance, shot), rule cdf >= p.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bfinish_sample(const vec2_t<T> *__restrict__ buf,
+                                                           const BatchFinishArgs a) {
+    const int lane = threadIdx.x % kWave;
+    const uint64_t total = (uint64_t)a.count * a.shots;
+    const uint64_t sstep = (uint64_t)gridDim.x * (kBlock / kWave);
+    for (uint64_t s = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; s < total; s += sstep) {
+        const uint32_t i = (uint32_t)(s / a.shots);
+        const uint64_t base = a.chunk_base[i];
+        const uint64_t r = sample_one<T>(buf + a.inst_off[i], a.inst_len[i], a.chunk_base[i + 1] - base,
+                                         a.chunk_pre + base, a.u[s], 1, lane);
+        if (lane == 0) a.shot_out[s] = r;
     }
 }
 
@@ -841,6 +962,34 @@ int density_diagonal(qvmhip_state *s, uint32_t nq, double *host_out) {
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     hipFree(dev);
     if (e != hipSuccess) return fail(QVMHIP_EDEVICE, std::string("density diagonal: ") + hipGetErrorString(e));
+    return QVMHIP_OK;
+}
+
+uint64_t batch_finish_blocks(uint64_t len) { return (len + kFinishAmps - 1) / kFinishAmps; }
+
+int launch_batch_finish(hipStream_t st, int dtype, const void *buf, const BatchFinishArgs &a) {
+    if (a.blocks == 0) return QVMHIP_OK;
+    const uint32_t g1 = (uint32_t)std::min<uint64_t>(a.blocks, kMaxGrid);
+    if (dtype == QVMHIP_F64)
+        hipLaunchKernelGGL(k_bfinish_partial<double>, dim3(g1), dim3(kBlock), 0, st, (const double2 *)buf, a);
+    else
+        hipLaunchKernelGGL(k_bfinish_partial<float>, dim3(g1), dim3(kBlock), 0, st, (const float2 *)buf, a);
+    QH_LAUNCH_CHECK();
+    if (a.zz_part) {
+        hipLaunchKernelGGL(k_bfinish_zz, dim3(a.count), dim3(kBlock), 0, st, a);
+        QH_LAUNCH_CHECK();
+    }
+    if (a.shots) {
+        hipLaunchKernelGGL(k_bfinish_scan, dim3(a.count), dim3(kBlock), 0, st, a);
+        QH_LAUNCH_CHECK();
+        const uint64_t wpb = kBlock / kWave;
+        const uint32_t g = (uint32_t)std::min<uint64_t>(((uint64_t)a.count * a.shots + wpb - 1) / wpb, kMaxGrid);
+        if (dtype == QVMHIP_F64)
+            hipLaunchKernelGGL(k_bfinish_sample<double>, dim3(g), dim3(kBlock), 0, st, (const double2 *)buf, a);
+        else
+            hipLaunchKernelGGL(k_bfinish_sample<float>, dim3(g), dim3(kBlock), 0, st, (const float2 *)buf, a);
+        QH_LAUNCH_CHECK();
+    }
     return QVMHIP_OK;
 }
 

@@ -39,6 +39,7 @@ namespace qvmhip {
 constexpr uint32_t kTileXReg = 16;   // X on register bit (q0 - TB) where ctrl_mask holds
 constexpr uint32_t kTileRSwap = 17;  // exchange register bits q0 and q1 (both < RB)
 constexpr uint32_t kTileRemap = 18;  // relayout through LDS; new slot bit j <- old bit src[j]
+constexpr uint32_t kTileRDiag = 19;  // multiply register r by entry r (m holds R complex entries)
 
 // Register bits per thread (measured on MI355X): 3 (8 amplitudes) for FP64
 // and for exact FP32, 4 (16) for FP32 with FMA contraction; fewer for small
@@ -254,10 +255,44 @@ __device__ __forceinline__ void pick4(const qvmhip_tile_op &op, uint32_t j, T &d
     di = (j & 2u) ? ib : ia;
 }
 
+// Register r takes entry (c1) when its bit RS is set, (c0) otherwise.
+template <typename T, int R, int RS, bool CTRL, bool FMA>
+__device__ __forceinline__ void diag_rbit(vec2_t<T> (&x)[R], T c0r, T c0i, T c1r, T c1i, bool lane_ok,
+                                          uint32_t rcm) {
+    if constexpr ((1 << RS) < R) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const bool one = ((r >> RS) & 1) != 0;
+            vec2_t<T> y;
+            cmul<T, FMA>(x[r].x, x[r].y, one ? c1r : c0r, one ? c1i : c0i, y.x, y.y);
+            if constexpr (CTRL)
+                x[r] = vsel<T>(lane_ok && ((uint32_t)r & rcm) == rcm, y, x[r]);
+            else
+                x[r] = y;
+        }
+    }
+}
+
+// Register r times entry r: a diagonal whose index bits (and controls) are
+// all register bits, with its per-register entries worked out by the
+// scheduler.
+template <typename T, int R, bool FMA>
+__device__ __forceinline__ void reg_rdiag(vec2_t<T> (&x)[R], const qvmhip_tile_op &op) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        vec2_t<T> y;
+        cmul<T, FMA>(x[r].x, x[r].y, mval<T>(op, 2 * r), mval<T>(op, 2 * r + 1), y.x, y.y);
+        x[r] = y;
+    }
+}
+
 // diagonal 1q / 2q and CPHASE on any slot bits.  The diagonal index has a
 // lane part (target bits that are thread bits, the same for all registers)
-// and a register part (uniform per register), so the per-lane entry is
-// selected once per op and the per-register choice is scalar.
+// and a register part (uniform per register).  Three cases keep the
+// per-register choice free of vector selects: all index bits on registers
+// (scalar entries), all on lanes (one entry per lane), one of each (two
+// entries per lane, the register bit's position made a compile-time
+// constant by a switch).
 template <typename T, int R, int TB, bool CTRL, bool FMA>
 __device__ __forceinline__ void reg_diag(vec2_t<T> (&x)[R], const qvmhip_tile_op &op, uint32_t t, bool lane_ok,
                                          uint32_t rcm) {
@@ -293,8 +328,10 @@ __device__ __forceinline__ void reg_diag(vec2_t<T> (&x)[R], const qvmhip_tile_op
     }
     const bool two = kind == QVMHIP_TILE_DIAG2;
     const bool reg0 = q0 >= TB, reg1 = two && q1 >= TB;
-    if (reg0 && (!two || reg1) && (op.ctrl_mask & lanes) == 0) {
-        // index and control bits are all register bits: uniform entries
+    if (reg0 && (!two || reg1)) {
+        // index bits are all register bits (the scheduler turns such ops
+        // into register diagonals unless a control bit is a lane bit): the
+        // entry of each register is uniform
         const uint32_t rs0 = q0 - TB, rs1 = two ? q1 - TB : 0u;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -304,35 +341,45 @@ __device__ __forceinline__ void reg_diag(vec2_t<T> (&x)[R], const qvmhip_tile_op
                 pick4<T>(op, jr, dr, di);
                 V y;
                 cmul<T, FMA>(x[r].x, x[r].y, dr, di, y.x, y.y);
-                x[r] = y;
+                if constexpr (CTRL)
+                    x[r] = vsel<T>(lane_ok, y, x[r]);
+                else
+                    x[r] = y;
             }
         }
         return;
     }
     // lane part of the index
     const uint32_t jl = (reg0 ? 0u : ((t >> q0) & 1u)) | ((two && !reg1) ? (((t >> q1) & 1u) << 1) : 0u);
-    // register part: bit rs0 of r has weight w0, bit rs1 weight 2 (none: shift
-    // past the register bits)
-    const uint32_t rsa = reg0 ? q0 - TB : 31u, wa = 1u;
-    const uint32_t rsb = reg1 ? q1 - TB : 31u, wb = 2u;
-    // the up to four per-lane candidates, indexed by the register part
-    T cr[4], ci[4];
+    if (!reg0 && !reg1) {
+        // index bits are all lane bits: one entry per lane for every register
+        T dr, di;
+        pick4<T>(op, jl, dr, di);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) pick4<T>(op, jl | (uint32_t)k, cr[k], ci[k]);
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const uint32_t jr = (((uint32_t)r >> rsa) & 1u) * wa | (((uint32_t)r >> rsb) & 1u) * wb;
-        // jr is uniform: a scalar choice among the per-lane candidates
-        const T dr = jr == 0 ? cr[0] : jr == 1 ? cr[1] : jr == 2 ? cr[2] : cr[3];
-        const T di = jr == 0 ? ci[0] : jr == 1 ? ci[1] : jr == 2 ? ci[2] : ci[3];
-        V y;
-        cmul<T, FMA>(x[r].x, x[r].y, dr, di, y.x, y.y);
-        if constexpr (CTRL) {
-            const bool ok = lane_ok && ((uint32_t)r & rcm) == rcm;
-            x[r] = vsel<T>(ok, y, x[r]);
-        } else {
-            x[r] = y;
+        for (int r = 0; r < R; ++r) {
+            V y;
+            cmul<T, FMA>(x[r].x, x[r].y, dr, di, y.x, y.y);
+            if constexpr (CTRL)
+                x[r] = vsel<T>(lane_ok && ((uint32_t)r & rcm) == rcm, y, x[r]);
+            else
+                x[r] = y;
         }
+        return;
+    }
+    // one lane bit and one register bit: two entries per lane, and the
+    // register bit (a compile-time position below) chooses between them
+    const uint32_t rs = reg0 ? q0 - TB : q1 - TB;
+    const uint32_t w = reg0 ? 1u : 2u;
+    T c0r, c0i, c1r, c1i;
+    pick4<T>(op, jl, c0r, c0i);
+    pick4<T>(op, jl | w, c1r, c1i);
+    switch (rs) {
+        case 0: diag_rbit<T, R, 0, CTRL, FMA>(x, c0r, c0i, c1r, c1i, lane_ok, rcm); break;
+        case 1: diag_rbit<T, R, 1, CTRL, FMA>(x, c0r, c0i, c1r, c1i, lane_ok, rcm); break;
+        case 2: diag_rbit<T, R, 2, CTRL, FMA>(x, c0r, c0i, c1r, c1i, lane_ok, rcm); break;
+        case 3: diag_rbit<T, R, 3, CTRL, FMA>(x, c0r, c0i, c1r, c1i, lane_ok, rcm); break;
+        case 4: diag_rbit<T, R, 4, CTRL, FMA>(x, c0r, c0i, c1r, c1i, lane_ok, rcm); break;
+        default: break;
     }
 }
 
@@ -477,6 +524,9 @@ __device__ inline void tile_ops(vec2_t<T> (&x)[1 << RB], const qvmhip_tile_op *_
             case kTileRemap:
                 reg_remap<T, R, TB>(x, sa, op, t);
                 break;
+            case kTileRDiag:
+                reg_rdiag<T, R, FMA>(x, op);
+                break;
             case QVMHIP_TILE_GDIAG: {
                 const uint32_t d = op.q1;
                 uint32_t sel = (uint32_t)(gb >> (d & 63u)) & 1u;
@@ -598,6 +648,42 @@ int check_tile_globals(uint32_t n, uint32_t m, const uint32_t *tq, uint32_t nops
 }
 
 namespace {
+
+// A scheduled DIAG1 / DIAG2 / CPHASE whose index bits and controls are all
+// register bits becomes a register diagonal: entry r is the factor register
+// r takes (1 where the controls are not met).
+bool to_register_diagonal(qvmhip_tile_op &o, int tb, int rb) {
+    if (o.kind != QVMHIP_TILE_DIAG1 && o.kind != QVMHIP_TILE_DIAG2 && o.kind != QVMHIP_TILE_CPHASE) return false;
+    const bool two = o.kind != QVMHIP_TILE_DIAG1;
+    const uint32_t lanes = (1u << tb) - 1u;
+    if ((int)o.q0 < tb || (two && (int)o.q1 < tb) || (o.ctrl_mask & lanes)) return false;
+    const uint32_t rcm = o.ctrl_mask >> tb, b0 = o.q0 - tb, b1 = two ? o.q1 - tb : 0u;
+    double f[32];
+    for (int r = 0; r < (1 << rb); ++r) {
+        double re = 1.0, im = 0.0;
+        const uint32_t ur = (uint32_t)r;
+        if ((ur & rcm) == rcm) {
+            const uint32_t j0 = (ur >> b0) & 1u, j1 = two ? (ur >> b1) & 1u : 0u;
+            if (o.kind == QVMHIP_TILE_CPHASE) {
+                if (j0 && j1) {
+                    re = o.m[0];
+                    im = o.m[1];
+                }
+            } else {
+                const uint32_t j = j0 | (j1 << 1);
+                re = o.m[2 * j];
+                im = o.m[2 * j + 1];
+            }
+        }
+        f[2 * r] = re;
+        f[2 * r + 1] = im;
+    }
+    o.kind = kTileRDiag;
+    o.q0 = o.q1 = o.ctrl_mask = 0;
+    std::memset(o.m, 0, sizeof(o.m));
+    std::memcpy(o.m, f, sizeof(double) * 2 * (1 << rb));
+    return true;
+}
 
 // Local bits an op needs on register bits (its non-diagonal targets).
 uint32_t reg_need(const qvmhip_tile_op &op) {
@@ -775,6 +861,20 @@ void schedule_tile_ops(int dtype, bool fma, uint32_t m, const qvmhip_tile_op *in
             o.kind = kTileXReg;
             o.ctrl_mask |= 1u << L.l2s[op.q1];
             o.q1 = 0;
+        }
+        if (to_register_diagonal(o, L.tb, L.rb)) {
+            // consecutive register diagonals multiply into one when the
+            // pass contracts with FMAs (no promise of gate-by-gate rounding)
+            if (fma && !out.empty() && out.back().kind == kTileRDiag) {
+                qvmhip_tile_op &prev = out.back();
+                for (int r = 0; r < (1 << L.rb); ++r) {
+                    const double ar = prev.m[2 * r], ai = prev.m[2 * r + 1];
+                    const double br = o.m[2 * r], bi = o.m[2 * r + 1];
+                    prev.m[2 * r] = ar * br - ai * bi;
+                    prev.m[2 * r + 1] = ar * bi + ai * br;
+                }
+                continue;
+            }
         }
         out.push_back(o);
     }

@@ -316,6 +316,9 @@ def _apply_vec(X, op, m, tb=None, f32=False, gbits=None):
         for j in range(m):
             old |= ((idx >> j) & 1) << int(src[j])
         return X[old]
+    if k == N.TILE_RDIAG:  # slot s is in register s >> tb
+        f = mm[:32].view(np.complex128) if not f32 else mm[:32].reshape(-1, 2) @ np.array([1, 1j])
+        return X * f[idx >> tb]
     if k == N.TILE_RSWAP:
         A, B = tb + op.q0, tb + op.q1
         sw = idx.copy()
