@@ -100,6 +100,7 @@ struct qvmhip_batch {
     uint64_t *starts_dev = nullptr;
     // finishing pass (<H_C> and shots of every instance in four launches)
     BatchFinishArgs fin{};
+    bool first_pass_resets = false;  // every program starts with a tile pass (no reset sweep)
     std::vector<void *> fin_dev;  // its device tables, freed with the batch
     std::vector<BLaunch> launches;
     std::vector<char> entries;
@@ -144,7 +145,9 @@ size_t push(std::vector<char> &blob, const void *p, size_t bytes) {
 // Enqueue the whole batch on its stream (kernel launches only, capturable).
 int enqueue(qvmhip_batch *b) {
     const uint32_t rg = (uint32_t)std::min<uint64_t>((b->total + 255) / 256, 256 * 64);
-    if (b->dtype == QVMHIP_F64) {
+    if (b->first_pass_resets) {
+        // every instance starts with a tile pass that builds |0...0> itself
+    } else if (b->dtype == QVMHIP_F64) {
         hipLaunchKernelGGL(k_batch_reset<double>, dim3(rg), dim3(256), 0, b->stream, (double2 *)b->buf, b->total);
         hipLaunchKernelGGL(k_batch_ones<double>, dim3((b->count + 255) / 256), dim3(256), 0, b->stream,
                            (double2 *)b->buf, b->starts_dev, b->count);
@@ -398,6 +401,11 @@ int qvmhip_batch_finalize(qvmhip_batch *b, int use_graph) {
         int rc = build_finish(b);
         if (rc) return rc;
     }
+    // when every instance's first item is a tile pass, that pass builds the
+    // initial |0...0> in registers and the reset sweep of the buffer goes
+    b->first_pass_resets = true;
+    for (uint32_t i = 0; i < b->count; ++i)
+        if (b->prog[i].empty() || b->prog[i][0].kind != kItemTile) b->first_pass_resets = false;
     // levels: the L-th item of every instance; tile passes grouped by width
     size_t depth = 0;
     for (auto &p : b->prog) depth = std::max(depth, p.size());
@@ -424,7 +432,7 @@ int qvmhip_batch_finalize(qvmhip_batch *b, int use_graph) {
                 const uint64_t nb = build_tile_entry(
                     ent.data(), b->dtype, (b->flags & QVMHIP_FLAG_FMA_TILES) != 0, b->nq[i], it.k, it.qubits,
                     b->offset[i], blocks, it.op_off, it.nops,
-                    kTilesPerBlock);
+                    kTilesPerBlock, L == 0 && b->first_pass_resets);
                 b->entries.insert(b->entries.end(), ent.begin(), ent.end());
                 b->blkmap.insert(b->blkmap.end(), (size_t)nb, local);
                 blocks += nb;

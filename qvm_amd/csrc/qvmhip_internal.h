@@ -218,7 +218,7 @@ size_t tile_entry_bytes();
 // tiles_per_block tiles.
 uint64_t build_tile_entry(void *entry, int dtype, bool fma, uint32_t n, uint32_t m, const uint32_t *tq,
                           uint64_t amp_offset, uint64_t block_start, uint64_t op_off, uint32_t nops,
-                          uint32_t tiles_per_block);
+                          uint32_t tiles_per_block, bool from_zero);
 // op_buf: the device buffer the entries' op offsets point into
 int launch_tile_batch(hipStream_t st, int dtype, bool fma, uint32_t m, void *base, const void *entries_dev,
                       const uint32_t *blk_ent_dev, uint64_t total_blocks, uint32_t tiles_per_block,

@@ -577,7 +577,7 @@ struct TileEntry {
     uint64_t block_start;  // first work block of the launch that works on this entry
     uint64_t op_off;       // byte offset of the entry's scheduled ops in the op buffer
     uint32_t nops;
-    uint32_t pad;
+    uint32_t from_zero;    // 1: the state is |0...0>, built in registers instead of loaded
 };
 
 template <typename T, int M, int RB, bool FMA>
@@ -597,7 +597,16 @@ __global__ __launch_bounds__(1 << (M - RB)) void k_tile_batch(vec2_t<T> *__restr
         for (uint64_t tile = first; tile < last; ++tile) {
             V x[1 << RB];
             const uint64_t gb = group_base(e.args.g, tile);
-            tile_load<T, M, RB>(psi, e.args, gb, toff, x);
+            if (e.from_zero) {
+                // the first pass of a batch run: |0...0> needs no read
+#pragma unroll
+                for (int r = 0; r < (1 << RB); ++r) {
+                    x[r].x = (gb + toff + e.args.roff[r] == 0) ? T(1) : T(0);
+                    x[r].y = T(0);
+                }
+            } else {
+                tile_load<T, M, RB>(psi, e.args, gb, toff, x);
+            }
             tile_ops<T, M, RB, FMA>(x, (const qvmhip_tile_op *)(op_buf + e.op_off), e.nops, sa, gb);
             tile_store<T, M, RB>(psi, e.args, gb, toff, x);
         }
@@ -992,7 +1001,7 @@ size_t tile_entry_bytes() { return sizeof(TileEntry); }
 
 uint64_t build_tile_entry(void *entry, int dtype, bool fma, uint32_t n, uint32_t m, const uint32_t *tq,
                           uint64_t amp_offset, uint64_t block_start, uint64_t op_off, uint32_t nops,
-                          uint32_t tiles_per_block) {
+                          uint32_t tiles_per_block, bool from_zero) {
     TileEntry e;
     std::memset(&e, 0, sizeof(e));
     make_tile_args(n, m, tq, (int)m - tile_reg_bits((int)m, dtype == QVMHIP_F64, fma), e.args);
@@ -1000,6 +1009,7 @@ uint64_t build_tile_entry(void *entry, int dtype, bool fma, uint32_t n, uint32_t
     e.block_start = block_start;
     e.op_off = op_off;
     e.nops = nops;
+    e.from_zero = from_zero ? 1u : 0u;
     std::memcpy(entry, &e, sizeof(e));
     return (e.args.g.ngroups + tiles_per_block - 1) / tiles_per_block;
 }

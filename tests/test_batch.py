@@ -64,3 +64,29 @@ def test_qaoa_batch_f32_and_eager(oracle):
         assert abs(h64[i] - zz_expectation(ref, edges)) < 1e-12
     b64.close()
     b32.close()
+
+
+@pytest.mark.parametrize("gate_first", [False, True])
+def test_batch_initial_state_each_run(oracle, gate_first):
+    """Every run starts from |0...0>: through the first tile pass when every
+    program begins with one (no reset sweep), through the reset sweep when
+    one program begins with a standalone gate (a program of 3-qubit gates
+    only, which the planner keeps out of tile passes)."""
+    from qvm_amd.batch import Batch
+    rng = np.random.default_rng(8)
+    ns = [9, 12, 10]
+    circuits = []
+    for n in ns:
+        gates = CI.layered(n, 2, seed=int(rng.integers(100)))
+        circuits.append(gates)
+    if gate_first:
+        circuits[1] = [(np.linalg.qr(rng.normal(size=(8, 8)) + 1j * rng.normal(size=(8, 8)))[0], qs)
+                       for qs in ([0, 5, 11], [3, 1, 7], [11, 2, 9])]
+    b = Batch(circuits, ns, "f64", use_graph=True)
+    for run in range(3):
+        b.run()
+        b.sync()
+        for i, n in enumerate(ns):
+            ref = oracle_state(oracle, n, circuits[i])
+            assert np.max(np.abs(b.amplitudes(i) - ref)) < 1e-13, (run, i)
+    b.close()
