@@ -112,6 +112,72 @@ __global__ __launch_bounds__(kBlock) void k_dense(vec2_t<T> *__restrict__ psi,
     }
 }
 
+// Uncontrolled 1-qubit dense operators on qubit q, one amplitude per thread
+// with non-temporal accesses, so that each wavefront reads and writes one
+// contiguous run: on MI355X these in-place sweeps run at 6.6-6.8 TB/s where
+// the pair-per-thread form of k_dense reaches 5.5-5.9 TB/s for q < 8.  The
+// output of the amplitude with bit q = r is row r of the matrix applied to
+// (lower member, upper member), with k_dense's products and sums in k_dense's
+// order, so the results are bit-identical.
+template <typename T>
+struct Dense1Coef {
+    T ur[4], ui[4];  // row-major 2x2
+};
+
+template <typename T>
+__device__ __forceinline__ vec2_t<T> dense1_row(const vec2_t<T> &lo, const vec2_t<T> &hi, uint32_t r,
+                                                const Dense1Coef<T> &u) {
+    const T u0r = r ? u.ur[2] : u.ur[0], u0i = r ? u.ui[2] : u.ui[0];
+    const T u1r = r ? u.ur[3] : u.ur[1], u1i = r ? u.ui[3] : u.ui[1];
+    T yr = lo.x * u0r - lo.y * u0i;
+    T yi = lo.x * u0i + lo.y * u0r;
+    const T pr = hi.x * u1r - hi.y * u1i;
+    const T pi = hi.x * u1i + hi.y * u1r;
+    vec2_t<T> y;
+    y.x = yr + pr;
+    y.y = yi + pi;
+    return y;
+}
+
+// q < 6: the partner (index i ^ 2^q) is in the same wavefront; a lane shuffle
+// brings it.  len is a multiple of kBlock, so every lane of every wave is live.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dense1_lane(vec2_t<T> *__restrict__ psi, uint64_t len, uint32_t q,
+                                                        const Dense1Coef<T> u) {
+    using V = vec2_t<T>;
+    const uint64_t step = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < len; i += step) {
+        const V x = ldg<true>(&psi[i]);
+        V p;
+        p.x = __shfl_xor(x.x, 1 << q, 64);
+        p.y = __shfl_xor(x.y, 1 << q, 64);
+        const uint32_t r = (uint32_t)(i >> q) & 1u;
+        stg<true>(&psi[i], dense1_row<T>(r ? p : x, r ? x : p, r, u));
+    }
+}
+
+// q >= 6: a workgroup takes 128 pairs; waves 0 and 2 load 64 lower members
+// each (contiguous), waves 1 and 3 the matching upper members, and the
+// partners are exchanged through LDS.  npairs is a multiple of 128.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dense1_xchg(vec2_t<T> *__restrict__ psi, uint64_t npairs, uint32_t q,
+                                                        const Dense1Coef<T> u) {
+    using V = vec2_t<T>;
+    __shared__ V sm[kBlock];
+    const uint32_t t = threadIdx.x, h = (t >> 6) & 1u;
+    const uint64_t nchunk = npairs / 128;
+    for (uint64_t c = blockIdx.x; c < nchunk; c += gridDim.x) {
+        const uint64_t pair = c * 128 + ((t >> 7) << 6) + (t & 63u);
+        const uint64_t a = insert_zero(pair, q) | ((uint64_t)h << q);
+        const V x = ldg<true>(&psi[a]);
+        sm[t] = x;
+        __syncthreads();
+        const V p = sm[t ^ 64u];
+        __syncthreads();  // sm is rewritten by the next chunk
+        stg<true>(&psi[a], dense1_row<T>(h ? p : x, h ? x : p, h, u));
+    }
+}
+
 // Dense k-qubit operator, 4 <= k <= 6: the matrix is staged in LDS
 // (transposed so that the D lanes of a group read consecutive words), each
 // group's D amplitudes are staged in LDS, lane r of the group produces row r.
@@ -162,31 +228,41 @@ __global__ __launch_bounds__(kBlock) void k_dense_big(vec2_t<T> *__restrict__ ps
 }
 
 // Diagonal operator with every entry != 1 and no controls: contiguous sweep,
-// amplitude i is multiplied by d[bits of i at the targets].
-template <typename T, int PPT, bool NTM>
+// amplitude i is multiplied by d[bits of i at the targets].  KT = 1..3: the
+// entry is chosen from the (scalar) argument block by a compile-time chain
+// of selects; KT = 0: any width, runtime loop and indexed entry.
+// One amplitude per thread; the loop is written so that the store keeps its
+// non-temporal flag (a conditional store inside an unrolled per-thread batch
+// lost it).
+template <typename T, bool NTM, int KT>
 __global__ __launch_bounds__(kBlock) void k_diag_full(vec2_t<T> *__restrict__ psi,
                                                       const DiagFullArgs<T> a) {
     using V = vec2_t<T>;
-    const uint64_t step = (uint64_t)gridDim.x * (kBlock * PPT);
-    for (uint64_t first = (uint64_t)blockIdx.x * (kBlock * PPT) + threadIdx.x; first < a.len; first += step) {
-        V x[PPT];
+    const uint64_t step = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < a.len; i += step) {
+        const V x = ldg<NTM>(&psi[i]);
+        T mr, mi;
+        if constexpr (KT > 0) {
+            uint32_t j = 0;
 #pragma unroll
-        for (int p = 0; p < PPT; ++p) {
-            const uint64_t i = first + (uint64_t)p * kBlock;
-            if (i < a.len) x[p] = ldg<NTM>(&psi[i]);
-        }
+            for (int t = 0; t < KT; ++t) j |= (uint32_t)((i >> a.tq[t]) & 1u) << t;
+            mr = a.dr[0];
+            mi = a.di[0];
 #pragma unroll
-        for (int p = 0; p < PPT; ++p) {
-            const uint64_t i = first + (uint64_t)p * kBlock;
-            if (i >= a.len) continue;
+            for (uint32_t e = 1; e < (1u << KT); ++e) {
+                mr = j == e ? a.dr[e] : mr;
+                mi = j == e ? a.di[e] : mi;
+            }
+        } else {
             uint32_t j = 0;
             for (uint32_t t = 0; t < a.kt; ++t) j |= (uint32_t)((i >> a.tq[t]) & 1u) << t;
-            const T mr = a.dr[j], mi = a.di[j];
-            V y;
-            y.x = x[p].x * mr - x[p].y * mi;
-            y.y = x[p].x * mi + x[p].y * mr;
-            stg<NTM>(&psi[i], y);
+            mr = a.dr[j];
+            mi = a.di[j];
         }
+        V y;
+        y.x = x.x * mr - x.y * mi;
+        y.y = x.x * mi + x.y * mr;
+        stg<NTM>(&psi[i], y);
     }
 }
 
@@ -251,6 +327,31 @@ __global__ __launch_bounds__(kBlock) void k_perm(vec2_t<T> *__restrict__ psi,
                 for (uint32_t e = 0; e < a.nent; ++e) psi[base + a.off[e]] = tmp[e];
             }
         }
+    }
+}
+
+// A single transposition (CNOT, SWAP, CCNOT, CSWAP) whose target and control
+// bits are all below 6 (and the lowest below 3): every amplitude is in the
+// same wavefront as its partner, so the kernel sweeps the whole state one
+// amplitude per thread and takes the partner through a lane shuffle.  Amplitude i is replaced by
+// amplitude i ^ flip when its target bits (mask tmask) equal o0 or o1 and its
+// controls are set; the others are written back unchanged, which keeps every
+// store a full contiguous run (the pair-per-group form touches half of every
+// line when the bits are this low).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_perm_lane(vec2_t<T> *__restrict__ psi, uint64_t len, uint32_t tmask,
+                                                      uint32_t o0, uint32_t o1, uint32_t ctrl) {
+    using V = vec2_t<T>;
+    const uint32_t flip = o0 ^ o1;
+    const uint64_t step = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < len; i += step) {
+        const V x = ldg<true>(&psi[i]);
+        V p;
+        p.x = __shfl_xor(x.x, (int)flip, 64);
+        p.y = __shfl_xor(x.y, (int)flip, 64);
+        const uint32_t lo = (uint32_t)i & 63u;
+        const bool moved = ((lo & tmask) == o0 || (lo & tmask) == o1) && (lo & ctrl) == ctrl;
+        stg<true>(&psi[i], moved ? p : x);
     }
 }
 
@@ -420,6 +521,27 @@ int run_dense_small(qvmhip_state *s, const GateSplit &gs) {
         a.ur[i] = (T)gs.V[i].real();
         a.ui[i] = (T)gs.V[i].imag();
     }
+    if constexpr (KT == 1) {
+        if (a.g.ctrl_mask == 0 && s->n >= 8 && gs.tq[0] < 8) {
+            // low target: one amplitude per thread (k_dense1_lane / k_dense1_xchg)
+            Dense1Coef<T> u;
+            for (int i = 0; i < 4; ++i) {
+                u.ur[i] = (T)gs.V[i].real();
+                u.ui[i] = (T)gs.V[i].imag();
+            }
+            const uint32_t q = gs.tq[0];
+            prof_begin(s, KT);
+            if (q < 6)
+                hipLaunchKernelGGL((k_dense1_lane<T>), dim3(grid_for(s->len, kBlock)), dim3(kBlock), 0, s->stream,
+                                   (vec2_t<T> *)s->amps, s->len, q, u);
+            else
+                hipLaunchKernelGGL((k_dense1_xchg<T>), dim3(grid_for(s->len / 2, 128)), dim3(kBlock), 0, s->stream,
+                                   (vec2_t<T> *)s->amps, s->len / 2, q, u);
+            prof_end(s);
+            QH_LAUNCH_CHECK();
+            return QVMHIP_OK;
+        }
+    }
     const dim3 grid(grid_for(a.g.ngroups, (uint64_t)kBlock * GPT));
     prof_begin(s, KT);
     if (long_runs(a.g, s->amp_bytes))
@@ -476,10 +598,18 @@ int run_diag(qvmhip_state *s, const GateSplit &gs) {
             a.dr[j] = (T)gs.V[(size_t)j * Dr + j].real();
             a.di[j] = (T)gs.V[(size_t)j * Dr + j].imag();
         }
-        constexpr int PPT = 4;
+        // one amplitude per thread and no per-lane loads of the entries:
+        // measured 6.6 TB/s on MI355X for the in-place sweep, against
+        // 5.9 TB/s with four amplitudes per thread and indexed entries
+        const dim3 grid(grid_for(a.len, (uint64_t)kBlock));
+        V2Ptr<T> psi = (vec2_t<T> *)s->amps;
         prof_begin(s, QVMHIP_K_DIAG_FULL);
-        hipLaunchKernelGGL((k_diag_full<T, PPT, true>), dim3(grid_for(a.len, (uint64_t)kBlock * PPT)),
-                           dim3(kBlock), 0, s->stream, (vec2_t<T> *)s->amps, a);
+        switch (gs.kt) {
+            case 1: hipLaunchKernelGGL((k_diag_full<T, true, 1>), grid, dim3(kBlock), 0, s->stream, psi, a); break;
+            case 2: hipLaunchKernelGGL((k_diag_full<T, true, 2>), grid, dim3(kBlock), 0, s->stream, psi, a); break;
+            case 3: hipLaunchKernelGGL((k_diag_full<T, true, 3>), grid, dim3(kBlock), 0, s->stream, psi, a); break;
+            default: hipLaunchKernelGGL((k_diag_full<T, true, 0>), grid, dim3(kBlock), 0, s->stream, psi, a);
+        }
         prof_end(s);
         QH_LAUNCH_CHECK();
         return QVMHIP_OK;
@@ -523,6 +653,22 @@ int run_perm(qvmhip_state *s, const GateSplit &gs) {
         ++a.nent;
     }
     if (a.nent == 0) return QVMHIP_OK;
+    if (a.nent == 2 && s->n >= 8 && a.g.nsorted && a.g.sorted[a.g.nsorted - 1] < 6 &&
+        !long_runs(a.g, s->amp_bytes)) {
+        // every bit involved is a lane bit and the moved amplitudes come in
+        // runs shorter than a 128 B line: a whole-state sweep with shuffles
+        // (5.3 ms at n = 30) beats the half-state pair form (6.4 ms); from
+        // 128 B runs on the pair form moves half the bytes (3.0 ms)
+        uint32_t tmask = 0;
+        for (uint32_t t = 0; t < gs.kt; ++t) tmask |= 1u << gs.tq[t];
+        prof_begin(s, QVMHIP_K_PERM);
+        hipLaunchKernelGGL((k_perm_lane<T>), dim3(grid_for(s->len, kBlock)), dim3(kBlock), 0, s->stream,
+                           (vec2_t<T> *)s->amps, s->len, tmask, (uint32_t)a.off[0], (uint32_t)a.off[1],
+                           (uint32_t)a.g.ctrl_mask);
+        prof_end(s);
+        QH_LAUNCH_CHECK();
+        return QVMHIP_OK;
+    }
     constexpr int GPT = 1;
     const dim3 grid(grid_for(a.g.ngroups, (uint64_t)kBlock * GPT));
     prof_begin(s, QVMHIP_K_PERM);

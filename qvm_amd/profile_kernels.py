@@ -28,11 +28,15 @@ def main(argv=None) -> int:
     ap.add_argument("--reps", type=int, default=8)
     ap.add_argument("--fma", action="store_true", help="tile passes with FMA-contracted products")
     ap.add_argument("--tile-m", type=int, default=0, help="tile width (default: the planner's)")
+    ap.add_argument("--qubits", default="", help="comma-separated target qubits, cycled (default: random)")
     a = ap.parse_args(argv)
+    fixed = [int(x) for x in a.qubits.split(",") if x.strip()]
     st = HipPureState(a.n, a.dtype, fma_tiles=a.fma)
     rng = np.random.default_rng(0)
     for r in range(a.reps):
         q = int(rng.integers(0, a.n))
+        if fixed:
+            q = fixed[r % len(fixed)]
         if a.kernel == "dense1":
             st.apply_matrix(G.gate_matrix("RX", [0.1 + r]), [q])
         elif a.kernel == "diag_full":
