@@ -53,7 +53,7 @@ struct FusedOff {
     uint32_t nat[6]; // position in the nat-tuple of the b-th lowest target
 };
 
-template <typename T, int K>
+template <typename T, int K, bool NTM>
 __global__ __launch_bounds__(256) void k_fused(vec2_t<T> *__restrict__ psi, const double *__restrict__ U,
                                                GroupSpec g, FusedOff off) {
     using V = vec2_t<T>;
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(256) void k_fused(vec2_t<T> *__restrict__ psi, cons
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
             if (ok) {
-                xv[ks] = psi[base + (off.o[4 * ks] | lo_ld)];
+                xv[ks] = ldg<NTM>(&psi[base + (off.o[4 * ks] | lo_ld)]);
             } else {
                 xv[ks].x = T(0);
                 xv[ks].y = T(0);
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256) void k_fused(vec2_t<T> *__restrict__ psi, cons
                         V y;
                         y.x = ar[mt][e];
                         y.y = ai[mt][e];
-                        psi[base + (off.o[hi] | lo_st)] = y;
+                        stg<NTM>(&psi[base + (off.o[hi] | lo_st)], y);
                     }
                 }
             }
@@ -214,18 +214,33 @@ int run_fused(qvmhip_state *s, const void *mat_dev, const uint32_t *qubits) {
     static int per_cu = 0;
     if (per_cu == 0) {
         int nb = 0;
-        per_cu = (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fused<T, K>, 256, 0) == hipSuccess && nb > 0)
-                     ? nb
-                     : 2;
+        per_cu =
+            (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fused<T, K, false>, 256, 0) == hipSuccess && nb > 0)
+                ? nb
+                : 2;
     }
+    // Non-temporal accesses when every 128 B line a k-step touches is read
+    // whole in that k-step: the k-step's rows vary the two lowest targets, so
+    // no other target may lie below the line size.  Otherwise a line is read
+    // in two k-steps and streaming it past the cache reads it twice
+    // (measured: targets 0,1,2 at k = 3, 7.7 ms cached vs 8.7 ms streamed;
+    // targets 0,5,11, 6.5 vs 6.3 ms).
+    const uint32_t line_bits = sizeof(T) == 8 ? 3u : 4u;
+    bool ntm = true;
+    for (int b = 2; b < K; ++b)
+        if (pos[b] < line_bits) ntm = false;
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess || ncu <= 0)
         ncu = 256;
     const uint64_t ncg = (g.ngroups + 15) / 16;
     const uint32_t grid = (uint32_t)std::min<uint64_t>((ncg + 3) / 4, (uint64_t)per_cu * (uint64_t)ncu);
     prof_begin(s, QVMHIP_K_FUSED);
-    hipLaunchKernelGGL((k_fused<T, K>), dim3(grid), dim3(256), 0, s->stream, (vec2_t<T> *)s->amps,
-                       (const double *)mat_dev, g, off);
+    if (ntm)
+        hipLaunchKernelGGL((k_fused<T, K, true>), dim3(grid), dim3(256), 0, s->stream, (vec2_t<T> *)s->amps,
+                           (const double *)mat_dev, g, off);
+    else
+        hipLaunchKernelGGL((k_fused<T, K, false>), dim3(grid), dim3(256), 0, s->stream, (vec2_t<T> *)s->amps,
+                           (const double *)mat_dev, g, off);
     prof_end(s);
     QH_LAUNCH_CHECK();
     return QVMHIP_OK;

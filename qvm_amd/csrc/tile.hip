@@ -456,7 +456,7 @@ __device__ __forceinline__ void tile_load(const vec2_t<T> *__restrict__ psi, con
                                           uint64_t toff, vec2_t<T> (&x)[1 << RB]) {
     const vec2_t<T> *p = psi + gb + toff;
 #pragma unroll
-    for (int r = 0; r < (1 << RB); ++r) x[r] = p[a.roff[r]];
+    for (int r = 0; r < (1 << RB); ++r) x[r] = ldg<true>(&p[a.roff[r]]);
 }
 
 template <typename T, int M, int RB>
@@ -467,7 +467,7 @@ __device__ __forceinline__ void tile_store(vec2_t<T> *__restrict__ psi, const Ti
     // the op loop
     asm volatile("" : "+v"(p));
 #pragma unroll
-    for (int r = 0; r < (1 << RB); ++r) p[a.roff[r]] = x[r];
+    for (int r = 0; r < (1 << RB); ++r) stg<true>(&p[a.roff[r]], x[r]);
 }
 
 // Run a scheduled op list on the block held in x.
