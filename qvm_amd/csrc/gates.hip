@@ -234,6 +234,49 @@ __global__ __launch_bounds__(kBlock) void k_dense_big(vec2_t<T> *__restrict__ ps
 // One amplitude per thread; the loop is written so that the store keeps its
 // non-temporal flag (a conditional store inside an unrolled per-thread batch
 // lost it).
+template <typename T, int KT>
+__device__ __forceinline__ void diag_entry(const DiagFullArgs<T> &a, uint64_t i, T &mr, T &mi) {
+    if constexpr (KT > 0) {
+        uint32_t j = 0;
+#pragma unroll
+        for (int t = 0; t < KT; ++t) j |= (uint32_t)((i >> a.tq[t]) & 1u) << t;
+        mr = a.dr[0];
+        mi = a.di[0];
+#pragma unroll
+        for (uint32_t e = 1; e < (1u << KT); ++e) {
+            mr = j == e ? a.dr[e] : mr;
+            mi = j == e ? a.di[e] : mi;
+        }
+    } else {
+        uint32_t j = 0;
+        for (uint32_t t = 0; t < a.kt; ++t) j |= (uint32_t)((i >> a.tq[t]) & 1u) << t;
+        mr = a.dr[j];
+        mi = a.di[j];
+    }
+}
+
+// FP32: two amplitudes per thread as one 16-byte access (the FP64 kernel's
+// one-per-lane form would make 512 B wavefront accesses).  len is even.
+template <int KT>
+__global__ __launch_bounds__(kBlock) void k_diag_full_f32x2(float2 *__restrict__ psi, const DiagFullArgs<float> a) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    f4 *p4 = (f4 *)psi;
+    const uint64_t n2 = a.len / 2;
+    const uint64_t step = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < n2; k += step) {
+        const f4 x = __builtin_nontemporal_load(&p4[k]);
+        float m0r, m0i, m1r, m1i;
+        diag_entry<float, KT>(a, 2 * k, m0r, m0i);
+        diag_entry<float, KT>(a, 2 * k + 1, m1r, m1i);
+        f4 y;
+        y.x = x.x * m0r - x.y * m0i;
+        y.y = x.x * m0i + x.y * m0r;
+        y.z = x.z * m1r - x.w * m1i;
+        y.w = x.z * m1i + x.w * m1r;
+        __builtin_nontemporal_store(y, &p4[k]);
+    }
+}
+
 template <typename T, bool NTM, int KT>
 __global__ __launch_bounds__(kBlock) void k_diag_full(vec2_t<T> *__restrict__ psi,
                                                       const DiagFullArgs<T> a) {
@@ -242,23 +285,7 @@ __global__ __launch_bounds__(kBlock) void k_diag_full(vec2_t<T> *__restrict__ ps
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < a.len; i += step) {
         const V x = ldg<NTM>(&psi[i]);
         T mr, mi;
-        if constexpr (KT > 0) {
-            uint32_t j = 0;
-#pragma unroll
-            for (int t = 0; t < KT; ++t) j |= (uint32_t)((i >> a.tq[t]) & 1u) << t;
-            mr = a.dr[0];
-            mi = a.di[0];
-#pragma unroll
-            for (uint32_t e = 1; e < (1u << KT); ++e) {
-                mr = j == e ? a.dr[e] : mr;
-                mi = j == e ? a.di[e] : mi;
-            }
-        } else {
-            uint32_t j = 0;
-            for (uint32_t t = 0; t < a.kt; ++t) j |= (uint32_t)((i >> a.tq[t]) & 1u) << t;
-            mr = a.dr[j];
-            mi = a.di[j];
-        }
+        diag_entry<T, KT>(a, i, mr, mi);
         V y;
         y.x = x.x * mr - x.y * mi;
         y.y = x.x * mi + x.y * mr;
@@ -521,7 +548,9 @@ int run_dense_small(qvmhip_state *s, const GateSplit &gs) {
         a.ur[i] = (T)gs.V[i].real();
         a.ui[i] = (T)gs.V[i].imag();
     }
-    if constexpr (KT == 1) {
+    if constexpr (KT == 1 && sizeof(T) == 8) {
+        // FP64 only: with 8-byte FP32 amplitudes one per lane is a 512 B
+        // wavefront access, slower than the pair form (5.8 vs 6.6 TB/s)
         if (a.g.ctrl_mask == 0 && s->n >= 8 && gs.tq[0] < 8) {
             // low target: one amplitude per thread (k_dense1_lane / k_dense1_xchg)
             Dense1Coef<T> u;
@@ -601,8 +630,23 @@ int run_diag(qvmhip_state *s, const GateSplit &gs) {
         // one amplitude per thread and no per-lane loads of the entries:
         // measured 6.6 TB/s on MI355X for the in-place sweep, against
         // 5.9 TB/s with four amplitudes per thread and indexed entries
-        const dim3 grid(grid_for(a.len, (uint64_t)kBlock));
         V2Ptr<T> psi = (vec2_t<T> *)s->amps;
+        if constexpr (sizeof(T) == 4) {
+            if (s->len >= 2) {
+                const dim3 g2(grid_for(a.len / 2, (uint64_t)kBlock));
+                prof_begin(s, QVMHIP_K_DIAG_FULL);
+                switch (gs.kt) {
+                    case 1: hipLaunchKernelGGL((k_diag_full_f32x2<1>), g2, dim3(kBlock), 0, s->stream, psi, a); break;
+                    case 2: hipLaunchKernelGGL((k_diag_full_f32x2<2>), g2, dim3(kBlock), 0, s->stream, psi, a); break;
+                    case 3: hipLaunchKernelGGL((k_diag_full_f32x2<3>), g2, dim3(kBlock), 0, s->stream, psi, a); break;
+                    default: hipLaunchKernelGGL((k_diag_full_f32x2<0>), g2, dim3(kBlock), 0, s->stream, psi, a);
+                }
+                prof_end(s);
+                QH_LAUNCH_CHECK();
+                return QVMHIP_OK;
+            }
+        }
+        const dim3 grid(grid_for(a.len, (uint64_t)kBlock));
         prof_begin(s, QVMHIP_K_DIAG_FULL);
         switch (gs.kt) {
             case 1: hipLaunchKernelGGL((k_diag_full<T, true, 1>), grid, dim3(kBlock), 0, s->stream, psi, a); break;
@@ -657,8 +701,9 @@ int run_perm(qvmhip_state *s, const GateSplit &gs) {
         !long_runs(a.g, s->amp_bytes)) {
         // every bit involved is a lane bit and the moved amplitudes come in
         // runs shorter than a 128 B line: a whole-state sweep with shuffles
-        // (5.3 ms at n = 30) beats the half-state pair form (6.4 ms); from
-        // 128 B runs on the pair form moves half the bytes (3.0 ms)
+        // beats the half-state pair form (n = 30: 5.3 vs 6.4 ms FP64, 2.8 vs
+        // 3.2 ms FP32); from 128 B runs on the pair form moves half the
+        // bytes (3.0 ms FP64)
         uint32_t tmask = 0;
         for (uint32_t t = 0; t < gs.kt; ++t) tmask |= 1u << gs.tq[t];
         prof_begin(s, QVMHIP_K_PERM);
