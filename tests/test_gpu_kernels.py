@@ -1,0 +1,107 @@
+"""Plain-kernel dispatch paths against the oracle, one case per path: 1-qubit
+dense gates on every target (one amplitude per thread with lane shuffles for
+targets < 6, LDS exchange for 6 and 7, the pair form above; controlled gates
+always take the pair form), full diagonals of width 1..4 (compile-time entry
+selects for 1..3, indexed entries for 4; two amplitudes per access in FP32),
+and single transpositions whose bits are all low (whole-state lane sweep) or
+not.  FP64 results are bit-identical to the oracle's serial arithmetic; FP32
+within 1e-6."""
+import sys
+
+import numpy as np
+import pytest
+
+from qvm_amd import _native as N
+from qvm_amd import gates as G
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def S():
+    from qvm_amd.state import HipPureState
+    assert N.device_count() >= 1, "no HIP device visible"
+    return HipPureState
+
+
+def rand_state(n, rng):
+    v = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    return v / np.linalg.norm(v)
+
+
+def rand_u(k, rng):
+    Z = rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k))
+    return np.linalg.qr(Z)[0]
+
+
+def check(st, ref, dtype):
+    got = st.amplitudes()
+    if dtype == "f64":
+        np.testing.assert_array_equal(got, ref)
+    else:
+        assert np.max(np.abs(got - ref)) < 1e-6
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("n", [8, 13])
+def test_dense1_every_target(S, oracle, dtype, n):
+    rng = np.random.default_rng(n)
+    say(f"dense 1q every target n={n} {dtype}")
+    psi = rand_state(n, rng)
+    st = S(n, dtype)
+    st.write_amplitudes(psi)
+    ref = psi.astype(np.complex64).astype(np.complex128) if dtype == "f32" else psi.copy()
+    for q in list(range(n)) + list(range(n - 1, -1, -1)):
+        U = rand_u(1, rng)
+        st.apply_matrix(U, [q])
+        oracle.apply_matrix(ref, U, [q])
+    # controlled 1q gates (pair form) on low targets
+    for t, c in ((0, 1), (5, 2), (6, 7), (7, 0), (2, n - 1)):
+        U = G.controlled(rand_u(1, rng))
+        st.apply_matrix(U, G.nat_tuple(c, t))
+        oracle.apply_matrix(ref, U, G.nat_tuple(c, t))
+    check(st, ref, dtype)
+    st.close()
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_full_diagonals_of_every_width(S, oracle, dtype):
+    n = 12
+    rng = np.random.default_rng(3)
+    say(f"full diagonals {dtype}")
+    psi = rand_state(n, rng)
+    st = S(n, dtype)
+    st.write_amplitudes(psi)
+    ref = psi.astype(np.complex64).astype(np.complex128) if dtype == "f32" else psi.copy()
+    for qs in ([0], [7], [11], [1, 9], [10, 2], [0, 5, 11], [3, 1, 8], [0, 4, 7, 10]):
+        k = len(qs)
+        D = np.diag(np.exp(1j * rng.uniform(0.1, 6.0, 1 << k)))  # every entry != 1
+        st.apply_matrix(D, list(qs))
+        oracle.apply_matrix(ref, D, list(qs))
+    check(st, ref, dtype)
+    st.close()
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_transpositions_low_and_high_bits(S, oracle, dtype):
+    n = 11
+    rng = np.random.default_rng(5)
+    say(f"transpositions {dtype}")
+    psi = rand_state(n, rng)
+    st = S(n, dtype)
+    st.write_amplitudes(psi)
+    ref = psi.astype(np.complex64).astype(np.complex128) if dtype == "f32" else psi.copy()
+    cases = [("CNOT", [1, 0]), ("CNOT", [0, 2]), ("CNOT", [4, 1]), ("SWAP", [0, 5]), ("SWAP", [2, 1]),
+             ("CCNOT", [0, 1, 2]), ("CSWAP", [2, 0, 1]),        # every bit < 6, lowest < 3: lane sweep
+             ("CNOT", [3, 4]), ("CNOT", [9, 0]), ("SWAP", [1, 10]), ("CCNOT", [8, 2, 5])]
+    for name, qs in cases:
+        U = G.gate_matrix(name)
+        st.apply_matrix(U, G.nat_tuple(*qs))
+        oracle.apply_matrix(ref, U, G.nat_tuple(*qs))
+    check(st, ref, dtype)
+    st.close()
