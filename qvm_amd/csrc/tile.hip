@@ -255,20 +255,20 @@ __device__ __forceinline__ void pick4(const qvmhip_tile_op &op, uint32_t j, T &d
     di = (j & 2u) ? ib : ia;
 }
 
-// Register r takes entry (c1) when its bit RS is set, (c0) otherwise.
-template <typename T, int R, int RS, bool CTRL, bool FMA>
-__device__ __forceinline__ void diag_rbit(vec2_t<T> (&x)[R], T c0r, T c0i, T c1r, T c1i, bool lane_ok,
-                                          uint32_t rcm) {
+// Register r takes entry (c1) when its bit RS is set, (c0) otherwise;
+// registers whose register-control bits (rcm) are not all set are skipped (a
+// uniform test).  Lane controls are folded into the entries by the caller.
+template <typename T, int R, int RS, bool FMA>
+__device__ __forceinline__ void diag_rbit(vec2_t<T> (&x)[R], T c0r, T c0i, T c1r, T c1i, uint32_t rcm) {
     if constexpr ((1 << RS) < R) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const bool one = ((r >> RS) & 1) != 0;
-            vec2_t<T> y;
-            cmul<T, FMA>(x[r].x, x[r].y, one ? c1r : c0r, one ? c1i : c0i, y.x, y.y);
-            if constexpr (CTRL)
-                x[r] = vsel<T>(lane_ok && ((uint32_t)r & rcm) == rcm, y, x[r]);
-            else
+            if (((uint32_t)r & rcm) == rcm) {
+                const bool one = ((r >> RS) & 1) != 0;
+                vec2_t<T> y;
+                cmul<T, FMA>(x[r].x, x[r].y, one ? c1r : c0r, one ? c1i : c0i, y.x, y.y);
                 x[r] = y;
+            }
         }
     }
 }
@@ -288,50 +288,44 @@ __device__ __forceinline__ void reg_rdiag(vec2_t<T> (&x)[R], const qvmhip_tile_o
 
 // diagonal 1q / 2q and CPHASE on any slot bits.  The diagonal index has a
 // lane part (target bits that are thread bits, the same for all registers)
-// and a register part (uniform per register).  Three cases keep the
-// per-register choice free of vector selects: all index bits on registers
-// (scalar entries), all on lanes (one entry per lane), one of each (two
-// entries per lane, the register bit's position made a compile-time
-// constant by a switch).
+// and a register part (uniform per register).  No case needs a per-register
+// vector select: the lane part picks one entry (or two) per lane once per op,
+// a lane control (or CPHASE condition) turns a lane's entry into 1, register
+// controls skip whole registers, and the register part is a scalar choice or,
+// for one lane and one register bit, a compile-time one (switch on the
+// register bit).  A lane whose condition fails multiplies by exactly 1, which
+// leaves its amplitude's value unchanged.
 template <typename T, int R, int TB, bool CTRL, bool FMA>
 __device__ __forceinline__ void reg_diag(vec2_t<T> (&x)[R], const qvmhip_tile_op &op, uint32_t t, bool lane_ok,
-                                         uint32_t rcm) {
+                                         uint32_t rcm_in) {
     using V = vec2_t<T>;
     const uint32_t kind = op.kind, q0 = op.q0, q1 = op.q1;
     constexpr uint32_t lanes = (1u << TB) - 1u;
+    const uint32_t rcm = CTRL ? rcm_in : 0u;
     if (kind == QVMHIP_TILE_CPHASE) {
         const uint32_t need = op.ctrl_mask | (1u << q0) | (1u << q1);
-        const uint32_t rn = need >> TB;
-        const T mr = mval<T>(op, 0), mi = mval<T>(op, 1);
-        if ((need & lanes) == 0) {
-            // every condition bit is a register bit: a uniform test per
-            // register, the other registers are not touched
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                if (((uint32_t)r & rn) == rn) {
-                    V y;
-                    cmul<T, FMA>(x[r].x, x[r].y, mr, mi, y.x, y.y);
-                    x[r] = y;
-                }
-            }
-            return;
+        const uint32_t rn = need >> TB, ln = need & lanes;
+        T fr = mval<T>(op, 0), fi = mval<T>(op, 1);
+        if (ln) {
+            const bool lok = (t & ln) == ln;
+            fr = lok ? fr : T(1);
+            fi = lok ? fi : T(0);
         }
-        const bool lok = (t & need) == (need & lanes);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const bool ok = lok && ((uint32_t)r & rn) == rn;
-            V y;
-            cmul<T, FMA>(x[r].x, x[r].y, mr, mi, y.x, y.y);
-            x[r] = vsel<T>(ok, y, x[r]);
+            if (((uint32_t)r & rn) == rn) {
+                V y;
+                cmul<T, FMA>(x[r].x, x[r].y, fr, fi, y.x, y.y);
+                x[r] = y;
+            }
         }
         return;
     }
     const bool two = kind == QVMHIP_TILE_DIAG2;
     const bool reg0 = q0 >= TB, reg1 = two && q1 >= TB;
     if (reg0 && (!two || reg1)) {
-        // index bits are all register bits (the scheduler turns such ops
-        // into register diagonals unless a control bit is a lane bit): the
-        // entry of each register is uniform
+        // index bits all on registers (the scheduler turns such ops into
+        // register diagonals unless a control bit is a lane bit)
         const uint32_t rs0 = q0 - TB, rs1 = two ? q1 - TB : 0u;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -339,12 +333,13 @@ __device__ __forceinline__ void reg_diag(vec2_t<T> (&x)[R], const qvmhip_tile_op
                 const uint32_t jr = (((uint32_t)r >> rs0) & 1u) | (two ? ((((uint32_t)r >> rs1) & 1u) << 1) : 0u);
                 T dr, di;
                 pick4<T>(op, jr, dr, di);
+                if constexpr (CTRL) {
+                    dr = lane_ok ? dr : T(1);
+                    di = lane_ok ? di : T(0);
+                }
                 V y;
                 cmul<T, FMA>(x[r].x, x[r].y, dr, di, y.x, y.y);
-                if constexpr (CTRL)
-                    x[r] = vsel<T>(lane_ok, y, x[r]);
-                else
-                    x[r] = y;
+                x[r] = y;
             }
         }
         return;
@@ -352,17 +347,25 @@ __device__ __forceinline__ void reg_diag(vec2_t<T> (&x)[R], const qvmhip_tile_op
     // lane part of the index
     const uint32_t jl = (reg0 ? 0u : ((t >> q0) & 1u)) | ((two && !reg1) ? (((t >> q1) & 1u) << 1) : 0u);
     if (!reg0 && !reg1) {
-        // index bits are all lane bits: one entry per lane for every register
+        // index bits all on lanes: one entry per lane for every register
         T dr, di;
-        pick4<T>(op, jl, dr, di);
+        if (two) {
+            pick4<T>(op, jl, dr, di);
+        } else {
+            dr = jl ? mval<T>(op, 2) : mval<T>(op, 0);
+            di = jl ? mval<T>(op, 3) : mval<T>(op, 1);
+        }
+        if constexpr (CTRL) {
+            dr = lane_ok ? dr : T(1);
+            di = lane_ok ? di : T(0);
+        }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            V y;
-            cmul<T, FMA>(x[r].x, x[r].y, dr, di, y.x, y.y);
-            if constexpr (CTRL)
-                x[r] = vsel<T>(lane_ok && ((uint32_t)r & rcm) == rcm, y, x[r]);
-            else
+            if (((uint32_t)r & rcm) == rcm) {
+                V y;
+                cmul<T, FMA>(x[r].x, x[r].y, dr, di, y.x, y.y);
                 x[r] = y;
+            }
         }
         return;
     }
@@ -373,12 +376,18 @@ __device__ __forceinline__ void reg_diag(vec2_t<T> (&x)[R], const qvmhip_tile_op
     T c0r, c0i, c1r, c1i;
     pick4<T>(op, jl, c0r, c0i);
     pick4<T>(op, jl | w, c1r, c1i);
+    if constexpr (CTRL) {
+        c0r = lane_ok ? c0r : T(1);
+        c0i = lane_ok ? c0i : T(0);
+        c1r = lane_ok ? c1r : T(1);
+        c1i = lane_ok ? c1i : T(0);
+    }
     switch (rs) {
-        case 0: diag_rbit<T, R, 0, CTRL, FMA>(x, c0r, c0i, c1r, c1i, lane_ok, rcm); break;
-        case 1: diag_rbit<T, R, 1, CTRL, FMA>(x, c0r, c0i, c1r, c1i, lane_ok, rcm); break;
-        case 2: diag_rbit<T, R, 2, CTRL, FMA>(x, c0r, c0i, c1r, c1i, lane_ok, rcm); break;
-        case 3: diag_rbit<T, R, 3, CTRL, FMA>(x, c0r, c0i, c1r, c1i, lane_ok, rcm); break;
-        case 4: diag_rbit<T, R, 4, CTRL, FMA>(x, c0r, c0i, c1r, c1i, lane_ok, rcm); break;
+        case 0: diag_rbit<T, R, 0, FMA>(x, c0r, c0i, c1r, c1i, rcm); break;
+        case 1: diag_rbit<T, R, 1, FMA>(x, c0r, c0i, c1r, c1i, rcm); break;
+        case 2: diag_rbit<T, R, 2, FMA>(x, c0r, c0i, c1r, c1i, rcm); break;
+        case 3: diag_rbit<T, R, 3, FMA>(x, c0r, c0i, c1r, c1i, rcm); break;
+        case 4: diag_rbit<T, R, 4, FMA>(x, c0r, c0i, c1r, c1i, rcm); break;
         default: break;
     }
 }

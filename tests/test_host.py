@@ -247,6 +247,17 @@ def test_error_paths_without_a_device():
         N.tile_schedule("f64", 12, [op])  # local bit outside the tile
     assert L.qvmhip_batch_set_flags(None, 0) == N.EINVAL
     assert L.qvmhip_total_probability(None, None) == N.EINVAL
+    # GDIAG: one or two outside qubits, no controls, local q0 inside the tile
+    g = N.TileOp()
+    g.kind, g.q0, g.q1 = N.TILE_GDIAG, 3, 20 | (1 << 16)
+    assert len(N.tile_schedule("f64", 11, [g])) == 1
+    for q0, q1, ctrl in ((3, 20, 0), (3, 20 | (3 << 16), 0), (11, 20 | (1 << 16), 0), (3, 20 | (1 << 16), 1)):
+        g.q0, g.q1, g.ctrl_mask = q0, q1, ctrl
+        with pytest.raises(N.QVMHipError):
+            N.tile_schedule("f64", 11, [g])
+    g.kind = 20  # beyond the public kinds
+    with pytest.raises(N.QVMHipError):
+        N.tile_schedule("f64", 11, [g])
 
 
 # --- compiled measure chains and noise helpers (host logic) ------------------
