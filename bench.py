@@ -347,10 +347,13 @@ def batch(args) -> int:
            "instances": len(inst), "gates": sum(len(c) for c in circuits), "amp_updates_per_run": upd,
            "shots_per_instance": 1024, "results": {}}
     hcs = {}
+    chosen = {}
     for dtype in ("f64", "f32"):
         t0 = time.perf_counter()
-        b, _, _ = qaoa_batch(inst, dtype, shots=1024, seed=9, use_graph=True, fma_tiles=True)
+        # tile width picked by the autotuner (timed on every 4th instance)
+        b, _, _ = qaoa_batch(inst, dtype, shots=1024, seed=9, use_graph=True, fma_tiles=True, tile_m="auto")
         t_plan = time.perf_counter() - t0
+        chosen[dtype] = b.tile_m
         b.run()
         b.sync()
         t0 = time.perf_counter()
@@ -363,13 +366,16 @@ def batch(args) -> int:
         hcs[dtype] = hc
         out["results"][dtype] = {"s_per_run": dt, "instances_per_s": len(inst) / dt, "amp_updates_per_s": upd / dt,
                                  "plan_and_capture_s": t_plan, "launch_items": b.items,
+                                 "tile_m": b.tile_m,
+                                 "autotune_s_per_run_on_sample": {str(k): v for k, v in b.tile_timings.items()},
                                  "state_GiB": sum((1 << n) for n, _, _, _ in inst) * (16 if dtype == "f64" else 8) / 2 ** 30,
                                  "mean_expected_cut": float(np.mean([0.5 * (len(e) - h) for (n, e, _, _), h in zip(inst, hc)]))}
-        log(f"batch {dtype}: {dt * 1e3:.1f} ms/run, {len(inst) / dt:.0f} instances/s")
+        log(f"batch {dtype}: {dt * 1e3:.1f} ms/run, {len(inst) / dt:.0f} instances/s, tile width {b.tile_m} "
+            f"(autotuner sample: {', '.join(f'm={k} {v * 1e3:.1f} ms' for k, v in b.tile_timings.items())})")
         b.close()
     out["results"]["f32_max_abs_hc_diff_vs_f64"] = float(np.max(np.abs(hcs["f32"] - hcs["f64"])))
     # eager (uncaptured) FP64 run for the graph's launch-overhead saving
-    b, _, _ = qaoa_batch(inst, "f64", shots=1024, seed=9, use_graph=False, fma_tiles=True)
+    b, _, _ = qaoa_batch(inst, "f64", shots=1024, seed=9, use_graph=False, fma_tiles=True, tile_m=chosen["f64"])
     b.run()
     b.sync()
     t0 = time.perf_counter()

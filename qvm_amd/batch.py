@@ -6,7 +6,8 @@ list into LDS tile passes, hands them to the engine's batch object
 (qvmhip_batch_*), which schedules them level by level -- one batched tile
 kernel per level and tile width for all instances -- then computes each
 instance's MaxCut cost <H_C> = sum_edges <Z_a Z_b> and draws its shots, all
-replayed as one hipGraph.
+replayed as one hipGraph.  ``autotune_tile_m`` picks the tile width by timing
+candidate widths on a sample of the instances.
 """
 from __future__ import annotations
 
@@ -100,13 +101,61 @@ class Batch:
             pass
 
 
+def tile_candidates(dtype: str) -> List[int]:
+    """Tile widths the autotuner tries: the planner's default and one either side
+    (FP64 10..12, FP32 11..13; 13 is FP32-only)."""
+    m = P.default_tile_m(dtype)
+    return [x for x in (m - 1, m, m + 1) if 8 <= x <= (12 if dtype == "f64" else 13)]
+
+
+def autotune_tile_m(circuits: Sequence[Sequence], nqubits: Sequence[int], dtype: str = "f64",
+                    fma_tiles: bool = True, candidates: Sequence[int] | None = None, sample_every: int = 4,
+                    reps: int = 2):
+    """Pick the batch tile width by measurement: build the batch of every
+    ``sample_every``-th instance once per candidate width, time ``reps``
+    graph launches of each (after one warm-up) with the engine's own kernels,
+    and return (best width, {width: seconds per run}).  The sample keeps the
+    mix of qubit counts; a width above an instance's qubit count tiles that
+    instance whole, as the full batch will."""
+    import time
+    cands = list(candidates) if candidates else tile_candidates(dtype)
+    idx = list(range(0, len(circuits), max(1, int(sample_every))))
+    sc = [circuits[i] for i in idx]
+    sn = [nqubits[i] for i in idx]
+    timings = {}
+    for m in cands:
+        b = Batch(sc, sn, dtype, use_graph=True, fma_tiles=fma_tiles, tile_m=m)
+        try:
+            b.run()
+            b.sync()
+            t0 = time.perf_counter()
+            for _ in range(max(1, reps)):
+                b.run()
+            b.sync()
+            timings[m] = (time.perf_counter() - t0) / max(1, reps)
+        finally:
+            b.close()
+    best = min(timings, key=timings.get)
+    return best, timings
+
+
 def qaoa_batch(instances, dtype: str = "f64", shots: int = 1024, seed: int = 9, use_graph: bool = True,
-               fma_tiles: bool = False, tile_m: int = 0):
-    """Batch for (n, edges, gammas, betas) instances (circuits.qaoa_instances)."""
+               fma_tiles: bool = False, tile_m=0):
+    """Batch for (n, edges, gammas, betas) instances (circuits.qaoa_instances).
+    ``tile_m``: 0 for the planner's default width, a width, or "auto" for
+    the width ``autotune_tile_m`` measures fastest on a sample of the
+    instances (the Batch then carries ``tile_m`` and ``tile_timings``)."""
     circuits = [CI.qaoa_maxcut(n, e, g, b) for n, e, g, b in instances]
+    nq = [n for n, _, _, _ in instances]
+    timings = None
+    if tile_m == "auto":
+        tile_m, timings = autotune_tile_m(circuits, nq, dtype, fma_tiles=fma_tiles)
     u = CI.uniforms(seed, len(instances) * shots) if shots else None
-    return Batch(circuits, [n for n, _, _, _ in instances], dtype, edges=[e for _, e, _, _ in instances],
-                 shots=shots, uniforms=u, use_graph=use_graph, fma_tiles=fma_tiles, tile_m=tile_m), circuits, u
+    b = Batch(circuits, nq, dtype, edges=[e for _, e, _, _ in instances],
+              shots=shots, uniforms=u, use_graph=use_graph, fma_tiles=fma_tiles, tile_m=int(tile_m))
+    b.tile_m = int(tile_m) or P.default_tile_m(dtype)
+    b.tile_timings = timings
+    return b, circuits, u
 
 
 def expected_cut(nedges: int, hc: float) -> float:

@@ -90,3 +90,22 @@ def test_batch_initial_state_each_run(oracle, gate_first):
             ref = oracle_state(oracle, n, circuits[i])
             assert np.max(np.abs(b.amplitudes(i) - ref)) < 1e-13, (run, i)
     b.close()
+
+
+def test_tile_width_autotuner(oracle):
+    """The autotuner times every candidate width on a sample and the batch it
+    picks matches the oracle."""
+    from qvm_amd import batch as B
+    inst = CI.qaoa_instances(8, seed=6, nmin=10, nmax=14)
+    circuits = [CI.qaoa_maxcut(n, e, g, b) for n, e, g, b in inst]
+    best, timings = B.autotune_tile_m(circuits, [n for n, _, _, _ in inst], "f64", sample_every=2)
+    assert set(timings) == set(B.tile_candidates("f64")) and best in timings
+    assert all(t > 0 for t in timings.values())
+    b, circuits, _ = B.qaoa_batch(inst, "f64", shots=0, fma_tiles=True, tile_m="auto")
+    assert b.tile_m in timings and b.tile_timings is not None
+    b.run()
+    b.sync()
+    for i, (n, _, _, _) in enumerate(inst):
+        ref = oracle_state(oracle, n, circuits[i])
+        assert np.max(np.abs(b.amplitudes(i) - ref)) < 1e-13
+    b.close()
