@@ -176,7 +176,8 @@ int build_finish(qvmhip_batch *b) {
     BatchFinishArgs &f = b->fin;
     f.count = b->count;
     std::vector<uint64_t> len(b->count), chunk_base(b->count + 1, 0), blk_base(b->count + 1, 0);
-    std::vector<uint32_t> blk_inst, edge_off(b->count), nedges(b->count);
+    std::vector<uint32_t> blk_inst, edge_off(b->count), nedges(b->count), ecount(3 * (size_t)b->count, 0);
+    std::vector<uint32_t> edges;  // per instance: low endpoint first, grouped below / across / above the chunk bits
     bool any_edges = false;
     for (uint32_t i = 0; i < b->count; ++i) {
         len[i] = uint64_t(1) << b->nq[i];
@@ -184,9 +185,21 @@ int build_finish(qvmhip_batch *b) {
         const uint64_t nb = batch_finish_blocks(len[i]);
         blk_base[i + 1] = blk_base[i] + nb;
         blk_inst.insert(blk_inst.end(), (size_t)nb, i);
-        edge_off[i] = (uint32_t)(b->edge_off[i] / sizeof(uint32_t));
+        edge_off[i] = (uint32_t)edges.size();
         nedges[i] = (uint32_t)(b->edges[i].size() / 2);
         any_edges = any_edges || nedges[i] != 0;
+        const uint32_t lowb = (uint32_t)kChunkLog2;
+        for (int group = 0; group < 3; ++group) {
+            for (uint32_t e = 0; e < nedges[i]; ++e) {
+                uint32_t x = b->edges[i][2 * e], y = b->edges[i][2 * e + 1];
+                if (x > y) std::swap(x, y);
+                const int g = (y < lowb) ? 0 : (x < lowb ? 1 : 2);
+                if (g != group) continue;
+                edges.push_back(x);
+                edges.push_back(y);
+                ++ecount[3 * (size_t)i + g];
+            }
+        }
     }
     f.blocks = blk_base[b->count];
     if (!any_edges && !b->shots) return QVMHIP_OK;
@@ -212,7 +225,10 @@ int build_finish(qvmhip_batch *b) {
     f.edge_off = (const uint32_t *)d;
     if ((rc = up(nedges.data(), 4 * nedges.size(), &d))) return rc;
     f.nedges = (const uint32_t *)d;
-    f.edges = b->edges_dev;
+    if ((rc = up(edges.data(), 4 * edges.size(), &d))) return rc;
+    f.edges = (const uint32_t *)d;
+    if ((rc = up(ecount.data(), 4 * ecount.size(), &d))) return rc;
+    f.ecount = (const uint32_t *)d;
     if (any_edges) {
         if ((rc = up(nullptr, 8 * f.blocks, &d))) return rc;
         f.zz_part = (double *)d;

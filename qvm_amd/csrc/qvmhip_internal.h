@@ -198,6 +198,10 @@ struct BatchFinishArgs {
     uint64_t blocks;
     const uint64_t *inst_off, *inst_len, *chunk_base, *blk_base;
     const uint32_t *blk_inst, *edges, *edge_off, *nedges;
+    // per instance: its edges are stored low endpoint first and grouped as
+    // [both endpoints below the chunk bits | one below, one above | both
+    // above]; ecount[3 i + 0..2] counts the groups
+    const uint32_t *ecount;
     double *zz_part, *exp_out, *chunk_sum, *chunk_pre;
     const double *u;
     uint64_t shots;

@@ -418,6 +418,23 @@ __global__ __launch_bounds__(kBlock) void k_bfinish_partial(const vec2_t<T> *__r
         const vec2_t<T> *psi = buf + a.inst_off[i];
         const uint64_t len = a.inst_len[i];
         const uint64_t nch = a.chunk_base[i + 1] - a.chunk_base[i];
+        // cut(x) for x = chunk * 256 + lo: edges with both endpoints in the
+        // low 8 bits depend on lo only (win), edges with both above on the
+        // chunk only (hh), and the cross edges are popcount(A(lo) ^ B(chunk))
+        // with one bit per edge -- per amplitude a XOR and a popcount
+        // instead of a loop over the edges
+        const uint32_t nll = a.ecount[3 * i], ncr = a.ecount[3 * i + 1];
+        const bool fast = ncr <= 64;
+        int win[4] = {0, 0, 0, 0};
+        uint64_t A[4] = {0, 0, 0, 0};
+        if (ne && fast) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t lo = (uint32_t)lane * 4 + k;
+                for (uint32_t e = 0; e < nll; ++e) win[k] += (int)(((lo >> se[2 * e]) ^ (lo >> se[2 * e + 1])) & 1u);
+                for (uint32_t j = 0; j < ncr; ++j) A[k] |= (uint64_t)((lo >> se[2 * (nll + j)]) & 1u) << j;
+            }
+        }
         double acc = 0.0;
 #pragma unroll 1
         for (int k4 = 0; k4 < 4; ++k4) {
@@ -429,7 +446,21 @@ __global__ __launch_bounds__(kBlock) void k_bfinish_partial(const vec2_t<T> *__r
                 const double incl = wave_inclusive_scan(((p[0] + p[1]) + p[2]) + p[3], lane);
                 if (lane == kWave - 1) a.chunk_sum[a.chunk_base[i] + c] = incl;
             }
-            if (ne) {
+            if (ne && fast) {
+                // the chunk part: one mask of the cross edges' upper bits and
+                // the count of cut edges above the chunk bits (uniform)
+                uint64_t B = 0;
+                int hh = 0;
+                for (uint32_t j = 0; j < ncr; ++j)
+                    B |= (uint64_t)((c >> (se[2 * (nll + j) + 1] - kChunkLog2)) & 1u) << j;
+                for (uint32_t e = nll + ncr; e < ne; ++e)
+                    hh += (int)(((c >> (se[2 * e] - kChunkLog2)) ^ (c >> (se[2 * e + 1] - kChunkLog2))) & 1u);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int cut = win[k] + hh + __popcll(A[k] ^ B);
+                    acc += p[k] * (double)((int)ne - 2 * cut);
+                }
+            } else if (ne) {
                 const uint64_t x0 = (c << kChunkLog2) + (uint64_t)lane * 4;
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
