@@ -1,6 +1,5 @@
 """The learned fusion cost model: its CPU oracle, its GPU kernels, and what
 the trained model gains over the planner without it."""
-import math
 
 import numpy as np
 import pytest
