@@ -129,11 +129,8 @@ int qvmhip_create(uint32_t n, int dtype, const qvmhip_config *cfg, qvmhip_state 
         return fail(QVMHIP_EDEVICE, "no HIP device available");
     int dev = (cfg && cfg->device >= 0) ? cfg->device : -1;
     if (dev >= ndev) return fail(QVMHIP_EINVAL, "device ordinal out of range");
-    if (dev < 0) {
-        QH_CHECK(hipGetDevice(&dev));
-    } else {
-        QH_CHECK(hipSetDevice(dev));
-    }
+    if (dev < 0) QH_CHECK(hipGetDevice(&dev));
+    DeviceScope ds(dev);  // the caller's current device is left as it was
     qvmhip_state *s = new qvmhip_state();
     s->n = n;
     s->dtype = dtype;
@@ -178,7 +175,7 @@ int qvmhip_create(uint32_t n, int dtype, const qvmhip_config *cfg, qvmhip_state 
 
 int qvmhip_destroy(qvmhip_state *s) {
     if (!s) return QVMHIP_OK;
-    hipSetDevice(s->device);
+    DeviceScope ds(s->device);
     hipStreamSynchronize(s->stream);
     for (qvmhip_program *p : s->programs) program_detach(p);
     free_state(s);
@@ -199,11 +196,13 @@ int qvmhip_dtype(const qvmhip_state *s, int *dtype) {
 
 int qvmhip_reset_zero(qvmhip_state *s) {
     if (!s) return fail(QVMHIP_EINVAL, "null state");
+    DeviceScope ds(s->device);
     return launch_fill_zero_state(s);
 }
 
 int qvmhip_sync(qvmhip_state *s) {
     if (!s) return fail(QVMHIP_EINVAL, "null state");
+    DeviceScope ds(s->device);
     QH_CHECK(hipStreamSynchronize(s->stream));
     return QVMHIP_OK;
 }
@@ -211,6 +210,7 @@ int qvmhip_sync(qvmhip_state *s) {
 int qvmhip_apply_matrix(qvmhip_state *s, const double *U, uint32_t k, const uint32_t *qubits) {
     int rc = check_qubits(s, k, qubits);
     if (rc) return rc;
+    DeviceScope ds(s->device);
     if (!U || !finite_matrix(U, k)) return fail(QVMHIP_EINVAL, "matrix is null or not finite");
     return launch_gate(s, U, k, qubits, 0);
 }
@@ -218,6 +218,7 @@ int qvmhip_apply_matrix(qvmhip_state *s, const double *U, uint32_t k, const uint
 int qvmhip_apply_matrix_dense(qvmhip_state *s, const double *U, uint32_t k, const uint32_t *qubits) {
     int rc = check_qubits(s, k, qubits);
     if (rc) return rc;
+    DeviceScope ds(s->device);
     if (!U || !finite_matrix(U, k)) return fail(QVMHIP_EINVAL, "matrix is null or not finite");
     return launch_gate(s, U, k, qubits, 1);
 }
@@ -225,6 +226,7 @@ int qvmhip_apply_matrix_dense(qvmhip_state *s, const double *U, uint32_t k, cons
 int qvmhip_apply_diagonal(qvmhip_state *s, const double *d, uint32_t k, const uint32_t *qubits) {
     int rc = check_qubits(s, k, qubits);
     if (rc) return rc;
+    DeviceScope ds(s->device);
     if (!d) return fail(QVMHIP_EINVAL, "null diagonal");
     for (uint32_t i = 0; i < (2u << k); ++i)
         if (!std::isfinite(d[i])) return fail(QVMHIP_EINVAL, "diagonal not finite");
@@ -234,6 +236,7 @@ int qvmhip_apply_diagonal(qvmhip_state *s, const double *d, uint32_t k, const ui
 int qvmhip_apply_permutation(qvmhip_state *s, const uint32_t *perm, uint32_t k, const uint32_t *qubits) {
     int rc = check_qubits(s, k, qubits);
     if (rc) return rc;
+    DeviceScope ds(s->device);
     if (!perm) return fail(QVMHIP_EINVAL, "null permutation");
     return launch_permutation(s, perm, k, qubits);
 }
@@ -241,6 +244,7 @@ int qvmhip_apply_permutation(qvmhip_state *s, const uint32_t *perm, uint32_t k, 
 int qvmhip_apply_fused(qvmhip_state *s, const double *U, uint32_t k, const uint32_t *qubits) {
     int rc = check_qubits(s, k, qubits);
     if (rc) return rc;
+    DeviceScope ds(s->device);
     if (k < 3) return fail(QVMHIP_EINVAL, "fused blocks are 3..6 qubits");
     if (!U || !finite_matrix(U, k)) return fail(QVMHIP_EINVAL, "matrix is null or not finite");
     // upload to the matrix slot's second half (first half: ops / dense)
@@ -253,6 +257,7 @@ int qvmhip_apply_fused(qvmhip_state *s, const double *U, uint32_t k, const uint3
 int qvmhip_apply_tile_pass(qvmhip_state *s, uint32_t m, const uint32_t *tile_qubits, uint32_t nops,
                            const qvmhip_tile_op *ops) {
     if (!s) return fail(QVMHIP_EINVAL, "null state");
+    DeviceScope ds(s->device);
     if (m > s->n) return fail(QVMHIP_EINVAL, "tile wider than the state");
     if (!tile_qubits || (nops && !ops)) return fail(QVMHIP_EINVAL, "null tile arguments");
     for (uint32_t i = 0; i < m; ++i) {
@@ -283,6 +288,7 @@ int qvmhip_tile_schedule(int dtype, uint32_t flags, uint32_t m, uint32_t nops, c
 
 int qvmhip_prob_excited(qvmhip_state *s, uint32_t q, double *p1) {
     if (!s || !p1) return fail(QVMHIP_EINVAL, "null argument");
+    DeviceScope ds(s->device);
     if (q >= s->n) return fail(QVMHIP_EINVAL, "qubit out of range");
     int rc = reduce_probability(s, 1, q, s->red + kRedBlocks * 2);
     if (rc) return rc;
@@ -291,6 +297,7 @@ int qvmhip_prob_excited(qvmhip_state *s, uint32_t q, double *p1) {
 
 int qvmhip_prob_ground(qvmhip_state *s, uint32_t q, double *p0) {
     if (!s || !p0) return fail(QVMHIP_EINVAL, "null argument");
+    DeviceScope ds(s->device);
     if (q >= s->n) return fail(QVMHIP_EINVAL, "qubit out of range");
     int rc = reduce_probability(s, 2, q, s->red + kRedBlocks * 2);
     if (rc) return rc;
@@ -299,6 +306,7 @@ int qvmhip_prob_ground(qvmhip_state *s, uint32_t q, double *p0) {
 
 int qvmhip_norm(qvmhip_state *s, double *out) {
     if (!s || !out) return fail(QVMHIP_EINVAL, "null argument");
+    DeviceScope ds(s->device);
     double n2 = 0;
     int rc = reduce_norm2(s, s->red + kRedBlocks * 2);
     if (rc) return rc;
@@ -310,6 +318,7 @@ int qvmhip_norm(qvmhip_state *s, double *out) {
 
 int qvmhip_total_probability(qvmhip_state *s, double *out) {
     if (!s || !out) return fail(QVMHIP_EINVAL, "null argument");
+    DeviceScope ds(s->device);
     int rc = reduce_norm2(s, s->red + kRedBlocks * 2);
     if (rc) return rc;
     return scalar_to_host(s, s->red + kRedBlocks * 2, out);
@@ -317,6 +326,7 @@ int qvmhip_total_probability(qvmhip_state *s, double *out) {
 
 int qvmhip_normalize(qvmhip_state *s, double length) {
     if (!s) return fail(QVMHIP_EINVAL, "null state");
+    DeviceScope ds(s->device);
     double len = length;
     if (!(len > 0)) {
         int rc = qvmhip_norm(s, &len);
@@ -328,6 +338,7 @@ int qvmhip_normalize(qvmhip_state *s, double length) {
 
 int qvmhip_probabilities(qvmhip_state *s, uint64_t start, uint64_t count, double *out) {
     if (!s || (count && !out)) return fail(QVMHIP_EINVAL, "null argument");
+    DeviceScope ds(s->device);
     if (start > s->len || count > s->len - start) return fail(QVMHIP_EINVAL, "range out of bounds");
     if (!count) return QVMHIP_OK;
     return probabilities(s, start, count, out);
@@ -335,6 +346,7 @@ int qvmhip_probabilities(qvmhip_state *s, uint64_t start, uint64_t count, double
 
 int qvmhip_expectation_zz(qvmhip_state *s, const uint32_t *edges, uint32_t nedges, double *out) {
     if (!s || !out || (nedges && !edges)) return fail(QVMHIP_EINVAL, "null argument");
+    DeviceScope ds(s->device);
     for (uint32_t i = 0; i < 2 * nedges; ++i)
         if (edges[i] >= s->n) return fail(QVMHIP_EINVAL, "edge qubit out of range");
     return expectation_zz(s, edges, nedges, out);
@@ -342,6 +354,7 @@ int qvmhip_expectation_zz(qvmhip_state *s, const uint32_t *edges, uint32_t nedge
 
 int qvmhip_inner_product(qvmhip_state *a, qvmhip_state *b, double *re, double *im) {
     if (!a || !b || !re || !im) return fail(QVMHIP_EINVAL, "null argument");
+    DeviceScope ds(a->device);
     if (a->n != b->n) return fail(QVMHIP_EINVAL, "states differ in size");
     if (a != b) {
         // b's queued work must finish before a's stream reads it
@@ -352,6 +365,7 @@ int qvmhip_inner_product(qvmhip_state *a, qvmhip_state *b, double *re, double *i
 
 int qvmhip_max_abs_diff(qvmhip_state *a, qvmhip_state *b, double *out) {
     if (!a || !b || !out) return fail(QVMHIP_EINVAL, "null argument");
+    DeviceScope ds(a->device);
     if (a->n != b->n) return fail(QVMHIP_EINVAL, "states differ in size");
     if (a != b) QH_CHECK(hipStreamSynchronize(b->stream));
     return max_abs_diff(a, b, out);
@@ -359,6 +373,7 @@ int qvmhip_max_abs_diff(qvmhip_state *a, qvmhip_state *b, double *out) {
 
 int qvmhip_tv_distance(qvmhip_state *a, qvmhip_state *b, double *out) {
     if (!a || !b || !out) return fail(QVMHIP_EINVAL, "null argument");
+    DeviceScope ds(a->device);
     if (a->n != b->n) return fail(QVMHIP_EINVAL, "states differ in size");
     if (a != b) QH_CHECK(hipStreamSynchronize(b->stream));
     return tv_distance(a, b, out);
@@ -366,6 +381,7 @@ int qvmhip_tv_distance(qvmhip_state *a, qvmhip_state *b, double *out) {
 
 int qvmhip_profile_enable(qvmhip_state *s, int on) {
     if (!s) return fail(QVMHIP_EINVAL, "null state");
+    DeviceScope ds(s->device);
     QH_CHECK(hipStreamSynchronize(s->stream));
     for (auto &r : s->prof) {
         if (r.a) s->event_pool.push_back(r.a);
@@ -378,6 +394,7 @@ int qvmhip_profile_enable(qvmhip_state *s, int on) {
 
 int qvmhip_profile_summary(qvmhip_state *s, int kind, uint64_t *count, double *total_ms) {
     if (!s || !count || !total_ms) return fail(QVMHIP_EINVAL, "null argument");
+    DeviceScope ds(s->device);
     QH_CHECK(hipStreamSynchronize(s->stream));
     uint64_t c = 0;
     double t = 0.0;
@@ -395,16 +412,22 @@ int qvmhip_profile_summary(qvmhip_state *s, int kind, uint64_t *count, double *t
 
 int qvmhip_force_measure(qvmhip_state *s, uint32_t q, int bit, double p1) {
     if (!s) return fail(QVMHIP_EINVAL, "null state");
+    DeviceScope ds(s->device);
     if (q >= s->n) return fail(QVMHIP_EINVAL, "qubit out of range");
     if (bit != 0 && bit != 1) return fail(QVMHIP_EINVAL, "bit must be 0 or 1");
     if (!(p1 >= 0.0 && p1 <= 1.0)) return fail(QVMHIP_ERANGE, "excited probability outside [0,1]");
     // src/measurement.lisp:18-21: 1/sqrt(p1) for bit 1, 1/sqrt(1 - p1) for bit 0
     const double inv_norm = bit == 1 ? 1.0 / std::sqrt(p1) : 1.0 / std::sqrt(1.0 - p1);
+    // the reference's (/ (sqrt 0)) signals division-by-zero there
+    if (!std::isfinite(inv_norm))
+        return fail(QVMHIP_ERANGE, "cannot force qubit " + std::to_string(q) + " to " + std::to_string(bit) +
+                                       ": that outcome has probability 0");
     return launch_force_measure(s, q, bit, inv_norm, 0);
 }
 
 int qvmhip_measure(qvmhip_state *s, uint32_t q, double u, int compiled, int *bit, int *consumed) {
     if (!s || !bit || !consumed) return fail(QVMHIP_EINVAL, "null argument");
+    DeviceScope ds(s->device);
     if (q >= s->n) return fail(QVMHIP_EINVAL, "Trying to measure qubit " + std::to_string(q) + " on a QVM with only " +
                                                  std::to_string(s->n) + " qubits");
     if (!(u >= 0.0 && u <= 1.0)) return fail(QVMHIP_ERANGE, "uniform draw outside [0,1]");
@@ -431,7 +454,9 @@ int qvmhip_measure(qvmhip_state *s, uint32_t q, double u, int compiled, int *bit
     int rc = qvmhip_prob_ground(s, q, &p0);
     if (rc) return rc;
     *consumed = 1;
-    const bool keep_zero = u < p0;
+    // the reference draws u in [0, 1), so u < p0 always holds when p0 >= 1;
+    // keeping that for u = 1 avoids a 1/sqrt(1 - p0) of 0 or less
+    const bool keep_zero = u < p0 || p0 >= 1.0;
     const int b = keep_zero ? 0 : 1;
     const double inv_norm = keep_zero ? 1.0 / std::sqrt(p0) : 1.0 / std::sqrt(1.0 - p0);
     rc = launch_force_measure(s, q, b, inv_norm, 1);
@@ -442,6 +467,7 @@ int qvmhip_measure(qvmhip_state *s, uint32_t q, double u, int compiled, int *bit
 
 int qvmhip_measure_all(qvmhip_state *s, double u, uint64_t *index) {
     if (!s || !index) return fail(QVMHIP_EINVAL, "null argument");
+    DeviceScope ds(s->device);
     if (!(u >= 0.0 && u <= 1.0)) return fail(QVMHIP_ERANGE, "uniform draw outside [0,1]");
     int rc = sample_shots(s, &u, 1, nullptr, 0);
     if (rc) return rc;
@@ -461,6 +487,7 @@ static int check_draws(const double *u, uint64_t nshots) {
 
 int qvmhip_sample_measure_all(qvmhip_state *s, const double *u, uint64_t nshots, uint64_t *out) {
     if (!s || (nshots && (!u || !out))) return fail(QVMHIP_EINVAL, "null argument");
+    DeviceScope ds(s->device);
     int rc = check_draws(u, nshots);
     if (rc) return rc;
     return sample_shots(s, u, nshots, out, 0);
@@ -468,6 +495,7 @@ int qvmhip_sample_measure_all(qvmhip_state *s, const double *u, uint64_t nshots,
 
 int qvmhip_sample(qvmhip_state *s, const double *u, uint64_t nshots, uint64_t *out) {
     if (!s || (nshots && (!u || !out))) return fail(QVMHIP_EINVAL, "null argument");
+    DeviceScope ds(s->device);
     int rc = check_draws(u, nshots);
     if (rc) return rc;
     return sample_shots(s, u, nshots, out, 1);
@@ -475,6 +503,7 @@ int qvmhip_sample(qvmhip_state *s, const double *u, uint64_t nshots, uint64_t *o
 
 int qvmhip_read_amplitudes(qvmhip_state *s, uint64_t start, uint64_t count, double *out) {
     if (!s || (count && !out)) return fail(QVMHIP_EINVAL, "null argument");
+    DeviceScope ds(s->device);
     if (start > s->len || count > s->len - start) return fail(QVMHIP_EINVAL, "range out of bounds");
     if (!count) return QVMHIP_OK;
     return export_range(s, start, count, kExportAmps, out);
@@ -482,6 +511,7 @@ int qvmhip_read_amplitudes(qvmhip_state *s, uint64_t start, uint64_t count, doub
 
 int qvmhip_read_wavefunction_be(qvmhip_state *s, uint64_t start, uint64_t count, uint8_t *out) {
     if (!s || (count && !out)) return fail(QVMHIP_EINVAL, "null argument");
+    DeviceScope ds(s->device);
     if (start > s->len || count > s->len - start) return fail(QVMHIP_EINVAL, "range out of bounds");
     if (!count) return QVMHIP_OK;
     return export_range(s, start, count, kExportAmpsBE, out);
@@ -489,6 +519,7 @@ int qvmhip_read_wavefunction_be(qvmhip_state *s, uint64_t start, uint64_t count,
 
 int qvmhip_read_probabilities_be(qvmhip_state *s, uint64_t start, uint64_t count, uint8_t *out) {
     if (!s || (count && !out)) return fail(QVMHIP_EINVAL, "null argument");
+    DeviceScope ds(s->device);
     if (start > s->len || count > s->len - start) return fail(QVMHIP_EINVAL, "range out of bounds");
     if (!count) return QVMHIP_OK;
     return export_range(s, start, count, kExportProbsBE, out);
@@ -496,11 +527,23 @@ int qvmhip_read_probabilities_be(qvmhip_state *s, uint64_t start, uint64_t count
 
 int qvmhip_write_amplitudes(qvmhip_state *s, uint64_t start, uint64_t count, const double *in) {
     if (!s || (count && !in)) return fail(QVMHIP_EINVAL, "null argument");
+    DeviceScope ds(s->device);
     if (start > s->len || count > s->len - start) return fail(QVMHIP_EINVAL, "range out of bounds");
     if (!count) return QVMHIP_OK;
     const uint64_t per = s->stage_bytes / 16;
-    double *wide = nullptr;
-    if (s->dtype == QVMHIP_F32) QH_CHECK(hipMalloc((void **)&wide, per * 16));
+    struct Wide {
+        double *p = nullptr;
+        hipStream_t st = nullptr;
+        ~Wide() {
+            if (p) {
+                hipStreamSynchronize(st);
+                hipFree(p);
+            }
+        }
+    } wide_buf;
+    wide_buf.st = s->stream;
+    if (s->dtype == QVMHIP_F32) QH_CHECK(hipMalloc((void **)&wide_buf.p, per * 16));
+    double *wide = wide_buf.p;
     for (uint64_t off = 0; off < count; off += per) {
         const uint64_t c = std::min<uint64_t>(per, count - off);
         QH_CHECK(hipStreamSynchronize(s->stream));  // stage is free again
@@ -511,36 +554,40 @@ int qvmhip_write_amplitudes(qvmhip_state *s, uint64_t start, uint64_t count, con
         } else {
             QH_CHECK(hipMemcpyAsync(wide, s->stage, c * 16, hipMemcpyHostToDevice, s->stream));
             int rc = convert_f64_to_f32(s, start + off, c, wide);
-            if (rc) {
-                hipFree(wide);
-                return rc;
-            }
+            if (rc) return rc;
         }
     }
     QH_CHECK(hipStreamSynchronize(s->stream));
-    if (wide) hipFree(wide);
     return QVMHIP_OK;
 }
 
 int qvmhip_accumulate(qvmhip_state *dst, const qvmhip_state *src) {
     if (!dst || !src) return fail(QVMHIP_EINVAL, "null state");
+    DeviceScope ds(dst->device);
     if (dst->n != src->n || dst->dtype != src->dtype) return fail(QVMHIP_EINVAL, "states differ in size or dtype");
-    if (dst != src) QH_CHECK(hipStreamSynchronize(src->stream));
-    return accumulate(dst, src);
+    if (dst == src) return accumulate(dst, src);
+    QH_CHECK(stream_after(dst->stream, src->stream));  // src's queued work first
+    int rc = accumulate(dst, src);
+    if (rc) return rc;
+    QH_CHECK(stream_after(src->stream, dst->stream));  // src's next call after the read
+    return QVMHIP_OK;
 }
 
 int qvmhip_density_diagonal(qvmhip_state *s, uint32_t nq, double *out) {
     if (!s || !out) return fail(QVMHIP_EINVAL, "null argument");
+    DeviceScope ds(s->device);
     if (2 * nq != s->n) return fail(QVMHIP_EINVAL, "a density state of nq qubits has 2*nq qubits");
     return density_diagonal(s, nq, out);
 }
 
 int qvmhip_copy(qvmhip_state *dst, const qvmhip_state *src) {
     if (!dst || !src) return fail(QVMHIP_EINVAL, "null state");
+    DeviceScope ds(dst->device);
     if (dst->n != src->n || dst->dtype != src->dtype) return fail(QVMHIP_EINVAL, "states differ in size or dtype");
     if (dst == src) return QVMHIP_OK;
-    QH_CHECK(hipStreamSynchronize(src->stream));
+    QH_CHECK(stream_after(dst->stream, src->stream));  // src's queued work first
     QH_CHECK(hipMemcpyAsync(dst->amps, src->amps, dst->len * dst->amp_bytes, hipMemcpyDeviceToDevice, dst->stream));
+    QH_CHECK(stream_after(src->stream, dst->stream));  // src's next call after the copy
     return QVMHIP_OK;
 }
 

@@ -110,6 +110,7 @@ struct qvmhip_batch {
     bool finalized = false, use_graph = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    int device = 0;  // the caller's current device at creation
 };
 
 namespace {
@@ -254,7 +255,10 @@ int qvmhip_batch_create(uint32_t count, const uint32_t *nqubits, int dtype, qvmh
     if (!out || !nqubits || count == 0) return fail(QVMHIP_EINVAL, "bad argument");
     if (dtype != QVMHIP_F64 && dtype != QVMHIP_F32) return fail(QVMHIP_EINVAL, "dtype must be F64 or F32");
     *out = nullptr;
+    int dev = 0;
+    QH_CHECK(hipGetDevice(&dev));
     qvmhip_batch *b = new qvmhip_batch();
+    b->device = dev;
     b->count = count;
     b->dtype = dtype;
     b->amp_bytes = dtype == QVMHIP_F64 ? 16 : 8;
@@ -285,16 +289,18 @@ int qvmhip_batch_create(uint32_t count, const uint32_t *nqubits, int dtype, qvmh
     if ((e = hipMalloc((void **)&b->scan_tmp, sizeof(double) * nseg)) != hipSuccess) return bail("scan");
     if ((e = hipMalloc(&b->mat, 128 * 1024)) != hipSuccess) return bail("matrix slot");
     if ((e = hipMalloc((void **)&b->exp_dev, sizeof(double) * count)) != hipSuccess) return bail("expectations");
-    if ((e = hipMemset(b->exp_dev, 0, sizeof(double) * count)) != hipSuccess) return bail("memset");
+    if ((e = hipMemsetAsync(b->exp_dev, 0, sizeof(double) * count, b->stream)) != hipSuccess) return bail("memset");
     if ((e = hipMalloc((void **)&b->starts_dev, sizeof(uint64_t) * count)) != hipSuccess) return bail("starts");
-    if ((e = hipMemcpy(b->starts_dev, b->offset.data(), sizeof(uint64_t) * count, hipMemcpyHostToDevice)) !=
-        hipSuccess)
+    if ((e = hipMemcpyAsync(b->starts_dev, b->offset.data(), sizeof(uint64_t) * count, hipMemcpyHostToDevice,
+                            b->stream)) != hipSuccess)
         return bail("starts upload");
+    if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) return bail("upload");
     b->views.resize(count);
     for (uint32_t i = 0; i < count; ++i) {
         qvmhip_state &v = b->views[i];
         v.n = b->nq[i];
         v.dtype = dtype;
+        v.device = dev;
         v.len = uint64_t(1) << v.n;
         v.amp_bytes = b->amp_bytes;
         v.amps = (char *)b->buf + b->offset[i] * b->amp_bytes;
@@ -392,6 +398,7 @@ int qvmhip_batch_set_shots(qvmhip_batch *b, uint64_t shots, const double *u) {
 
 int qvmhip_batch_finalize(qvmhip_batch *b, int use_graph) {
     if (!b) return fail(QVMHIP_EINVAL, "null batch");
+    DeviceScope ds(b->device);
     if (b->finalized) return fail(QVMHIP_EINVAL, "batch already finalized");
     hipError_t e;
     if (!b->blob.empty()) {
@@ -486,6 +493,7 @@ int qvmhip_batch_finalize(qvmhip_batch *b, int use_graph) {
 
 int qvmhip_batch_run(qvmhip_batch *b) {
     if (!b) return fail(QVMHIP_EINVAL, "null batch");
+    DeviceScope ds(b->device);
     if (!b->finalized) return fail(QVMHIP_EINVAL, "batch not finalized");
     if (b->use_graph) {
         QH_CHECK(hipGraphLaunch(b->exec, b->stream));
@@ -496,12 +504,14 @@ int qvmhip_batch_run(qvmhip_batch *b) {
 
 int qvmhip_batch_sync(qvmhip_batch *b) {
     if (!b) return fail(QVMHIP_EINVAL, "null batch");
+    DeviceScope ds(b->device);
     QH_CHECK(hipStreamSynchronize(b->stream));
     return QVMHIP_OK;
 }
 
 int qvmhip_batch_results(qvmhip_batch *b, double *expectations, uint64_t *shots) {
     if (!b) return fail(QVMHIP_EINVAL, "null batch");
+    DeviceScope ds(b->device);
     QH_CHECK(hipStreamSynchronize(b->stream));
     if (expectations) QH_CHECK(hipMemcpy(expectations, b->exp_dev, sizeof(double) * b->count, hipMemcpyDeviceToHost));
     if (shots && b->shots)
@@ -511,6 +521,7 @@ int qvmhip_batch_results(qvmhip_batch *b, double *expectations, uint64_t *shots)
 
 int qvmhip_batch_read_amplitudes(qvmhip_batch *b, uint32_t inst, uint64_t start, uint64_t count, double *out) {
     if (!b || inst >= b->count || (count && !out)) return fail(QVMHIP_EINVAL, "bad argument");
+    DeviceScope ds(b->device);
     const uint64_t len = uint64_t(1) << b->nq[inst];
     if (start > len || count > len - start) return fail(QVMHIP_EINVAL, "range out of bounds");
     QH_CHECK(hipStreamSynchronize(b->stream));
@@ -527,6 +538,7 @@ int qvmhip_batch_read_amplitudes(qvmhip_batch *b, uint32_t inst, uint64_t start,
 
 int qvmhip_batch_destroy(qvmhip_batch *b) {
     if (!b) return QVMHIP_OK;
+    DeviceScope ds(b->device);
     hipStreamSynchronize(b->stream);
     free_batch(b);
     return QVMHIP_OK;

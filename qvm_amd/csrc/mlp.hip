@@ -135,6 +135,7 @@ struct qvmhip_mlp {
     float *X = nullptr, *T = nullptr, *G = nullptr, *lossb = nullptr, *out = nullptr;
     int cap = 0;
     int step = 0;
+    int device = 0;  // the caller's current device at creation
 };
 
 namespace {
@@ -157,19 +158,26 @@ extern "C" {
 int qvmhip_mlp_create(int din, int hidden, const float *init_params, qvmhip_mlp **out) {
     if (!out || din < 1 || din > kMaxIn || hidden < 1 || hidden > kMaxH)
         return fail(QVMHIP_EINVAL, "mlp: din must be 1..32 and hidden 1..64");
+    *out = nullptr;
     qvmhip_mlp *h = new qvmhip_mlp();
     h->S = make_shape(din, hidden);
-    QH_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    hipError_t e = hipGetDevice(&h->device);
     const size_t bytes = sizeof(float) * h->S.np;
-    QH_CHECK(hipMalloc((void **)&h->p, bytes));
-    QH_CHECK(hipMalloc((void **)&h->m, bytes));
-    QH_CHECK(hipMalloc((void **)&h->v, bytes));
-    QH_CHECK(hipMalloc((void **)&h->grad, bytes));
-    QH_CHECK(hipMalloc((void **)&h->loss1, sizeof(float)));
-    if (init_params) QH_CHECK(hipMemcpy(h->p, init_params, bytes, hipMemcpyHostToDevice));
-    else QH_CHECK(hipMemset(h->p, 0, bytes));
-    QH_CHECK(hipMemset(h->m, 0, bytes));
-    QH_CHECK(hipMemset(h->v, 0, bytes));
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    for (float **q : {&h->p, &h->m, &h->v, &h->grad})
+        if (e == hipSuccess) e = hipMalloc((void **)q, bytes);
+    if (e == hipSuccess) e = hipMalloc((void **)&h->loss1, sizeof(float));
+    if (e == hipSuccess)
+        e = init_params ? hipMemcpyAsync(h->p, init_params, bytes, hipMemcpyHostToDevice, h->stream)
+                        : hipMemsetAsync(h->p, 0, bytes, h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(h->m, 0, bytes, h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(h->v, 0, bytes, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) {
+        qvmhip_mlp_destroy(h);
+        return fail(e == hipErrorOutOfMemory ? QVMHIP_ENOMEM : QVMHIP_EDEVICE,
+                    std::string("mlp create: ") + hipGetErrorString(e));
+    }
     *out = h;
     return QVMHIP_OK;
 }
@@ -182,23 +190,26 @@ int qvmhip_mlp_num_params(const qvmhip_mlp *h, int *np) {
 
 int qvmhip_mlp_get_params(qvmhip_mlp *h, float *out) {
     if (!h || !out) return fail(QVMHIP_EINVAL, "null argument");
+    DeviceScope ds(h->device);
+    QH_CHECK(hipMemcpyAsync(out, h->p, sizeof(float) * h->S.np, hipMemcpyDeviceToHost, h->stream));
     QH_CHECK(hipStreamSynchronize(h->stream));
-    QH_CHECK(hipMemcpy(out, h->p, sizeof(float) * h->S.np, hipMemcpyDeviceToHost));
     return QVMHIP_OK;
 }
 
 int qvmhip_mlp_set_params(qvmhip_mlp *h, const float *in) {
     if (!h || !in) return fail(QVMHIP_EINVAL, "null argument");
-    QH_CHECK(hipStreamSynchronize(h->stream));
-    QH_CHECK(hipMemcpy(h->p, in, sizeof(float) * h->S.np, hipMemcpyHostToDevice));
+    DeviceScope ds(h->device);
+    QH_CHECK(hipMemcpyAsync(h->p, in, sizeof(float) * h->S.np, hipMemcpyHostToDevice, h->stream));
     h->step = 0;
-    QH_CHECK(hipMemset(h->m, 0, sizeof(float) * h->S.np));
-    QH_CHECK(hipMemset(h->v, 0, sizeof(float) * h->S.np));
+    QH_CHECK(hipMemsetAsync(h->m, 0, sizeof(float) * h->S.np, h->stream));
+    QH_CHECK(hipMemsetAsync(h->v, 0, sizeof(float) * h->S.np, h->stream));
+    QH_CHECK(hipStreamSynchronize(h->stream));
     return QVMHIP_OK;
 }
 
 int qvmhip_mlp_predict(qvmhip_mlp *h, const float *X, int nb, float *out) {
     if (!h || nb < 0 || (nb && (!X || !out))) return fail(QVMHIP_EINVAL, "null argument");
+    DeviceScope ds(h->device);
     if (!nb) return QVMHIP_OK;
     int rc = ensure_batch(h, nb);
     if (rc) return rc;
@@ -213,6 +224,7 @@ int qvmhip_mlp_predict(qvmhip_mlp *h, const float *X, int nb, float *out) {
 // Gradient of the mean of 0.5*(y - t)^2 at the current parameters.
 int qvmhip_mlp_gradient(qvmhip_mlp *h, const float *X, const float *T, int nb, float *grad_out, float *loss_out) {
     if (!h || nb < 1 || !X || !T || !grad_out) return fail(QVMHIP_EINVAL, "bad argument");
+    DeviceScope ds(h->device);
     int rc = ensure_batch(h, nb);
     if (rc) return rc;
     QH_CHECK(hipMemcpyAsync(h->X, X, sizeof(float) * nb * h->S.din, hipMemcpyHostToDevice, h->stream));
@@ -235,6 +247,7 @@ int qvmhip_mlp_gradient(qvmhip_mlp *h, const float *X, const float *T, int nb, f
 int qvmhip_mlp_train(qvmhip_mlp *h, const float *X, const float *T, int nb, int epochs, float lr,
                      float *final_loss) {
     if (!h || nb < 1 || !X || !T || epochs < 1 || !(lr > 0)) return fail(QVMHIP_EINVAL, "bad argument");
+    DeviceScope ds(h->device);
     int rc = ensure_batch(h, nb);
     if (rc) return rc;
     QH_CHECK(hipMemcpyAsync(h->X, X, sizeof(float) * nb * h->S.din, hipMemcpyHostToDevice, h->stream));
@@ -260,10 +273,11 @@ int qvmhip_mlp_train(qvmhip_mlp *h, const float *X, const float *T, int nb, int 
 
 int qvmhip_mlp_destroy(qvmhip_mlp *h) {
     if (!h) return QVMHIP_OK;
-    hipStreamSynchronize(h->stream);
+    DeviceScope ds(h->device);
+    if (h->stream) hipStreamSynchronize(h->stream);
     for (float *q : {h->p, h->m, h->v, h->grad, h->loss1, h->X, h->T, h->G, h->lossb, h->out})
         if (q) hipFree(q);
-    hipStreamDestroy(h->stream);
+    if (h->stream) hipStreamDestroy(h->stream);
     delete h;
     return QVMHIP_OK;
 }

@@ -164,6 +164,7 @@ int qvmhip_program_finalize(qvmhip_program *p, int use_graph) {
     if (p->finalized) return fail(QVMHIP_EINVAL, "program already finalized");
     if (!p->s) return fail(QVMHIP_EINVAL, kStateGone);
     qvmhip_state *s = p->s;
+    DeviceScope ds(s->device);
     if (!p->blob.empty()) {
         QH_CHECK(hipMalloc(&p->dev, p->blob.size()));
         QH_CHECK(hipMemcpyAsync(p->dev, p->blob.data(), p->blob.size(), hipMemcpyHostToDevice, s->stream));
@@ -195,6 +196,7 @@ int qvmhip_program_run(qvmhip_program *p) {
     if (!p) return fail(QVMHIP_EINVAL, "null program");
     if (!p->finalized) return fail(QVMHIP_EINVAL, "program not finalized");
     if (!p->s) return fail(QVMHIP_EINVAL, kStateGone);
+    DeviceScope ds(p->s->device);
     if (p->use_graph) {
         QH_CHECK(hipGraphLaunch(p->exec, p->s->stream));
         return QVMHIP_OK;
@@ -205,6 +207,7 @@ int qvmhip_program_run(qvmhip_program *p) {
 int qvmhip_program_destroy(qvmhip_program *p) {
     if (!p) return QVMHIP_OK;
     if (p->s) {
+        DeviceScope ds(p->s->device);
         hipStreamSynchronize(p->s->stream);
         auto &v = p->s->programs;
         v.erase(std::remove(v.begin(), v.end(), p), v.end());
@@ -220,20 +223,28 @@ int qvmhip_time_program(qvmhip_program *p, int reps, double *ms) {
     if (!p || !ms || reps < 1) return fail(QVMHIP_EINVAL, "bad argument");
     if (!p->s) return fail(QVMHIP_EINVAL, kStateGone);
     qvmhip_state *s = p->s;
-    hipEvent_t a, b;
-    QH_CHECK(hipEventCreate(&a));
-    QH_CHECK(hipEventCreate(&b));
-    QH_CHECK(hipEventRecord(a, s->stream));
+    DeviceScope ds(s->device);
+    struct Events {  // destroyed on every return
+        hipEvent_t a = nullptr, b = nullptr;
+        ~Events() {
+            if (a) hipEventDestroy(a);
+            if (b) hipEventDestroy(b);
+        }
+    } ev;
+    QH_CHECK(hipEventCreate(&ev.a));
+    QH_CHECK(hipEventCreate(&ev.b));
+    QH_CHECK(hipEventRecord(ev.a, s->stream));
     for (int r = 0; r < reps; ++r) {
         int rc = qvmhip_program_run(p);
-        if (rc) return rc;
+        if (rc) {
+            hipStreamSynchronize(s->stream);  // no pending record of the events
+            return rc;
+        }
     }
-    QH_CHECK(hipEventRecord(b, s->stream));
-    QH_CHECK(hipEventSynchronize(b));
+    QH_CHECK(hipEventRecord(ev.b, s->stream));
+    QH_CHECK(hipEventSynchronize(ev.b));
     float t = 0;
-    QH_CHECK(hipEventElapsedTime(&t, a, b));
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+    QH_CHECK(hipEventElapsedTime(&t, ev.a, ev.b));
     *ms = (double)t / reps;
     return QVMHIP_OK;
 }

@@ -90,6 +90,37 @@ __device__ inline void stg(V *p, const V &x) {
 void set_error(const std::string &msg);
 int fail(int code, const std::string &msg);
 
+// Makes `dev` the calling thread's current device for the duration of one
+// C-ABI call, so that what a handle allocates and launches goes to the device
+// it was created on whatever device the caller has current, and restores the
+// caller's device on return.
+struct DeviceScope {
+    int prev = -1;
+    explicit DeviceScope(int dev) {
+        int cur = -1;
+        if (hipGetDevice(&cur) == hipSuccess && cur != dev && hipSetDevice(dev) == hipSuccess) prev = cur;
+    }
+    ~DeviceScope() {
+        if (prev >= 0) hipSetDevice(prev);
+    }
+    DeviceScope(const DeviceScope &) = delete;
+    DeviceScope &operator=(const DeviceScope &) = delete;
+};
+
+// Orders `waiter` after everything queued so far on `done` (a kernel on one
+// handle's stream that reads another handle's state must finish before the
+// other handle's next call can change it).
+inline hipError_t stream_after(hipStream_t waiter, hipStream_t done) {
+    if (waiter == done) return hipSuccess;
+    hipEvent_t ev = nullptr;
+    hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e != hipSuccess) return e;
+    e = hipEventRecord(ev, done);
+    if (e == hipSuccess) e = hipStreamWaitEvent(waiter, ev, 0);
+    hipEventDestroy(ev);  // released once the recorded work has completed
+    return e;
+}
+
 }  // namespace qvmhip
 
 // The state handle behind the opaque qvmhip_state of the C ABI.

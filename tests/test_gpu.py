@@ -229,6 +229,19 @@ def test_force_measure_and_errors(S, oracle):
         st.measure_all(1.5)
     assert e.value.code == N.ERANGE
     st.close()
+    # forcing an outcome of probability 0 has no normalisation: an error, and
+    # the state is left as it was
+    z = S(4, "f64")
+    with pytest.raises(N.QVMHipError) as e:
+        z.force_measure(2, 1, 0.0)
+    assert e.value.code == N.ERANGE
+    want = np.zeros(16, complex)
+    want[0] = 1
+    np.testing.assert_array_equal(z.amplitudes(), want)
+    # compiled projector at the end of the draw range: p0 = 1 keeps 0 for u = 1
+    assert z.measure(2, 1.0, True) == (0, 1)
+    np.testing.assert_array_equal(z.amplitudes(), want)
+    z.close()
 
 
 def test_sample_cdf_rule(S, oracle):
@@ -294,6 +307,31 @@ def test_amplitude_io_copy_inner_expectation(S):
     assert z[0] == 1 and np.count_nonzero(z) == 1
     a.close()
     b.close()
+
+
+def test_copy_and_accumulate_read_src_before_its_next_call(S):
+    """copy / accumulate run on the destination's stream; the source's next
+    call (on its own stream) must not change it before they have read it."""
+    rng = np.random.default_rng(16)
+    n = 22
+    psi = rand_state(n, rng)
+    src = load(S, psi)
+    dst = S(n, "f64")
+    acc = S(n, "f64")
+    acc.write_amplitudes(psi)
+    H = np.array([[1, 1], [1, -1]]) / np.sqrt(2)
+    for rep in range(3):
+        say(f"copy/accumulate ordering rep={rep}")
+        snap = src.amplitudes()
+        dst.copy_from(src)
+        acc.accumulate(src)
+        for q in range(n):  # queued on src's stream right behind the reads
+            src.apply_matrix(H, [q])
+        np.testing.assert_array_equal(dst.amplitudes(), snap)
+        np.testing.assert_array_equal(acc.amplitudes(), (psi if rep == 0 else prev_acc) + snap)
+        prev_acc = acc.amplitudes()
+    for st in (src, dst, acc):
+        st.close()
 
 
 def test_determinism(S):
