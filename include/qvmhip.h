@@ -22,7 +22,9 @@
  *  - Qubit q is bit q of the amplitude index (src/wavefunction.lisp:24-40).
  *  - Calls on one handle are ordered on that handle's HIP stream and are
  *    asynchronous unless they return host data; a handle is not re-entrant,
- *    distinct handles may be used from distinct threads.
+ *    distinct handles may be used from distinct threads.  Every call runs on
+ *    the device its handle was created on and leaves the calling thread's
+ *    current device as it found it.
  */
 #ifndef QVMHIP_H
 #define QVMHIP_H
@@ -184,13 +186,16 @@ QVMHIP_API int qvmhip_expectation_zz(qvmhip_state *s, const uint32_t *edges, uin
 QVMHIP_API int qvmhip_inner_product(qvmhip_state *a, qvmhip_state *b, double *re, double *im);
 
 /* --- measurement -------------------------------------------------------- */
-/* force-measurement, pure state: src/measurement.lisp:10-41. */
+/* force-measurement, pure state: src/measurement.lisp:10-41.  Forcing an
+ * outcome whose probability is 0 (bit 1 with p1 = 0, bit 0 with p1 = 1) has
+ * no normalisation and returns QVMHIP_ERANGE with the state unchanged. */
 QVMHIP_API int qvmhip_force_measure(qvmhip_state *s, uint32_t q, int bit, double p1);
 /* measure (src/measurement.lisp:87-105) with the caller's uniform u.
  * compiled = 0: interpreted rule -- p1 == 0 gives 0 and consumes no draw,
  *               else bit = (u <= p1).
  * compiled = 1: compiled projector (src/compile-gate.lisp:221-254) -- always
- *               consumes u, bit = (u < p0) ? 0 : 1.
+ *               consumes u, bit = (u < p0) ? 0 : 1 (0 also when p0 >= 1:
+ *               the reference's draw is below 1, this one may equal 1).
  * *consumed = number of uniforms used (0 or 1). */
 QVMHIP_API int qvmhip_measure(qvmhip_state *s, uint32_t q, double u, int compiled, int *bit,
                               int *consumed);
@@ -214,7 +219,9 @@ QVMHIP_API int qvmhip_read_amplitudes(qvmhip_state *s, uint64_t start, uint64_t 
 /* (setf state-elements) / (setf amplitudes). */
 QVMHIP_API int qvmhip_write_amplitudes(qvmhip_state *s, uint64_t start, uint64_t count,
                                        const double *in);
-/* copy-wavefunction (src/wavefunction.lisp:93-109): dst <- src, same n/dtype. */
+/* copy-wavefunction (src/wavefunction.lisp:93-109): dst <- src, same n/dtype.
+ * Queued on dst's stream after src's queued work; src's later calls wait for
+ * the copy (the same holds for qvmhip_accumulate). */
 QVMHIP_API int qvmhip_copy(qvmhip_state *dst, const qvmhip_state *src);
 /* dst += src elementwise, same n/dtype: the running sum of the Kraus-list
  * evolution of a density matrix (map-into sum #'+, src/apply-gate.lisp:76-96). */
