@@ -39,6 +39,8 @@ TILE_XREG = 16
 TILE_RSWAP = 17
 TILE_REMAP = 18
 TILE_RDIAG = 19
+TILE_D1REAL = 20
+TILE_D1RX = 21
 
 
 class QVMHipError(RuntimeError):

@@ -40,6 +40,12 @@ constexpr uint32_t kTileXReg = 16;   // X on register bit (q0 - TB) where ctrl_m
 constexpr uint32_t kTileRSwap = 17;  // exchange register bits q0 and q1 (both < RB)
 constexpr uint32_t kTileRemap = 18;  // relayout through LDS; new slot bit j <- old bit src[j]
 constexpr uint32_t kTileRDiag = 19;  // multiply register r by entry r (m holds R complex entries)
+// Uncontrolled 1-qubit ops on register bit (q0 - TB) whose entries have zero
+// parts, with those parts left out: every entry real (m[0..3] = u00, u01,
+// u10, u11: H, RY), or real diagonal and imaginary off-diagonal (m[0..3] =
+// Re u00, Im u01, Im u10, Re u11: RX).
+constexpr uint32_t kTileD1Real = 20;
+constexpr uint32_t kTileD1RX = 21;
 
 // Register bits per thread (measured on MI355X): 3 (8 amplitudes) for FP64
 // and for exact FP32, 4 (16) for FP32 with FMA contraction; fewer for small
@@ -183,6 +189,64 @@ __device__ __forceinline__ void reg_dense2(vec2_t<T> (&x)[R], const qvmhip_tile_
         const bool ok = !CTRL || (lane_ok && ((uint32_t)r0 & rcm) == rcm);
 #pragma unroll
         for (int j = 0; j < 4; ++j) x[r0 | j] = CTRL ? vsel<T>(ok, out[j], in[j]) : out[j];
+    }
+}
+
+// The 1-qubit product of reg_dense1 with the terms whose matrix factor is 0
+// left out.  x*0 is an exact zero and adding an exact zero leaves a nonzero
+// value as it is, so every output is the same rounded value as the general
+// form's, in both the separately rounded and the FMA form (an exact-zero
+// result may differ in its sign only).  Half the multiplies of reg_dense1.
+template <typename T, int R, int I, bool FMA>
+__device__ __forceinline__ void reg_dense1_real(vec2_t<T> (&x)[R], const qvmhip_tile_op &op) {
+    using V = vec2_t<T>;
+    const T u00 = mval<T>(op, 0), u01 = mval<T>(op, 1), u10 = mval<T>(op, 2), u11 = mval<T>(op, 3);
+#pragma unroll
+    for (int p = 0; p < R / 2; ++p) {
+        const int ra = rins0<I>(p), rb = ra | (1 << I);
+        const V xa = x[ra], xb = x[rb];
+        V ya, yb;
+        if constexpr (FMA) {
+            ya.x = qfma(xa.x, u00, xb.x * u01);
+            ya.y = qfma(xa.y, u00, xb.y * u01);
+            yb.x = qfma(xa.x, u10, xb.x * u11);
+            yb.y = qfma(xa.y, u10, xb.y * u11);
+        } else {
+            ya.x = xa.x * u00 + xb.x * u01;
+            ya.y = xa.y * u00 + xb.y * u01;
+            yb.x = xa.x * u10 + xb.x * u11;
+            yb.y = xa.y * u10 + xb.y * u11;
+        }
+        x[ra] = ya;
+        x[rb] = yb;
+    }
+}
+
+// u00 = a, u01 = i b, u10 = i c, u11 = d (a, b, c, d real):
+// ya = (a xa.re - b xb.im, a xa.im + b xb.re), yb = (d xb.re - c xa.im,
+// d xb.im + c xa.re), each sum in the general form's order.
+template <typename T, int R, int I, bool FMA>
+__device__ __forceinline__ void reg_dense1_rx(vec2_t<T> (&x)[R], const qvmhip_tile_op &op) {
+    using V = vec2_t<T>;
+    const T a = mval<T>(op, 0), b = mval<T>(op, 1), c = mval<T>(op, 2), d = mval<T>(op, 3);
+#pragma unroll
+    for (int p = 0; p < R / 2; ++p) {
+        const int ra = rins0<I>(p), rb = ra | (1 << I);
+        const V xa = x[ra], xb = x[rb];
+        V ya, yb;
+        if constexpr (FMA) {
+            ya.x = qfma(xa.x, a, -(xb.y * b));
+            ya.y = qfma(xa.y, a, xb.x * b);
+            yb.x = qfma(-xa.y, c, xb.x * d);
+            yb.y = qfma(xa.x, c, xb.y * d);
+        } else {
+            ya.x = xa.x * a - xb.y * b;
+            ya.y = xa.y * a + xb.x * b;
+            yb.x = xb.x * d - xa.y * c;
+            yb.y = xa.x * c + xb.y * d;
+        }
+        x[ra] = ya;
+        x[rb] = yb;
     }
 }
 
@@ -508,6 +572,28 @@ __device__ inline void tile_ops(vec2_t<T> (&x)[1 << RB], const qvmhip_tile_op *_
                 }
 #undef QH_D1
                 break;
+            case kTileD1Real:
+#define QH_D1R(I)                                                        \
+    case I:                                                              \
+        if constexpr (I < RB) reg_dense1_real<T, R, I, FMA>(x, op);      \
+        break;
+                switch (op.q0 - TB) {
+                    QH_D1R(0) QH_D1R(1) QH_D1R(2) QH_D1R(3) QH_D1R(4)
+                    default: break;
+                }
+#undef QH_D1R
+                break;
+            case kTileD1RX:
+#define QH_D1X(I)                                                        \
+    case I:                                                              \
+        if constexpr (I < RB) reg_dense1_rx<T, R, I, FMA>(x, op);        \
+        break;
+                switch (op.q0 - TB) {
+                    QH_D1X(0) QH_D1X(1) QH_D1X(2) QH_D1X(3) QH_D1X(4)
+                    default: break;
+                }
+#undef QH_D1X
+                break;
             case QVMHIP_TILE_DENSE2:
                 if (ctrl)
                     reg_dense2<T, R, true, FMA>(x, op, lane_ok, rcm);
@@ -703,6 +789,24 @@ bool to_register_diagonal(qvmhip_tile_op &o, int tb, int rb) {
     return true;
 }
 
+// An uncontrolled 1-qubit op whose entries are all real, or whose diagonal is
+// real and off-diagonal imaginary, becomes the matching reduced kind.
+void to_split_dense1(qvmhip_tile_op &o) {
+    const double *u = o.m;  // u00 = (u[0], u[1]), u01 = (u[2], u[3]), u10 = (u[4], u[5]), u11 = (u[6], u[7])
+    double v[4];
+    if (u[1] == 0.0 && u[3] == 0.0 && u[5] == 0.0 && u[7] == 0.0) {
+        o.kind = kTileD1Real;
+        v[0] = u[0], v[1] = u[2], v[2] = u[4], v[3] = u[6];
+    } else if (u[1] == 0.0 && u[2] == 0.0 && u[4] == 0.0 && u[7] == 0.0) {
+        o.kind = kTileD1RX;
+        v[0] = u[0], v[1] = u[3], v[2] = u[5], v[3] = u[6];
+    } else {
+        return;
+    }
+    std::memset(o.m, 0, sizeof(o.m));
+    std::memcpy(o.m, v, sizeof(v));
+}
+
 // Local bits an op needs on register bits (its non-diagonal targets).
 uint32_t reg_need(const qvmhip_tile_op &op) {
     switch (op.kind) {
@@ -880,6 +984,7 @@ void schedule_tile_ops(int dtype, bool fma, uint32_t m, const qvmhip_tile_op *in
             o.ctrl_mask |= 1u << L.l2s[op.q1];
             o.q1 = 0;
         }
+        if (o.kind == QVMHIP_TILE_DENSE1 && o.ctrl_mask == 0) to_split_dense1(o);
         if (to_register_diagonal(o, L.tb, L.rb)) {
             // consecutive register diagonals multiply into one when the
             // pass contracts with FMAs (no promise of gate-by-gate rounding)

@@ -393,7 +393,13 @@ def _random_tile_ops(m, count, rng, outside=()):
             op.q1 = g[0] | ((g[1] << 8) if ng == 2 else 0) | (ng << 16)
             vals = np.exp(1j * rng.uniform(0, 6, 2 << ng))
         elif kind == N.TILE_DENSE1:
-            vals = rand_unitary(1, rng).reshape(-1)
+            c, th = rng.integers(3), rng.uniform(0, 6)
+            if c == 1:  # real rotation and RX: the scheduler's reduced kinds
+                vals = np.array([np.cos(th), -np.sin(th), np.sin(th), np.cos(th)], dtype=np.complex128)
+            elif c == 2:
+                vals = np.array([np.cos(th), -1j * np.sin(th), -1j * np.sin(th), np.cos(th)])
+            else:
+                vals = rand_unitary(1, rng).reshape(-1)
         elif kind == N.TILE_DENSE2:
             vals = rand_unitary(2, rng).reshape(-1)
         elif kind == N.TILE_DIAG1:

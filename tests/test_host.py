@@ -336,6 +336,17 @@ def _apply_vec(X, op, m, tb=None, f32=False, gbits=None):
         diff = ((idx >> A) & 1) != ((idx >> B) & 1)
         sw[diff] ^= (1 << A) | (1 << B)
         return X[sw]
+    if k in (N.TILE_D1REAL, N.TILE_D1RX):  # reduced uncontrolled 1q on a register bit
+        assert op.q0 >= tb and cm == 0
+        a_, b_, c_, d_ = mm[:4]
+        u = np.array([[a_, b_], [c_, d_]], dtype=np.complex128) if k == N.TILE_D1REAL else \
+            np.array([[a_, 1j * b_], [1j * c_, d_]])
+        b = 1 << op.q0
+        a = idx[(idx & b) == 0]
+        X = X.copy()
+        xa, xb = X[a].copy(), X[a | b].copy()
+        X[a], X[a | b] = u[0, 0] * xa + u[0, 1] * xb, u[1, 0] * xa + u[1, 1] * xb
+        return X
     X = X.copy()
     if k == N.TILE_DENSE1:
         if tb is not None:
@@ -409,6 +420,18 @@ def _op(kind, q0, q1, ctrl, values):
     return op
 
 
+def _rand_1q(rng):
+    """A Haar 2x2, a real rotation (RY) or an RX: the last two have zero
+    parts, which the scheduler turns into its reduced register kinds."""
+    c = rng.integers(3)
+    th = rng.uniform(0, 6)
+    if c == 1:
+        return np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]], dtype=np.complex128)
+    if c == 2:
+        return np.array([[np.cos(th), -1j * np.sin(th)], [-1j * np.sin(th), np.cos(th)]])
+    return np.linalg.qr(rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2)))[0]
+
+
 def _random_tile_ops(m, count, rng, outside=()):
     """Random ops of every public kind on an m-qubit tile; GDIAG ops (one
     or two of the state qubits ``outside`` the tile) when outside is given."""
@@ -426,8 +449,7 @@ def _random_tile_ops(m, count, rng, outside=()):
             ops.append(_op(kind, q[0], g[0] | (g[1] << 8 if ng == 2 else 0) | (ng << 16), 0,
                            np.exp(1j * rng.uniform(0, 6, 2 << ng))))
         elif kind in (N.TILE_DENSE1,):
-            U = np.linalg.qr(rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2)))[0]
-            ops.append(_op(kind, q[0], 0, ctrl, U.reshape(-1)))
+            ops.append(_op(kind, q[0], 0, ctrl, _rand_1q(rng).reshape(-1)))
         elif kind == N.TILE_DENSE2:
             U = np.linalg.qr(rng.normal(size=(4, 4)) + 1j * rng.normal(size=(4, 4)))[0]
             ops.append(_op(kind, q[0], q[1], ctrl, U.reshape(-1)))
@@ -449,6 +471,7 @@ def test_tile_schedule_matches_op_list(dtype, m, fma):
     rng = np.random.default_rng(m + (100 if dtype == "f32" else 0))
     rb = min(m - 6, 4 if (dtype == "f32" and fma) else 3)
     tb = m - rb
+    kinds = set()
     for trial in range(3):
         # qubits m and m + 1 stand for state qubits outside the tile, with
         # fixed values over the block
@@ -471,6 +494,8 @@ def test_tile_schedule_matches_op_list(dtype, m, fma):
         assert np.max(np.abs(got - want)) < (1e-12 if dtype == "f64" else 1e-4)
         nremap = sum(1 for o in sched if o.kind == N.TILE_REMAP)
         assert nremap <= len(ops)
+        kinds |= {o.kind for o in sched}
+    assert N.TILE_D1REAL in kinds and N.TILE_D1RX in kinds
     # a pass whose targets are all register bits needs no relayout
     hi = list(range(tb, m))
     ops = [_op(N.TILE_DENSE1, q, 0, 0, np.eye(2).reshape(-1)) for q in hi]
