@@ -41,6 +41,8 @@ TILE_REMAP = 18
 TILE_RDIAG = 19
 TILE_D1REAL = 20
 TILE_D1RX = 21
+TILE_CUT_PHASE = 22
+TILE_DATA = 23
 
 
 class QVMHipError(RuntimeError):

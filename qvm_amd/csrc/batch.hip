@@ -29,6 +29,7 @@ struct BItem {
     uint32_t qubits[16] = {0};
     size_t op_off = 0;         // tile ops in blob
     uint32_t nops = 0;
+    bool cut = false;          // tile ops hold CUT_PHASE ops
     std::vector<double> U;     // gate matrix (k <= 3)
 };
 
@@ -45,6 +46,7 @@ struct BLaunch {
     size_t map_off;     // first element of this launch's block -> entry map
     uint32_t inst;      // gate launches
     const BItem *item;  // gate launches
+    bool cut;           // tile launches: an entry holds CUT_PHASE ops
 };
 
 template <typename T>
@@ -161,7 +163,7 @@ int enqueue(qvmhip_batch *b) {
     for (const BLaunch &L : b->launches) {
         int rc;
         if (L.kind == kItemTile)
-            rc = launch_tile_batch(b->stream, b->dtype, (b->flags & QVMHIP_FLAG_FMA_TILES) != 0, L.m, b->buf,
+            rc = launch_tile_batch(b->stream, b->dtype, (b->flags & QVMHIP_FLAG_FMA_TILES) != 0, L.cut, L.m, b->buf,
                                    (const char *)b->entries_dev + L.ent_off, b->blkmap_dev + L.map_off, L.blocks,
                                    kTilesPerBlock, b->blob_dev);
         else
@@ -351,6 +353,7 @@ int qvmhip_batch_add_tile_pass(qvmhip_batch *b, uint32_t inst, uint32_t m, const
     std::vector<qvmhip_tile_op> sched;
     schedule_tile_ops(b->dtype, (b->flags & QVMHIP_FLAG_FMA_TILES) != 0, m, ops, nops, sched);
     it.nops = (uint32_t)sched.size();
+    it.cut = tile_ops_have_cut(sched.data(), sched.size());
     it.op_off = push(b->blob, sched.data(), sizeof(qvmhip_tile_op) * sched.size());
     b->prog[inst].push_back(std::move(it));
     return QVMHIP_OK;
@@ -442,15 +445,16 @@ int qvmhip_batch_finalize(qvmhip_batch *b, int use_graph) {
             if (it.kind == kItemTile)
                 by_m[it.k].push_back(i);
             else
-                b->launches.push_back({kItemGate, 0, 0, 0, 0, 0, i, &it});
+                b->launches.push_back({kItemGate, 0, 0, 0, 0, 0, i, &it, false});
         }
         for (auto &kv : by_m) {
             BLaunch bl{kItemTile, kv.first, b->entries.size(), (uint32_t)kv.second.size(), 0, b->blkmap.size(), 0,
-                       nullptr};
+                       nullptr, false};
             uint64_t blocks = 0;
             uint32_t local = 0;
             for (uint32_t i : kv.second) {
                 const BItem &it = b->prog[i][L];
+                bl.cut = bl.cut || it.cut;
                 std::vector<char> ent(esz);
                 const uint64_t nb = build_tile_entry(
                     ent.data(), b->dtype, (b->flags & QVMHIP_FLAG_FMA_TILES) != 0, b->nq[i], it.k, it.qubits,

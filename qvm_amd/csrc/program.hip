@@ -22,6 +22,7 @@ struct Item {
     size_t host_off = 0;        // operand bytes in blob
     size_t bytes = 0;
     uint32_t nops = 0;
+    bool cut = false;           // tile ops hold CUT_PHASE ops
     std::vector<double> U;      // small gate matrices (launched by value)
 };
 
@@ -60,7 +61,7 @@ int replay(qvmhip_program *p) {
                 rc = launch_fused_dense(s, dev, it.k, it.qubits);
                 break;
             case kTile:
-                rc = launch_tile_pass(s, it.k, it.qubits, it.nops, (const qvmhip_tile_op *)dev, 1);
+                rc = launch_tile_pass(s, it.k, it.qubits, it.nops, (const qvmhip_tile_op *)dev, 1, it.cut);
                 break;
         }
         if (rc) return rc;
@@ -153,6 +154,7 @@ int qvmhip_program_add_tile_pass(qvmhip_program *p, uint32_t m, const uint32_t *
     std::vector<qvmhip_tile_op> sched;
     schedule_tile_ops(p->s->dtype, (p->s->flags & QVMHIP_FLAG_FMA_TILES) != 0, m, ops, nops, sched);
     it.nops = (uint32_t)sched.size();
+    it.cut = tile_ops_have_cut(sched.data(), sched.size());
     it.bytes = sizeof(qvmhip_tile_op) * sched.size();
     it.host_off = push_blob(p, sched.data(), it.bytes);
     p->items.push_back(std::move(it));

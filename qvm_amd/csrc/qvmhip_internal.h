@@ -201,8 +201,11 @@ int sample_shots(qvmhip_state *s, const double *u, uint64_t nshots, uint64_t *ou
 int collapse_to_basis(qvmhip_state *s, const uint64_t *index_dev);
 // ops_on_device = 0: public ops on the host (scheduled here); 1: an already
 // scheduled program on the device.
+// cut: an already scheduled program holds CUT_PHASE ops (for host ops the
+// scheduler's output decides).
 int launch_tile_pass(qvmhip_state *s, uint32_t m, const uint32_t *tile_qubits, uint32_t nops,
-                     const qvmhip_tile_op *ops_dev_or_host, int ops_on_device);
+                     const qvmhip_tile_op *ops_dev_or_host, int ops_on_device, bool cut = false);
+bool tile_ops_have_cut(const qvmhip_tile_op *ops, size_t n);
 int launch_fused_dense(qvmhip_state *s, const void *mat_dev, uint32_t k, const uint32_t *qubits);
 int expectation_zz(qvmhip_state *s, const uint32_t *edges, uint32_t nedges, double *out);
 int inner_product(qvmhip_state *a, qvmhip_state *b, double *re, double *im);
@@ -255,9 +258,10 @@ uint64_t build_tile_entry(void *entry, int dtype, bool fma, uint32_t n, uint32_t
                           uint64_t amp_offset, uint64_t block_start, uint64_t op_off, uint32_t nops,
                           uint32_t tiles_per_block, bool from_zero);
 // op_buf: the device buffer the entries' op offsets point into
-int launch_tile_batch(hipStream_t st, int dtype, bool fma, uint32_t m, void *base, const void *entries_dev,
-                      const uint32_t *blk_ent_dev, uint64_t total_blocks, uint32_t tiles_per_block,
-                      const void *op_buf);
+// cut: some entry of the launch holds CUT_PHASE ops
+int launch_tile_batch(hipStream_t st, int dtype, bool fma, bool cut, uint32_t m, void *base,
+                      const void *entries_dev, const uint32_t *blk_ent_dev, uint64_t total_blocks,
+                      uint32_t tiles_per_block, const void *op_buf);
 }  // namespace qvmhip
 
 #define QH_CHECK(expr)                                                                    \

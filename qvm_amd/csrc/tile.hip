@@ -28,6 +28,7 @@
 // in-order pass is bit-identical to gate-by-gate application.
 #include <algorithm>
 #include <array>
+#include <complex>
 #include <cstring>
 #include <vector>
 
@@ -46,6 +47,15 @@ constexpr uint32_t kTileRDiag = 19;  // multiply register r by entry r (m holds 
 // Re u00, Im u01, Im u10, Re u11: RX).
 constexpr uint32_t kTileD1Real = 20;
 constexpr uint32_t kTileD1RX = 21;
+// FMA passes: a run of ZZ phases with one angle (diag(a, b, b, a) on qubit
+// pairs, the MaxCut cost layer of QAOA) multiplies amplitude x by
+// T[cut(x)], cut(x) = the number of the run's pairs whose bits differ in x,
+// T[c] = a^(E - c) b^c.  CUT_PHASE holds the pairs by where their ends sit
+// (byte layout in cut_phase_op); the ceil((E + 1) / 16) ops after it are
+// DATA ops holding T, which the kernel reads and skips.
+constexpr uint32_t kTileCutPhase = 22;
+constexpr uint32_t kTileData = 23;
+constexpr int kCutMaxEdges = 32;
 
 // Register bits per thread (measured on MI355X): 3 (8 amplitudes) for FP64
 // and for exact FP32, 4 (16) for FP32 with FMA contraction; fewer for small
@@ -486,6 +496,49 @@ __device__ __forceinline__ void reg_gdiag(vec2_t<T> (&x)[R], const qvmhip_tile_o
     }
 }
 
+// CUT_PHASE (the layout of its bytes is described at cut_phase_op): the
+// phase table T of the DATA ops after it goes to LDS, each thread counts the
+// cut pairs of its amplitudes and multiplies register r by T[cut].
+template <typename T, int R, int TB, bool FMA>
+__device__ __forceinline__ void reg_cut_phase(vec2_t<T> (&x)[R], const qvmhip_tile_op *__restrict__ op,
+                                              vec2_t<T> *sa, uint32_t t, uint64_t gb) {
+    const uint8_t *b = (const uint8_t *)op->m;
+    const uint32_t E = b[0];
+    // values derived from t stay inside this op (hoisted out of the op loop
+    // they would hold registers through every other op)
+    asm volatile("" : "+v"(t));
+    __syncthreads();  // earlier readers of sa are done
+    if (t <= E) {
+        const qvmhip_tile_op &d = op[1 + t / 16];
+        vec2_t<T> v;
+        v.x = mval<T>(d, 2 * (t & 15u));
+        v.y = mval<T>(d, 2 * (t & 15u) + 1);
+        sa[t] = v;
+    }
+    uint32_t c = 0, A = 0;
+#pragma unroll 1  // unrolled, the pair terms were computed ahead and spilled
+    for (uint32_t i = 0; i < E; ++i) {
+        {
+            const uint32_t cls = b[72 + i], e0 = b[8 + 2 * i], e1 = b[9 + 2 * i];
+            if (cls == 0u) c += ((t >> e0) ^ (t >> e1)) & 1u;                          // LL
+            else if (cls == 1u) c += ((t >> e0) ^ (uint32_t)(gb >> e1)) & 1u;          // LO
+            else if (cls == 2u) A |= ((t >> e0) & 1u) << i;                            // LR
+            else if (cls == 3u) A |= ((uint32_t)(gb >> e1) & 1u) << i;                 // RO
+            else if (cls == 5u) c += (uint32_t)((gb >> e0) ^ (gb >> e1)) & 1u;         // OO
+        }
+    }
+    const uint32_t *B = (const uint32_t *)(b + 128);
+    __syncthreads();  // the table is in LDS
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const vec2_t<T> f = sa[c + (uint32_t)__popc(A ^ B[r])];
+        vec2_t<T> y;
+        cmul<T, FMA>(x[r].x, x[r].y, f.x, f.y, y.x, y.y);
+        x[r] = y;
+        __builtin_amdgcn_sched_barrier(0);  // one table read at a time: R reads ahead hold 2R registers
+    }
+}
+
 // LDS address of slot s: register row r = s >> TB of NT slots is padded by
 // one element, so a thread's store of register r is t + r*(NT+1) (a fixed
 // offset from t), and a lane bit that a REMAP takes from register bit b of the
@@ -544,7 +597,10 @@ __device__ __forceinline__ void tile_store(vec2_t<T> *__restrict__ psi, const Ti
 }
 
 // Run a scheduled op list on the block held in x.
-template <typename T, int M, int RB, bool FMA>
+// CUT: the op list holds CUT_PHASE ops (FMA passes only).  Kernels without
+// them are built without that code: its registers and code size slow every
+// other op down.
+template <typename T, int M, int RB, bool FMA, bool CUT>
 __device__ inline void tile_ops(vec2_t<T> (&x)[1 << RB], const qvmhip_tile_op *__restrict__ ops, uint32_t nops,
                                 vec2_t<T> *sa, uint64_t gb) {
     constexpr int R = 1 << RB, TB = M - RB;
@@ -629,7 +685,15 @@ __device__ inline void tile_ops(vec2_t<T> (&x)[1 << RB], const qvmhip_tile_op *_
                 reg_gdiag<T, R, TB, FMA>(x, op, sel, t);
                 break;
             }
-            default:  // DIAG1, DIAG2, CPHASE
+            default:  // DIAG1, DIAG2, CPHASE; CUT_PHASE (no case label of its own: each label
+                      // deepens the dispatch of every op)
+                if constexpr (FMA && CUT) {
+                    if (kind == kTileCutPhase) {
+                        reg_cut_phase<T, R, TB, FMA>(x, ops + oi, sa, t, gb);
+                        oi += ((const uint8_t *)op.m)[1];  // its DATA ops
+                        break;
+                    }
+                }
                 if (ctrl)
                     reg_diag<T, R, TB, true, FMA>(x, op, t, lane_ok, rcm);
                 else
@@ -640,7 +704,7 @@ __device__ inline void tile_ops(vec2_t<T> (&x)[1 << RB], const qvmhip_tile_op *_
 
 }
 
-template <typename T, int M, int RB, bool FMA>
+template <typename T, int M, int RB, bool FMA, bool CUT>
 __global__ __launch_bounds__(1 << (M - RB)) void k_tile(vec2_t<T> *__restrict__ psi, const TileArgs a,
                                                        const qvmhip_tile_op *__restrict__ ops, uint32_t nops) {
     using V = vec2_t<T>;
@@ -652,7 +716,7 @@ __global__ __launch_bounds__(1 << (M - RB)) void k_tile(vec2_t<T> *__restrict__ 
         V x[1 << RB];
         const uint64_t gb = group_base(a.g, tile);
         tile_load<T, M, RB>(psi, a, gb, toff, x);
-        tile_ops<T, M, RB, FMA>(x, ops, nops, sa, gb);
+        tile_ops<T, M, RB, FMA, CUT>(x, ops, nops, sa, gb);
         tile_store<T, M, RB>(psi, a, gb, toff, x);
     }
 }
@@ -675,7 +739,7 @@ struct TileEntry {
     uint32_t from_zero;    // 1: the state is |0...0>, built in registers instead of loaded
 };
 
-template <typename T, int M, int RB, bool FMA>
+template <typename T, int M, int RB, bool FMA, bool CUT>
 __global__ __launch_bounds__(1 << (M - RB)) void k_tile_batch(vec2_t<T> *__restrict__ base,
                                                              const TileEntry *__restrict__ ent,
                                                              const uint32_t *__restrict__ blk_ent,
@@ -702,7 +766,7 @@ __global__ __launch_bounds__(1 << (M - RB)) void k_tile_batch(vec2_t<T> *__restr
             } else {
                 tile_load<T, M, RB>(psi, e.args, gb, toff, x);
             }
-            tile_ops<T, M, RB, FMA>(x, (const qvmhip_tile_op *)(op_buf + e.op_off), e.nops, sa, gb);
+            tile_ops<T, M, RB, FMA, CUT>(x, (const qvmhip_tile_op *)(op_buf + e.op_off), e.nops, sa, gb);
             tile_store<T, M, RB>(psi, e.args, gb, toff, x);
         }
     }
@@ -807,6 +871,99 @@ void to_split_dense1(qvmhip_tile_op &o) {
     std::memcpy(o.m, v, sizeof(v));
 }
 
+// A public op that is a ZZ phase diag(a, b, b, a) between two qubits, each
+// a tile bit (local position) or a state qubit outside the tile.
+struct ZZEdge {
+    std::complex<double> a, b;
+    int end[2];  // local position, or 64 + outside state qubit
+};
+
+bool zz_edge(const qvmhip_tile_op &op, ZZEdge &e) {
+    auto c = [&](int i) { return std::complex<double>(op.m[2 * i], op.m[2 * i + 1]); };
+    if (op.kind == QVMHIP_TILE_DIAG2 && op.ctrl_mask == 0) {
+        if (!(c(0) == c(3) && c(1) == c(2))) return false;
+        e.a = c(0), e.b = c(1);
+        e.end[0] = (int)op.q0, e.end[1] = (int)op.q1;
+        return true;
+    }
+    if (op.kind != QVMHIP_TILE_GDIAG) return false;
+    const uint32_t ng = op.q1 >> 16, g0 = op.q1 & 0xFFu, g1 = (op.q1 >> 8) & 0xFFu;
+    if (ng == 1) {  // entry index: local bit | outside bit << 1
+        if (!(c(0) == c(3) && c(1) == c(2))) return false;
+        e.a = c(0), e.b = c(1);
+        e.end[0] = (int)op.q0, e.end[1] = 64 + (int)g0;
+        return true;
+    }
+    // both ends outside: entries must not depend on the local bit
+    for (int i = 0; i < 8; i += 2)
+        if (!(c(i) == c(i + 1))) return false;
+    if (!(c(0) == c(6) && c(2) == c(4))) return false;
+    e.a = c(0), e.b = c(2);
+    e.end[0] = 64 + (int)g0, e.end[1] = 64 + (int)g1;
+    return true;
+}
+
+// Local positions an op reads or changes (a ZZ phase with both ends outside
+// the tile touches none).
+uint32_t touched_bits(const qvmhip_tile_op &op) {
+    ZZEdge e;
+    if (zz_edge(op, e)) {
+        uint32_t msk = 0;
+        for (int k = 0; k < 2; ++k)
+            if (e.end[k] < 64) msk |= 1u << e.end[k];
+        return msk;
+    }
+    uint32_t msk = (1u << op.q0) | op.ctrl_mask;
+    if (op.kind == QVMHIP_TILE_DENSE2 || op.kind == QVMHIP_TILE_DIAG2 || op.kind == QVMHIP_TILE_CPERM ||
+        op.kind == QVMHIP_TILE_SWAP || op.kind == QVMHIP_TILE_CPHASE)
+        msk |= 1u << op.q1;
+    return msk;
+}
+
+bool diagonal_public(uint32_t kind) {
+    return kind == QVMHIP_TILE_DIAG1 || kind == QVMHIP_TILE_DIAG2 || kind == QVMHIP_TILE_CPHASE ||
+           kind == QVMHIP_TILE_GDIAG;
+}
+
+// Ops in layers: op i goes one layer after the latest earlier op it does
+// not commute with (a shared bit, unless both are diagonal); within a layer
+// the ZZ phases come first, grouped by angle, in their original order.
+std::vector<qvmhip_tile_op> layer_order(const qvmhip_tile_op *in, uint32_t nin) {
+    std::vector<uint32_t> level(nin, 0), bits(nin);
+    std::vector<bool> diag(nin), zz(nin);
+    std::vector<ZZEdge> edge(nin);
+    for (uint32_t i = 0; i < nin; ++i) {
+        bits[i] = touched_bits(in[i]);
+        diag[i] = diagonal_public(in[i].kind);
+        zz[i] = zz_edge(in[i], edge[i]);
+        for (uint32_t j = 0; j < i; ++j)
+            if ((bits[i] & bits[j]) && !(diag[i] && diag[j])) level[i] = std::max(level[i], level[j] + 1);
+    }
+    // angle group of each ZZ phase: index of the first ZZ phase of its level with the same entries
+    std::vector<uint32_t> group(nin, UINT32_MAX);
+    for (uint32_t i = 0; i < nin; ++i) {
+        if (!zz[i]) continue;
+        group[i] = i;
+        for (uint32_t j = 0; j < i; ++j) {
+            if (zz[j] && level[j] == level[i] && edge[j].a == edge[i].a && edge[j].b == edge[i].b) {
+                group[i] = group[j];
+                break;
+            }
+        }
+    }
+    std::vector<uint32_t> order(nin);
+    for (uint32_t i = 0; i < nin; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+        if (level[x] != level[y]) return level[x] < level[y];
+        if (zz[x] != zz[y]) return (bool)zz[x];
+        if (zz[x]) return group[x] < group[y];
+        return false;
+    });
+    std::vector<qvmhip_tile_op> out(nin);
+    for (uint32_t i = 0; i < nin; ++i) out[i] = in[order[i]];
+    return out;
+}
+
 // Local bits an op needs on register bits (its non-diagonal targets).
 uint32_t reg_need(const qvmhip_tile_op &op) {
     switch (op.kind) {
@@ -851,18 +1008,107 @@ qvmhip_tile_op remap_op(const Layout &from, const int *new_s2l) {
     return o;
 }
 
+// Whether two consecutive ops are ZZ phases of one angle.
+bool has_cut_run(const std::vector<qvmhip_tile_op> &ops) {
+    ZZEdge prev, e;
+    bool have = false;
+    for (const qvmhip_tile_op &o : ops) {
+        const bool z = zz_edge(o, e);
+        if (z && have && e.a == prev.a && e.b == prev.b) return true;
+        have = z;
+        prev = e;
+    }
+    return false;
+}
+
+// CUT_PHASE for a run of ZZ phases of one angle under layout L, followed by
+// its DATA ops.  Bytes of m: [0] E (pairs, <= 32), [1] number of DATA ops;
+// pair i: ends at bytes 8 + 2i and 9 + 2i, class at byte 72 + i; bytes
+// 128..191: B[r] (uint32, r < 16).  Classes by where the ends sit: LL (two
+// thread bits), LO (thread bit, outside qubit), LR (thread bit, register
+// bit), RO (register bit, outside qubit), RR (two register bits), OO (two
+// outside qubits).  For thread t, register r and block base gb:
+//   cut = sum over LL, LO, OO pairs of (bit ^ bit) + popc(A ^ B[r])
+// where bit i of A is t's thread bit (LR) or gb's outside bit (RO) of pair
+// i, and bit i of B[r] is r's register bit (LR, RO) or the xor of r's two
+// register bits (RR).
+enum : uint8_t { kCutLL = 0, kCutLO = 1, kCutLR = 2, kCutRO = 3, kCutRR = 4, kCutOO = 5 };
+
+void cut_phase_op(const Layout &L, const std::vector<ZZEdge> &run, std::vector<qvmhip_tile_op> &out) {
+    struct End {
+        int kind;  // 0 thread bit, 1 register bit, 2 outside qubit
+        int pos;
+    };
+    auto where = [&](int end) -> End {
+        if (end >= 64) return {2, end - 64};
+        const int sbit = L.l2s[end];
+        return sbit < L.tb ? End{0, sbit} : End{1, sbit - L.tb};
+    };
+    qvmhip_tile_op o;
+    std::memset(&o, 0, sizeof(o));
+    o.kind = kTileCutPhase;
+    uint8_t *b = (uint8_t *)o.m;
+    const int E = (int)run.size(), ndata = (E + 1 + 15) / 16;
+    b[0] = (uint8_t)E;
+    b[1] = (uint8_t)ndata;
+    uint32_t B[16] = {0};
+    for (int i = 0; i < E; ++i) {
+        End x = where(run[i].end[0]), y = where(run[i].end[1]);
+        if (x.kind > y.kind) std::swap(x, y);
+        uint8_t cls;
+        if (x.kind == 0) cls = y.kind == 0 ? kCutLL : (y.kind == 1 ? kCutLR : kCutLO);
+        else if (x.kind == 1) cls = y.kind == 1 ? kCutRR : kCutRO;
+        else cls = kCutOO;
+        b[72 + i] = cls;
+        b[8 + 2 * i] = (uint8_t)x.pos;
+        b[9 + 2 * i] = (uint8_t)y.pos;
+        for (int r = 0; r < (1 << L.rb); ++r) {
+            uint32_t v = 0;
+            if (cls == kCutLR) v = (uint32_t)(r >> y.pos) & 1u;
+            else if (cls == kCutRO) v = (uint32_t)(r >> x.pos) & 1u;
+            else if (cls == kCutRR) v = (uint32_t)((r >> x.pos) ^ (r >> y.pos)) & 1u;
+            B[r] |= v << i;
+        }
+    }
+    std::memcpy(b + 128, B, sizeof(B));
+    out.push_back(o);
+    // T[c] = a^(E - c) b^c
+    for (int d = 0; d < ndata; ++d) {
+        qvmhip_tile_op t;
+        std::memset(&t, 0, sizeof(t));
+        t.kind = kTileData;
+        for (int k = 0; k < 16 && 16 * d + k <= E; ++k) {
+            const int c = 16 * d + k;
+            std::complex<double> v(1.0, 0.0);
+            for (int j = 0; j < E - c; ++j) v *= run[0].a;
+            for (int j = 0; j < c; ++j) v *= run[0].b;
+            t.m[2 * k] = v.real();
+            t.m[2 * k + 1] = v.imag();
+        }
+        out.push_back(t);
+    }
+}
+
 }  // namespace
 
 // Turn a pass's op list (tile-local bit positions) into the slot-space
 // program the kernel runs.  `pin` places the targets of 2-qubit dense ops and
 // controlled SWAPs on register bits 0 and 1.
-void schedule_tile_ops(int dtype, bool fma, uint32_t m, const qvmhip_tile_op *in, uint32_t nin,
+void schedule_tile_ops(int dtype, bool fma, uint32_t m, const qvmhip_tile_op *in_ops, uint32_t nin,
                        std::vector<qvmhip_tile_op> &out) {
     Layout L;
     L.m = (int)m;
     L.rb = tile_reg_bits((int)m, dtype == QVMHIP_F64, fma);
     L.tb = L.m - L.rb;
     for (int i = 0; i < L.m; ++i) L.l2s[i] = L.s2l[i] = i;
+    // FMA passes do not keep gate-by-gate rounding: commuting ops are
+    // regrouped into layers so that ZZ phases of one angle become one op
+    std::vector<qvmhip_tile_op> layered;
+    if (fma) {
+        layered = layer_order(in_ops, nin);
+        if (!has_cut_run(layered)) layered.assign(in_ops, in_ops + nin);  // nothing to merge: keep the order
+    }
+    const qvmhip_tile_op *in = fma ? layered.data() : in_ops;
     out.clear();
     out.reserve(nin + nin / 2 + 2);
     std::vector<uint32_t> need(nin);
@@ -882,6 +1128,25 @@ void schedule_tile_ops(int dtype, bool fma, uint32_t m, const qvmhip_tile_op *in
     };
     for (uint32_t i = 0; i < nin; ++i) {
         const qvmhip_tile_op &op = in[i];
+        if (fma) {
+            ZZEdge e0;
+            if (zz_edge(op, e0)) {
+                // the run of ZZ phases of this angle that starts here
+                std::vector<ZZEdge> run{e0};
+                uint32_t k = i + 1;
+                ZZEdge e;
+                while (k < nin && (int)run.size() < kCutMaxEdges && zz_edge(in[k], e) && e.a == e0.a &&
+                       e.b == e0.b) {
+                    run.push_back(e);
+                    ++k;
+                }
+                if (run.size() >= 2) {
+                    cut_phase_op(L, run, out);
+                    i = k - 1;
+                    continue;
+                }
+            }
+        }
         if (op.kind == QVMHIP_TILE_SWAP && op.ctrl_mask == 0) {
             L.swap_slots(L.l2s[op.q0], L.l2s[op.q1]);  // relabel only
             continue;
@@ -1008,7 +1273,8 @@ void schedule_tile_ops(int dtype, bool fma, uint32_t m, const qvmhip_tile_op *in
     }
     if (dtype == QVMHIP_F32) {
         for (qvmhip_tile_op &o : out) {
-            if (o.kind == kTileRemap || o.kind == kTileRSwap || o.kind == kTileXReg) continue;
+            if (o.kind == kTileRemap || o.kind == kTileRSwap || o.kind == kTileXReg || o.kind == kTileCutPhase)
+                continue;
             float f[32];
             for (int i = 0; i < 32; ++i) f[i] = (float)o.m[i];
             std::memset(o.m, 0, sizeof(o.m));
@@ -1039,7 +1305,7 @@ void make_tile_args(uint32_t n, uint32_t m, const uint32_t *tq, int tb, TileArgs
     }
 }
 
-template <typename T, int M, bool FMA>
+template <typename T, int M, bool FMA, bool CUT>
 int run_tile(qvmhip_state *s, const uint32_t *tq, uint32_t m, const qvmhip_tile_op *ops_dev,
              uint32_t nops) {
     constexpr int RB = tile_reg_bits(M, sizeof(T) == 8, FMA);
@@ -1048,7 +1314,7 @@ int run_tile(qvmhip_state *s, const uint32_t *tq, uint32_t m, const qvmhip_tile_
     make_tile_args(s->n, m, tq, M - RB, a);
     prof_begin(s, QVMHIP_K_TILE);
     const uint32_t grid = (uint32_t)std::min<uint64_t>(a.g.ngroups, kMaxGrid);
-    hipLaunchKernelGGL((k_tile<T, M, RB, FMA>), dim3(grid), dim3(NT), 0, s->stream, (vec2_t<T> *)s->amps, a,
+    hipLaunchKernelGGL((k_tile<T, M, RB, FMA, CUT>), dim3(grid), dim3(NT), 0, s->stream, (vec2_t<T> *)s->amps, a,
                        ops_dev, nops);
     prof_end(s);
     QH_LAUNCH_CHECK();
@@ -1056,22 +1322,24 @@ int run_tile(qvmhip_state *s, const uint32_t *tq, uint32_t m, const qvmhip_tile_
 }
 
 template <typename T, int M>
-int run_tile_f(qvmhip_state *s, const uint32_t *tq, uint32_t m, const qvmhip_tile_op *ops_dev, uint32_t nops) {
-    return (s->flags & QVMHIP_FLAG_FMA_TILES) ? run_tile<T, M, true>(s, tq, m, ops_dev, nops)
-                                              : run_tile<T, M, false>(s, tq, m, ops_dev, nops);
+int run_tile_f(qvmhip_state *s, const uint32_t *tq, uint32_t m, const qvmhip_tile_op *ops_dev, uint32_t nops,
+               bool cut) {
+    if (!(s->flags & QVMHIP_FLAG_FMA_TILES)) return run_tile<T, M, false, false>(s, tq, m, ops_dev, nops);
+    return cut ? run_tile<T, M, true, true>(s, tq, m, ops_dev, nops)
+               : run_tile<T, M, true, false>(s, tq, m, ops_dev, nops);
 }
 
 template <typename T>
 int run_tile_m(qvmhip_state *s, uint32_t m, const uint32_t *tq, const qvmhip_tile_op *ops_dev,
-               uint32_t nops) {
+               uint32_t nops, bool cut) {
     switch (m) {
-        case 8: return run_tile_f<T, 8>(s, tq, m, ops_dev, nops);
-        case 9: return run_tile_f<T, 9>(s, tq, m, ops_dev, nops);
-        case 10: return run_tile_f<T, 10>(s, tq, m, ops_dev, nops);
-        case 11: return run_tile_f<T, 11>(s, tq, m, ops_dev, nops);
-        case 12: return run_tile_f<T, 12>(s, tq, m, ops_dev, nops);
+        case 8: return run_tile_f<T, 8>(s, tq, m, ops_dev, nops, cut);
+        case 9: return run_tile_f<T, 9>(s, tq, m, ops_dev, nops, cut);
+        case 10: return run_tile_f<T, 10>(s, tq, m, ops_dev, nops, cut);
+        case 11: return run_tile_f<T, 11>(s, tq, m, ops_dev, nops, cut);
+        case 12: return run_tile_f<T, 12>(s, tq, m, ops_dev, nops, cut);
         case 13:
-            if constexpr (sizeof(T) == 4) return run_tile_f<T, 13>(s, tq, m, ops_dev, nops);
+            if constexpr (sizeof(T) == 4) return run_tile_f<T, 13>(s, tq, m, ops_dev, nops, cut);
             break;
     }
     return fail(QVMHIP_EINVAL, "tile size m must be 8..12 (F64) or 8..13 (F32)");
@@ -1079,14 +1347,21 @@ int run_tile_m(qvmhip_state *s, uint32_t m, const uint32_t *tq, const qvmhip_til
 
 }  // namespace
 
+bool tile_ops_have_cut(const qvmhip_tile_op *ops, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if (ops[i].kind == kTileCutPhase) return true;
+    return false;
+}
+
 int launch_tile_pass(qvmhip_state *s, uint32_t m, const uint32_t *tile_qubits, uint32_t nops,
-                     const qvmhip_tile_op *ops, int ops_on_device) {
+                     const qvmhip_tile_op *ops, int ops_on_device, bool cut) {
     const qvmhip_tile_op *dev = ops;
     qvmhip_tile_op *tmp = nullptr;
     if (!ops_on_device) {
         std::vector<qvmhip_tile_op> sched;
         schedule_tile_ops(s->dtype, (s->flags & QVMHIP_FLAG_FMA_TILES) != 0, m, ops, nops, sched);
         nops = (uint32_t)sched.size();
+        cut = tile_ops_have_cut(sched.data(), sched.size());
         // Operands go through the matrix slot (64 KiB = 240 ops); longer lists
         // get a temporary buffer released after the stream drains.  The slot
         // is reused by the next call, so wait for any earlier reader first.
@@ -1102,8 +1377,8 @@ int launch_tile_pass(qvmhip_state *s, uint32_t m, const uint32_t *tile_qubits, u
             dev = tmp;
         }
     }
-    int rc = s->dtype == QVMHIP_F64 ? run_tile_m<double>(s, m, tile_qubits, dev, nops)
-                                    : run_tile_m<float>(s, m, tile_qubits, dev, nops);
+    int rc = s->dtype == QVMHIP_F64 ? run_tile_m<double>(s, m, tile_qubits, dev, nops, cut)
+                                    : run_tile_m<float>(s, m, tile_qubits, dev, nops, cut);
     if (tmp) {
         hipStreamSynchronize(s->stream);
         hipFree(tmp);
@@ -1129,27 +1404,27 @@ uint64_t build_tile_entry(void *entry, int dtype, bool fma, uint32_t n, uint32_t
 }
 
 namespace {
-template <typename T, int M, bool FMA>
+template <typename T, int M, bool FMA, bool CUT>
 void tile_batch_go(hipStream_t st, void *base, const void *ent, const uint32_t *blk_ent, uint64_t total,
                    uint32_t tpb, const void *op_buf) {
     constexpr int RB = tile_reg_bits(M, sizeof(T) == 8, FMA);
     constexpr int NT = 1 << (M - RB);
     const uint32_t grid = (uint32_t)std::min<uint64_t>(total, kMaxGrid);
-    hipLaunchKernelGGL((k_tile_batch<T, M, RB, FMA>), dim3(grid), dim3(NT), 0, st, (vec2_t<T> *)base,
+    hipLaunchKernelGGL((k_tile_batch<T, M, RB, FMA, CUT>), dim3(grid), dim3(NT), 0, st, (vec2_t<T> *)base,
                        (const TileEntry *)ent, blk_ent, total, tpb, (const char *)op_buf);
 }
-template <typename T, bool FMA>
+template <typename T, bool FMA, bool CUT>
 int tile_batch_m(hipStream_t st, uint32_t m, void *base, const void *ent, const uint32_t *blk_ent, uint64_t total,
                  uint32_t tpb, const void *op_buf) {
     switch (m) {
-        case 8: tile_batch_go<T, 8, FMA>(st, base, ent, blk_ent, total, tpb, op_buf); break;
-        case 9: tile_batch_go<T, 9, FMA>(st, base, ent, blk_ent, total, tpb, op_buf); break;
-        case 10: tile_batch_go<T, 10, FMA>(st, base, ent, blk_ent, total, tpb, op_buf); break;
-        case 11: tile_batch_go<T, 11, FMA>(st, base, ent, blk_ent, total, tpb, op_buf); break;
-        case 12: tile_batch_go<T, 12, FMA>(st, base, ent, blk_ent, total, tpb, op_buf); break;
+        case 8: tile_batch_go<T, 8, FMA, CUT>(st, base, ent, blk_ent, total, tpb, op_buf); break;
+        case 9: tile_batch_go<T, 9, FMA, CUT>(st, base, ent, blk_ent, total, tpb, op_buf); break;
+        case 10: tile_batch_go<T, 10, FMA, CUT>(st, base, ent, blk_ent, total, tpb, op_buf); break;
+        case 11: tile_batch_go<T, 11, FMA, CUT>(st, base, ent, blk_ent, total, tpb, op_buf); break;
+        case 12: tile_batch_go<T, 12, FMA, CUT>(st, base, ent, blk_ent, total, tpb, op_buf); break;
         case 13:
             if constexpr (sizeof(T) == 4) {
-                tile_batch_go<T, 13, FMA>(st, base, ent, blk_ent, total, tpb, op_buf);
+                tile_batch_go<T, 13, FMA, CUT>(st, base, ent, blk_ent, total, tpb, op_buf);
                 break;
             }
             return fail(QVMHIP_EINVAL, "tile size 13 is FP32 only");
@@ -1161,18 +1436,16 @@ int tile_batch_m(hipStream_t st, uint32_t m, void *base, const void *ent, const 
 }
 }  // namespace
 
-int launch_tile_batch(hipStream_t st, int dtype, bool fma, uint32_t m, void *base, const void *entries_dev,
-                      const uint32_t *blk_ent_dev, uint64_t total_blocks, uint32_t tiles_per_block,
-                      const void *op_buf) {
-    if (fma)
-        return dtype == QVMHIP_F64 ? tile_batch_m<double, true>(st, m, base, entries_dev, blk_ent_dev, total_blocks,
-                                                                tiles_per_block, op_buf)
-                                   : tile_batch_m<float, true>(st, m, base, entries_dev, blk_ent_dev, total_blocks,
-                                                               tiles_per_block, op_buf);
-    return dtype == QVMHIP_F64 ? tile_batch_m<double, false>(st, m, base, entries_dev, blk_ent_dev, total_blocks,
-                                                             tiles_per_block, op_buf)
-                               : tile_batch_m<float, false>(st, m, base, entries_dev, blk_ent_dev, total_blocks,
-                                                            tiles_per_block, op_buf);
+int launch_tile_batch(hipStream_t st, int dtype, bool fma, bool cut, uint32_t m, void *base,
+                      const void *entries_dev, const uint32_t *blk_ent_dev, uint64_t total_blocks,
+                      uint32_t tiles_per_block, const void *op_buf) {
+    const bool f64 = dtype == QVMHIP_F64;
+#define QH_TB(TT, F, C) \
+    tile_batch_m<TT, F, C>(st, m, base, entries_dev, blk_ent_dev, total_blocks, tiles_per_block, op_buf)
+    if (!fma) return f64 ? QH_TB(double, false, false) : QH_TB(float, false, false);
+    if (!cut) return f64 ? QH_TB(double, true, false) : QH_TB(float, true, false);
+    return f64 ? QH_TB(double, true, true) : QH_TB(float, true, true);
+#undef QH_TB
 }
 
 }  // namespace qvmhip

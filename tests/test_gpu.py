@@ -378,12 +378,32 @@ def _random_tile_ops(m, count, rng, outside=()):
     """Random tile ops of every public kind, some controlled (local bits);
     GDIAG ops on one or two of the state qubits ``outside`` the tile."""
     ops = []
-    kinds = [1, 1, 2, 3, 4, 5, 5, 6, 6, 7] + ([8, 8] if len(outside) else [])
+    kinds = [1, 1, 2, 3, 4, 5, 5, 6, 6, 7] + ([8, 8] if len(outside) else []) + [9, 9, 9]
+    za, zb = np.exp(1j * rng.uniform(0, 6, 2))  # one ZZ angle, diag(a, b, b, a)
     for _ in range(count):
         kind = int(rng.choice(kinds))
         q = [int(x) for x in rng.choice(m, 3, replace=False)]
         ctrl = (1 << q[2]) if rng.random() < 0.25 else 0
         op = N.TileOp()
+        if kind == 9:  # ZZ phases (FMA passes merge runs of them into one cut-phase op)
+            where = int(rng.integers(3)) if len(outside) >= 2 else 0
+            op.ctrl_mask = 0
+            if where == 0:
+                op.kind, op.q0, op.q1 = N.TILE_DIAG2, q[0], q[1]
+                vals = np.array([za, zb, zb, za])
+            elif where == 1:
+                op.kind, op.q0, op.q1 = N.TILE_GDIAG, q[0], int(rng.choice(list(outside))) | (1 << 16)
+                vals = np.array([za, zb, zb, za])
+            else:
+                g0, g1 = [int(x) for x in rng.choice(list(outside), 2, replace=False)]
+                op.kind, op.q0, op.q1 = N.TILE_GDIAG, 0, g0 | (g1 << 8) | (2 << 16)
+                vals = np.array([za, za, zb, zb, zb, zb, za, za])
+            v = np.zeros(16, dtype=np.complex128)
+            v[:vals.size] = vals
+            for k, x in enumerate(v.view(np.float64)):
+                op.m[k] = x
+            ops.append(op)
+            continue
         op.kind, op.q0, op.ctrl_mask = kind, q[0], ctrl
         op.q1 = q[1] if kind not in (N.TILE_DENSE1, N.TILE_DIAG1) else 0
         if kind == N.TILE_GDIAG:
@@ -474,6 +494,50 @@ def test_tile_pass_every_op_kind(S, oracle, dtype, m, fma):
         np.testing.assert_array_equal(got, ref)
     else:
         assert np.max(np.abs(got - ref)) < (1e-13 if dtype == "f64" else 2e-6)
+    st.close()
+
+
+@pytest.mark.parametrize("dtype,m", [("f64", 10), ("f64", 12), ("f32", 12)])
+def test_tile_pass_long_zz_runs(S, oracle, dtype, m):
+    """QAOA-shaped FMA passes: runs of up to 32 ZZ phases of one angle
+    (pairs of tile qubits, tile and outside qubits, two outside qubits) and
+    RX on every tile qubit, against the oracle applying each gate."""
+    from qvm_amd.state import HipPureState
+    rng = np.random.default_rng(70 + m)
+    n = 15
+    tq = sorted([0, 1, 2] + [int(x) for x in rng.choice(np.arange(3, n), m - 3, replace=False)])
+    outside = [q for q in range(n) if q not in tq]
+    psi = rand_state(n, rng)
+    st = HipPureState(n, dtype, fma_tiles=True)
+    st.write_amplitudes(psi)
+    ref = psi.copy()
+    ops = []
+
+    def op_of(kind, q0, q1, vals):
+        op = N.TileOp()
+        op.kind, op.q0, op.q1, op.ctrl_mask = kind, q0, q1, 0
+        v = np.zeros(16, dtype=np.complex128)
+        v[:len(vals)] = vals
+        for k, x in enumerate(v.view(np.float64)):
+            op.m[k] = x
+        return op
+    for layer in range(2):
+        a, b = np.exp(1j * rng.uniform(0, 6, 2))
+        pairs = [(x, y) for x in range(m) for y in range(x + 1, m)]
+        rng.shuffle(pairs)
+        ops += [op_of(N.TILE_DIAG2, x, y, [a, b, b, a]) for x, y in pairs[:30]]
+        ops += [op_of(N.TILE_GDIAG, x, outside[x % len(outside)] | (1 << 16), [a, b, b, a]) for x in range(m)]
+        ops.append(op_of(N.TILE_GDIAG, 0, outside[0] | (outside[1] << 8) | (2 << 16), [a, a, b, b, b, b, a, a]))
+        th = rng.uniform(0, 6)
+        ops += [op_of(N.TILE_DENSE1, x, 0, [np.cos(th), -1j * np.sin(th), -1j * np.sin(th), np.cos(th)])
+                for x in range(m)]
+    say(f"long zz runs {dtype} m={m} ops={len(ops)}")
+    st.apply_tile_pass(tq, ops)
+    for op in ops:
+        U, qs = _tile_op_as_gate(op, tq)
+        oracle.apply_matrix(ref, U, G.nat_tuple(*qs))
+    got = st.amplitudes()
+    assert np.max(np.abs(got - ref)) < (1e-13 if dtype == "f64" else 2e-6)
     st.close()
 
 

@@ -404,9 +404,53 @@ def _emulate_slot_program(sched, m, tb, psi_tile, f32=False, gbits=None):
     in slot s (register s >> tb of thread s & (2^tb - 1)); ops are in slot
     bits.  The final X must be in the identity layout."""
     X = psi_tile.copy()
-    for op in sched:
+    i = 0
+    while i < len(sched):
+        op = sched[i]
+        if op.kind == N.TILE_CUT_PHASE:
+            raw = bytes(bytearray(memoryview(op.m).cast("B")))
+            E, nd = raw[0], raw[1]
+            T = []
+            for d in sched[i + 1:i + 1 + nd]:
+                assert d.kind == N.TILE_DATA
+                mm = np.array([d.m[k] for k in range(32)])
+                if f32:
+                    mm = np.frombuffer(mm.tobytes(), dtype=np.float32)[:32].astype(np.float64)
+                T += list(mm.reshape(-1, 2) @ np.array([1, 1j]))
+            X = X * np.array(T)[_cut_counts(raw, m, tb, gbits or {})]
+            i += 1 + nd
+            continue
         X = _apply_vec(X, op, m, tb, f32, gbits)
+        i += 1
     return X
+
+
+def _cut_counts(raw, m, tb, gbits):
+    """Index into the phase table of every slot: the cut count that the
+    kernel forms from the CUT_PHASE bytes (pairs by class, masks B[r])."""
+    idx = np.arange(1 << m)
+    t, r = idx & ((1 << tb) - 1), idx >> tb
+    E = raw[0]
+    c = np.zeros(idx.size, dtype=np.int64)
+    A = np.zeros(idx.size, dtype=np.int64)
+    for i in range(E):
+        cls, e0, e1 = raw[72 + i], raw[8 + 2 * i], raw[9 + 2 * i]
+        if cls == 0:
+            c += ((t >> e0) ^ (t >> e1)) & 1
+        elif cls == 1:
+            c += ((t >> e0) & 1) ^ gbits.get(e1, 0)
+        elif cls == 2:
+            A |= ((t >> e0) & 1) << i
+        elif cls == 3:
+            A |= gbits.get(e1, 0) << i
+        elif cls == 5:
+            c += gbits.get(e0, 0) ^ gbits.get(e1, 0)
+        else:
+            assert cls == 4
+    B = np.frombuffer(raw[128:192], dtype=np.uint32).astype(np.int64)
+    x = A ^ B[r]
+    pop = np.array([bin(int(v)).count("1") for v in x])
+    return c + pop
 
 
 def _op(kind, q0, q1, ctrl, values):
@@ -436,13 +480,26 @@ def _random_tile_ops(m, count, rng, outside=()):
     """Random ops of every public kind on an m-qubit tile; GDIAG ops (one
     or two of the state qubits ``outside`` the tile) when outside is given."""
     ops = []
-    kinds = [1, 1, 2, 3, 4, 5, 5, 6, 6, 7] + ([8, 8] if len(outside) else [])
+    kinds = [1, 1, 2, 3, 4, 5, 5, 6, 6, 7] + ([8, 8] if len(outside) else []) + [9, 9, 9]
+    zz = np.exp(1j * rng.uniform(0, 6, 2))  # one ZZ angle: diag(a, b, b, a)
     for _ in range(count):
         kind = int(rng.choice(kinds))
         q = [int(x) for x in rng.choice(m, 3, replace=False)]
         ctrl = 0
         if rng.random() < 0.2:
             ctrl = 1 << q[2]
+        if kind == 9:  # a ZZ phase on a pair of tile bits or of tile and outside qubits
+            a, b = zz
+            where = int(rng.integers(3)) if len(outside) >= 2 else 0
+            if where == 0:
+                ops.append(_op(N.TILE_DIAG2, q[0], q[1], 0, np.array([a, b, b, a])))
+            elif where == 1:
+                g = int(rng.choice(list(outside)))
+                ops.append(_op(N.TILE_GDIAG, q[0], g | (1 << 16), 0, np.array([a, b, b, a])))
+            else:
+                g0, g1 = [int(x) for x in rng.choice(list(outside), 2, replace=False)]
+                ops.append(_op(N.TILE_GDIAG, 0, g0 | (g1 << 8) | (2 << 16), 0, np.array([a, a, b, b, b, b, a, a])))
+            continue
         if kind == N.TILE_GDIAG:
             ng = 1 if len(outside) == 1 else int(rng.integers(1, 3))
             g = [int(x) for x in rng.choice(list(outside), ng, replace=False)] + [0]
@@ -463,6 +520,43 @@ def _random_tile_ops(m, count, rng, outside=()):
             ops.append(_op(kind, q[0], q[1], ctrl if kind == N.TILE_CPERM or rng.random() < 0.5 else 0,
                                  np.zeros(1)))
     return ops
+
+
+@pytest.mark.parametrize("dtype,m", [("f64", 10), ("f64", 12), ("f32", 12)])
+def test_cut_phase_long_runs(dtype, m):
+    """QAOA-like passes (ZZ phases of one angle on many pairs, then RX on
+    every tile bit, two layers) under FMA scheduling: runs of up to 32 pairs
+    become one CUT_PHASE op with its phase table in several DATA ops."""
+    rng = np.random.default_rng(40 + m)
+    rb = min(m - 6, 4 if dtype == "f32" else 3)
+    tb = m - rb
+    outside = (m, m + 1, m + 2)
+    gbits = {q: int(rng.integers(2)) for q in outside}
+    ops = []
+    for layer in range(2):
+        a, b = np.exp(1j * rng.uniform(0, 6, 2))
+        pairs = [(x, y) for x in range(m) for y in range(x + 1, m)]
+        rng.shuffle(pairs)
+        for x, y in pairs[:30]:
+            ops.append(_op(N.TILE_DIAG2, x, y, 0, np.array([a, b, b, a])))
+        for x in range(m):
+            ops.append(_op(N.TILE_GDIAG, x, outside[x % 3] | (1 << 16), 0, np.array([a, b, b, a])))
+        ops.append(_op(N.TILE_GDIAG, 0, outside[0] | (outside[2] << 8) | (2 << 16), 0,
+                       np.array([a, a, b, b, b, b, a, a])))
+        th = rng.uniform(0, 6)
+        for x in range(m):
+            ops.append(_op(N.TILE_DENSE1, x, 0, 0, np.array([np.cos(th), -1j * np.sin(th), -1j * np.sin(th),
+                                                           np.cos(th)])))
+    sched = N.tile_schedule(dtype, m, ops, True)
+    cuts = [o for o in sched if o.kind == N.TILE_CUT_PHASE]
+    E = [bytes(bytearray(memoryview(o.m).cast("B")))[0] for o in cuts]
+    assert max(E) == 32 and len(cuts) >= 4
+    v = rng.normal(size=1 << m) + 1j * rng.normal(size=1 << m)
+    want = v.copy()
+    for op in ops:
+        want = _apply_vec(want, op, m, gbits=gbits)
+    got = _emulate_slot_program(sched, m, tb, v, dtype == "f32", gbits)
+    assert np.max(np.abs(got - want)) < (1e-11 if dtype == "f64" else 1e-3)
 
 
 @pytest.mark.parametrize("dtype,m,fma", [("f64", 8, False), ("f64", 9, False), ("f64", 11, False), ("f64", 12, True),
@@ -496,6 +590,7 @@ def test_tile_schedule_matches_op_list(dtype, m, fma):
         assert nremap <= len(ops)
         kinds |= {o.kind for o in sched}
     assert N.TILE_D1REAL in kinds and N.TILE_D1RX in kinds
+    assert (N.TILE_CUT_PHASE in kinds) == fma
     # a pass whose targets are all register bits needs no relayout
     hi = list(range(tb, m))
     ops = [_op(N.TILE_DENSE1, q, 0, 0, np.eye(2).reshape(-1)) for q in hi]
