@@ -85,6 +85,10 @@ typedef struct {
 /* qvmhip_config.flags: tile passes on this state contract their complex
  * products with fused multiply-adds (fewer roundings and instructions; the
  * results are then no longer bit-identical to gate-by-gate application).
+ * Such passes may also apply commuting ops of a pass in another order and
+ * combine consecutive diagonals (a run of ZZ phases diag(a, b, b, a) of one
+ * angle becomes one multiply by a^(E - c) b^c, c = the number of the run's
+ * qubit pairs whose bits differ); only the rounding changes.
  * The plain gate kernels always keep the reference's rounding. */
 #define QVMHIP_FLAG_FMA_TILES 1u
 
