@@ -763,6 +763,12 @@ __global__ __launch_bounds__(1 << (M - RB)) void k_tile_batch(vec2_t<T> *__restr
                     x[r].x = (gb + toff + e.args.roff[r] == 0) ? T(1) : T(0);
                     x[r].y = T(0);
                 }
+                if (gb != 0) {
+                    // a block without amplitude 0 is all zeros, and every tile
+                    // op is linear within the block: it stays zero
+                    tile_store<T, M, RB>(psi, e.args, gb, toff, x);
+                    continue;
+                }
             } else {
                 tile_load<T, M, RB>(psi, e.args, gb, toff, x);
             }
