@@ -432,6 +432,16 @@ int qvmhip_batch_finalize(qvmhip_batch *b, int use_graph) {
     b->first_pass_resets = true;
     for (uint32_t i = 0; i < b->count; ++i)
         if (b->prog[i].empty() || b->prog[i][0].kind != kItemTile) b->first_pass_resets = false;
+    // qubits touched before each item of a run (which starts from |0...0>):
+    // amplitudes with a bit set outside them are still zero
+    std::vector<std::vector<uint64_t>> touched(b->count);
+    for (uint32_t i = 0; i < b->count; ++i) {
+        uint64_t S = 0;
+        for (const BItem &it : b->prog[i]) {
+            touched[i].push_back(S);
+            for (uint32_t j = 0; j < it.k; ++j) S |= uint64_t(1) << it.qubits[j];
+        }
+    }
     // levels: the L-th item of every instance; tile passes grouped by width
     size_t depth = 0;
     for (auto &p : b->prog) depth = std::max(depth, p.size());
@@ -456,10 +466,13 @@ int qvmhip_batch_finalize(qvmhip_batch *b, int use_graph) {
                 const BItem &it = b->prog[i][L];
                 bl.cut = bl.cut || it.cut;
                 std::vector<char> ent(esz);
+                uint64_t tmask = 0;
+                for (uint32_t j = 0; j < it.k; ++j) tmask |= uint64_t(1) << it.qubits[j];
+                const uint64_t all = (uint64_t(1) << b->nq[i]) - 1;
                 const uint64_t nb = build_tile_entry(
                     ent.data(), b->dtype, (b->flags & QVMHIP_FLAG_FMA_TILES) != 0, b->nq[i], it.k, it.qubits,
-                    b->offset[i], blocks, it.op_off, it.nops,
-                    kTilesPerBlock, L == 0 && b->first_pass_resets);
+                    b->offset[i], blocks, it.op_off, it.nops, kTilesPerBlock, L == 0 && b->first_pass_resets,
+                    all & ~touched[i][L] & ~tmask);
                 b->entries.insert(b->entries.end(), ent.begin(), ent.end());
                 b->blkmap.insert(b->blkmap.end(), (size_t)nb, local);
                 blocks += nb;

@@ -253,10 +253,11 @@ void schedule_tile_ops(int dtype, bool fma, uint32_t m, const qvmhip_tile_op *in
                        std::vector<qvmhip_tile_op> &out);
 size_t tile_entry_bytes();
 // Fills one batch tile entry; returns its number of work blocks of
-// tiles_per_block tiles.
+// tiles_per_block tiles.  zero_mask: state qubits untouched before this pass
+// of a run that starts from |0...0> (blocks with one of them set are zero).
 uint64_t build_tile_entry(void *entry, int dtype, bool fma, uint32_t n, uint32_t m, const uint32_t *tq,
                           uint64_t amp_offset, uint64_t block_start, uint64_t op_off, uint32_t nops,
-                          uint32_t tiles_per_block, bool from_zero);
+                          uint32_t tiles_per_block, bool from_zero, uint64_t zero_mask);
 // op_buf: the device buffer the entries' op offsets point into
 // cut: some entry of the launch holds CUT_PHASE ops
 int launch_tile_batch(hipStream_t st, int dtype, bool fma, bool cut, uint32_t m, void *base,

@@ -735,6 +735,10 @@ struct TileEntry {
     uint64_t amp_offset;   // first amplitude of the state in the batch buffer
     uint64_t block_start;  // first work block of the launch that works on this entry
     uint64_t op_off;       // byte offset of the entry's scheduled ops in the op buffer
+    // state qubits that no earlier pass or gate of the run has touched: the
+    // run starts from |0...0>, so a block whose base index has one of them set
+    // is all zeros before this pass, and every tile op keeps it zero
+    uint64_t zero_mask;
     uint32_t nops;
     uint32_t from_zero;    // 1: the state is |0...0>, built in registers instead of loaded
 };
@@ -756,18 +760,23 @@ __global__ __launch_bounds__(1 << (M - RB)) void k_tile_batch(vec2_t<T> *__restr
         for (uint64_t tile = first; tile < last; ++tile) {
             V x[1 << RB];
             const uint64_t gb = group_base(e.args.g, tile);
+            if (gb & e.zero_mask) {
+                // an all-zero block stays zero: the first pass of a run stores
+                // it (the buffer holds the previous run), later passes leave
+                // the zeros in place
+                if (e.from_zero) {
+#pragma unroll
+                    for (int r = 0; r < (1 << RB); ++r) x[r].x = x[r].y = T(0);
+                    tile_store<T, M, RB>(psi, e.args, gb, toff, x);
+                }
+                continue;
+            }
             if (e.from_zero) {
                 // the first pass of a batch run: |0...0> needs no read
 #pragma unroll
                 for (int r = 0; r < (1 << RB); ++r) {
                     x[r].x = (gb + toff + e.args.roff[r] == 0) ? T(1) : T(0);
                     x[r].y = T(0);
-                }
-                if (gb != 0) {
-                    // a block without amplitude 0 is all zeros, and every tile
-                    // op is linear within the block: it stays zero
-                    tile_store<T, M, RB>(psi, e.args, gb, toff, x);
-                    continue;
                 }
             } else {
                 tile_load<T, M, RB>(psi, e.args, gb, toff, x);
@@ -1396,7 +1405,7 @@ size_t tile_entry_bytes() { return sizeof(TileEntry); }
 
 uint64_t build_tile_entry(void *entry, int dtype, bool fma, uint32_t n, uint32_t m, const uint32_t *tq,
                           uint64_t amp_offset, uint64_t block_start, uint64_t op_off, uint32_t nops,
-                          uint32_t tiles_per_block, bool from_zero) {
+                          uint32_t tiles_per_block, bool from_zero, uint64_t zero_mask) {
     TileEntry e;
     std::memset(&e, 0, sizeof(e));
     make_tile_args(n, m, tq, (int)m - tile_reg_bits((int)m, dtype == QVMHIP_F64, fma), e.args);
@@ -1405,6 +1414,7 @@ uint64_t build_tile_entry(void *entry, int dtype, bool fma, uint32_t n, uint32_t
     e.op_off = op_off;
     e.nops = nops;
     e.from_zero = from_zero ? 1u : 0u;
+    e.zero_mask = zero_mask;
     std::memcpy(entry, &e, sizeof(e));
     return (e.args.g.ngroups + tiles_per_block - 1) / tiles_per_block;
 }

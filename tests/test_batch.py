@@ -92,6 +92,35 @@ def test_batch_initial_state_each_run(oracle, gate_first):
     b.close()
 
 
+@pytest.mark.parametrize("fma", [False, True])
+def test_batch_passes_skip_zero_blocks(oracle, fma):
+    """Runs start from |0...0>, so a pass's blocks whose base has a qubit set
+    that no earlier pass or gate touched are zero and skipped: states of 14..18
+    qubits in 10-qubit tiles (several passes each), one with standalone
+    3-qubit gates between its passes, every run against the oracle."""
+    from qvm_amd.batch import Batch
+    rng = np.random.default_rng(21)
+    ns = [16, 18, 14]
+    circuits = []
+    for n in ns:
+        circuits.append(list(CI.layered(n, 3, seed=int(rng.integers(100)))))
+    U3 = np.linalg.qr(rng.normal(size=(8, 8)) + 1j * rng.normal(size=(8, 8)))[0]
+    circuits[2] = circuits[2][:20] + [(U3, [13, 2, 9])] + circuits[2][20:] + [(U3, [0, 12, 7])]
+    b = Batch(circuits, ns, "f64", use_graph=True, fma_tiles=fma, tile_m=10)
+    for run in range(2):
+        b.run()
+        b.sync()
+        for i, n in enumerate(ns):
+            say(f"zero-block skip fma={fma} run={run} instance={i} n={n}")
+            ref = oracle_state(oracle, n, circuits[i])
+            assert np.max(np.abs(b.amplitudes(i) - ref)) < 1e-13, (run, i)
+    b.close()
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
 def test_tile_width_autotuner(oracle):
     """The autotuner times every candidate width on a sample and the batch it
     picks matches the oracle."""
