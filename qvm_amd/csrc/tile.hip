@@ -1120,8 +1120,10 @@ void schedule_tile_ops(int dtype, bool fma, uint32_t m, const qvmhip_tile_op *in
     // regrouped into layers so that ZZ phases of one angle become one op
     std::vector<qvmhip_tile_op> layered;
     if (fma) {
-        layered = layer_order(in_ops, nin);
-        if (!has_cut_run(layered)) layered.assign(in_ops, in_ops + nin);  // nothing to merge: keep the order
+        // the layering compares every pair of ops: passes longer than 4096
+        // ops keep their order (and still merge the runs they hold)
+        if (nin <= 4096) layered = layer_order(in_ops, nin);
+        if (nin > 4096 || !has_cut_run(layered)) layered.assign(in_ops, in_ops + nin);  // nothing to merge
     }
     const qvmhip_tile_op *in = fma ? layered.data() : in_ops;
     out.clear();
