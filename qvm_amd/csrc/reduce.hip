@@ -236,6 +236,18 @@ __device__ inline void chunk_lane_probs(const vec2_t<T> *__restrict__ psi, uint6
     for (int u = 0; u < 4; ++u) p[u] = (first + u < len) ? prob<T>(psi[first + u]) : 0.0;
 }
 
+// The four amplitudes of chunk_lane_probs, unconverted (zeros past len).
+template <typename T>
+__device__ inline void chunk_lane_load(const vec2_t<T> *__restrict__ psi, uint64_t len, uint64_t chunk, int lane,
+                                       vec2_t<T> v[4]) {
+    const uint64_t first = (chunk << kChunkLog2) + (uint64_t)lane * 4;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        v[u].x = v[u].y = T(0);
+        if (first + u < len) v[u] = psi[first + u];
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_chunk_sums(const vec2_t<T> *__restrict__ psi,
                                                        uint64_t len, uint64_t nchunks,
@@ -395,12 +407,14 @@ __global__ __launch_bounds__(kBlock) void k_sample_search(const vec2_t<T> *__res
 
 // ---------------------------------------------------------------------------
 // Batched finish: every instance of a batch buffer at once.  A work block
-// covers kFinishAmps amplitudes (16 sampling chunks) of one instance: each
-// wave builds the chunk sums of four chunks and, reading the same
+// covers kFinishAmps amplitudes (64 sampling chunks) of one instance: each
+// wave builds the chunk sums of 16 chunks and, reading the same
 // probabilities, accumulates sum_x |psi_x|^2 (|E| - 2 cut(x)), the MaxCut
-// <H_C> partial of the block.
+// <H_C> partial of the block.  A block first sets up per-lane edge tables,
+// so it takes many chunks.
 
-constexpr uint64_t kFinishAmps = uint64_t(kBlock) * 16;
+constexpr int kFinishChunksPerWave = 16;
+constexpr uint64_t kFinishAmps = uint64_t(kBlock) * 4 * kFinishChunksPerWave;
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_bfinish_partial(const vec2_t<T> *__restrict__ buf,
@@ -436,12 +450,20 @@ __global__ __launch_bounds__(kBlock) void k_bfinish_partial(const vec2_t<T> *__r
             }
         }
         double acc = 0.0;
+        const uint64_t c0 = lb * (kBlock / kWave * kFinishChunksPerWave) + (uint64_t)wid * kFinishChunksPerWave;
+        // the next chunk's amplitudes are loaded while this one is summed
+        vec2_t<T> cur[4], nxt[4];
+        if (c0 < nch) chunk_lane_load<T>(psi, len, c0, lane, cur);
 #pragma unroll 1
-        for (int k4 = 0; k4 < 4; ++k4) {
-            const uint64_t c = lb * 16 + (uint64_t)wid * 4 + k4;
+        for (int k4 = 0; k4 < kFinishChunksPerWave; ++k4) {
+            const uint64_t c = c0 + k4;
             if (c >= nch) break;  // uniform per wave
+            if (k4 + 1 < kFinishChunksPerWave && c + 1 < nch) chunk_lane_load<T>(psi, len, c + 1, lane, nxt);
             double p[4];
-            chunk_lane_probs<T>(psi, len, c, lane, p);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) p[u] = prob<T>(cur[u]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) cur[u] = nxt[u];
             if (a.chunk_sum) {
                 const double incl = wave_inclusive_scan(((p[0] + p[1]) + p[2]) + p[3], lane);
                 if (lane == kWave - 1) a.chunk_sum[a.chunk_base[i] + c] = incl;
