@@ -607,7 +607,11 @@ __device__ inline void tile_ops(vec2_t<T> (&x)[1 << RB], const qvmhip_tile_op *_
     const uint32_t t = threadIdx.x;
     for (uint32_t oi = 0; oi < nops; ++oi) {
         const qvmhip_tile_op &op = ops[oi];
-        const uint32_t kind = op.kind, cm = op.ctrl_mask;
+        // the whole header in one scalar load: kind, targets and controls
+        // are then known after a single wait
+        const uint4 hdr = *reinterpret_cast<const uint4 *>(&op);
+        asm volatile("" ::"s"(hdr.y), "s"(hdr.z));  // keep the target fields in this load
+        const uint32_t kind = hdr.x, cm = hdr.w;
         const bool lane_ok = (t & cm) == (cm & ((1u << TB) - 1u));
         const uint32_t rcm = cm >> TB;
         const bool ctrl = cm != 0;
@@ -622,7 +626,7 @@ __device__ inline void tile_ops(vec2_t<T> (&x)[1 << RB], const qvmhip_tile_op *_
                 reg_dense1<T, R, I, false, FMA>(x, op, lane_ok, rcm);            \
         }                                                                    \
         break;
-                switch (op.q0 - TB) {
+                switch (hdr.y - TB) {
                     QH_D1(0) QH_D1(1) QH_D1(2) QH_D1(3) QH_D1(4)
                     default: break;
                 }
@@ -633,7 +637,7 @@ __device__ inline void tile_ops(vec2_t<T> (&x)[1 << RB], const qvmhip_tile_op *_
     case I:                                                              \
         if constexpr (I < RB) reg_dense1_real<T, R, I, FMA>(x, op);      \
         break;
-                switch (op.q0 - TB) {
+                switch (hdr.y - TB) {
                     QH_D1R(0) QH_D1R(1) QH_D1R(2) QH_D1R(3) QH_D1R(4)
                     default: break;
                 }
@@ -644,7 +648,7 @@ __device__ inline void tile_ops(vec2_t<T> (&x)[1 << RB], const qvmhip_tile_op *_
     case I:                                                              \
         if constexpr (I < RB) reg_dense1_rx<T, R, I, FMA>(x, op);        \
         break;
-                switch (op.q0 - TB) {
+                switch (hdr.y - TB) {
                     QH_D1X(0) QH_D1X(1) QH_D1X(2) QH_D1X(3) QH_D1X(4)
                     default: break;
                 }
@@ -657,7 +661,7 @@ __device__ inline void tile_ops(vec2_t<T> (&x)[1 << RB], const qvmhip_tile_op *_
                     reg_dense2<T, R, false, FMA>(x, op, lane_ok, rcm);
                 break;
             case kTileXReg:
-                switch (op.q0 - TB) {
+                switch (hdr.y - TB) {
                     case 0: reg_x<T, R, 0>(x, lane_ok, rcm); break;
                     case 1: reg_x<T, R, 1>(x, lane_ok, rcm); break;
                     case 2: if constexpr (RB > 2) reg_x<T, R, 2>(x, lane_ok, rcm); break;
@@ -670,7 +674,7 @@ __device__ inline void tile_ops(vec2_t<T> (&x)[1 << RB], const qvmhip_tile_op *_
                 reg_swap01<T, R>(x, lane_ok, rcm);
                 break;
             case kTileRSwap:
-                reg_rswap_any<T, R>(x, op.q0, op.q1);
+                reg_rswap_any<T, R>(x, hdr.y, hdr.z);
                 break;
             case kTileRemap:
                 reg_remap<T, R, TB>(x, sa, op, t);
@@ -679,7 +683,7 @@ __device__ inline void tile_ops(vec2_t<T> (&x)[1 << RB], const qvmhip_tile_op *_
                 reg_rdiag<T, R, FMA>(x, op);
                 break;
             case QVMHIP_TILE_GDIAG: {
-                const uint32_t d = op.q1;
+                const uint32_t d = hdr.z;
                 uint32_t sel = (uint32_t)(gb >> (d & 63u)) & 1u;
                 if ((d >> 16) == 2u) sel |= ((uint32_t)(gb >> ((d >> 8) & 63u)) & 1u) << 1;
                 reg_gdiag<T, R, TB, FMA>(x, op, sel, t);
