@@ -208,9 +208,8 @@ __device__ __forceinline__ void reg_dense2(vec2_t<T> (&x)[R], const qvmhip_tile_
 // form's, in both the separately rounded and the FMA form (an exact-zero
 // result may differ in its sign only).  Half the multiplies of reg_dense1.
 template <typename T, int R, int I, bool FMA>
-__device__ __forceinline__ void reg_dense1_real(vec2_t<T> (&x)[R], const qvmhip_tile_op &op) {
+__device__ __forceinline__ void reg_dense1_real(vec2_t<T> (&x)[R], T u00, T u01, T u10, T u11) {
     using V = vec2_t<T>;
-    const T u00 = mval<T>(op, 0), u01 = mval<T>(op, 1), u10 = mval<T>(op, 2), u11 = mval<T>(op, 3);
 #pragma unroll
     for (int p = 0; p < R / 2; ++p) {
         const int ra = rins0<I>(p), rb = ra | (1 << I);
@@ -236,9 +235,8 @@ __device__ __forceinline__ void reg_dense1_real(vec2_t<T> (&x)[R], const qvmhip_
 // ya = (a xa.re - b xb.im, a xa.im + b xb.re), yb = (d xb.re - c xa.im,
 // d xb.im + c xa.re), each sum in the general form's order.
 template <typename T, int R, int I, bool FMA>
-__device__ __forceinline__ void reg_dense1_rx(vec2_t<T> (&x)[R], const qvmhip_tile_op &op) {
+__device__ __forceinline__ void reg_dense1_rx(vec2_t<T> (&x)[R], T a, T b, T c, T d) {
     using V = vec2_t<T>;
-    const T a = mval<T>(op, 0), b = mval<T>(op, 1), c = mval<T>(op, 2), d = mval<T>(op, 3);
 #pragma unroll
     for (int p = 0; p < R / 2; ++p) {
         const int ra = rins0<I>(p), rb = ra | (1 << I);
@@ -607,10 +605,21 @@ __device__ inline void tile_ops(vec2_t<T> (&x)[1 << RB], const qvmhip_tile_op *_
     const uint32_t t = threadIdx.x;
     for (uint32_t oi = 0; oi < nops; ++oi) {
         const qvmhip_tile_op &op = ops[oi];
-        // the whole header in one scalar load: kind, targets and controls
-        // are then known after a single wait
-        const uint4 hdr = *reinterpret_cast<const uint4 *>(&op);
-        asm volatile("" ::"s"(hdr.y), "s"(hdr.z));  // keep the target fields in this load
+        // the header and the first 32 operand bytes in one scalar load: the
+        // reduced 1q ops (most ops of a pass) then start after a single wait
+        struct Head48 {
+            uint32_t w[12];
+        };
+        const Head48 h48 = *reinterpret_cast<const Head48 *>(&op);
+        asm volatile("" ::"s"(h48.w[1]), "s"(h48.w[2]), "s"(h48.w[4]), "s"(h48.w[5]), "s"(h48.w[6]),
+                     "s"(h48.w[7]), "s"(h48.w[8]), "s"(h48.w[9]), "s"(h48.w[10]), "s"(h48.w[11]));
+        const uint4 hdr = make_uint4(h48.w[0], h48.w[1], h48.w[2], h48.w[3]);
+        auto coef = [&](int k) -> T {  // operand k of a reduced 1q op
+            if constexpr (sizeof(T) == 8)
+                return __hiloint2double((int)h48.w[5 + 2 * k], (int)h48.w[4 + 2 * k]);
+            else
+                return __uint_as_float(h48.w[4 + k]);
+        };
         const uint32_t kind = hdr.x, cm = hdr.w;
         const bool lane_ok = (t & cm) == (cm & ((1u << TB) - 1u));
         const uint32_t rcm = cm >> TB;
@@ -635,7 +644,7 @@ __device__ inline void tile_ops(vec2_t<T> (&x)[1 << RB], const qvmhip_tile_op *_
             case kTileD1Real:
 #define QH_D1R(I)                                                        \
     case I:                                                              \
-        if constexpr (I < RB) reg_dense1_real<T, R, I, FMA>(x, op);      \
+        if constexpr (I < RB) reg_dense1_real<T, R, I, FMA>(x, coef(0), coef(1), coef(2), coef(3)); \
         break;
                 switch (hdr.y - TB) {
                     QH_D1R(0) QH_D1R(1) QH_D1R(2) QH_D1R(3) QH_D1R(4)
@@ -646,7 +655,7 @@ __device__ inline void tile_ops(vec2_t<T> (&x)[1 << RB], const qvmhip_tile_op *_
             case kTileD1RX:
 #define QH_D1X(I)                                                        \
     case I:                                                              \
-        if constexpr (I < RB) reg_dense1_rx<T, R, I, FMA>(x, op);        \
+        if constexpr (I < RB) reg_dense1_rx<T, R, I, FMA>(x, coef(0), coef(1), coef(2), coef(3)); \
         break;
                 switch (hdr.y - TB) {
                     QH_D1X(0) QH_D1X(1) QH_D1X(2) QH_D1X(3) QH_D1X(4)
