@@ -548,10 +548,12 @@ __device__ __forceinline__ uint32_t lds_addr(uint32_t s) {
     return s + (s >> TB);
 }
 
+// src4: the op's first 16 operand bytes as four words; byte j is the old
+// slot bit of new slot bit j.
 template <typename T, int R, int TB>
-__device__ __forceinline__ void reg_remap(vec2_t<T> (&x)[R], vec2_t<T> *sa, const qvmhip_tile_op &op, uint32_t t) {
+__device__ __forceinline__ void reg_remap(vec2_t<T> (&x)[R], vec2_t<T> *sa, const uint32_t (&src4)[4], uint32_t t) {
     constexpr uint32_t NT = 1u << TB;
-    const uint8_t *src = (const uint8_t *)op.m;  // src[j]: old slot bit of new slot bit j
+    auto src = [&](int j) -> uint32_t { return (src4[j >> 2] >> (8 * (j & 3))) & 0xFFu; };
     __syncthreads();  // readers of the previous layout are done with sa
 #pragma unroll
     for (int r = 0; r < R; ++r) sa[t + (uint32_t)r * (NT + 1)] = x[r];
@@ -559,10 +561,10 @@ __device__ __forceinline__ void reg_remap(vec2_t<T> (&x)[R], vec2_t<T> *sa, cons
     uint32_t base = 0;
 #pragma unroll
     for (int j = 0; j < TB; ++j)
-        if ((t >> j) & 1u) base |= 1u << src[j];
+        if ((t >> j) & 1u) base |= 1u << src(j);
     uint32_t pw[8];
 #pragma unroll
-    for (int b = 0; b < 8; ++b) pw[b] = (1 << b) < R ? (1u << src[TB + b]) : 0u;
+    for (int b = 0; b < 8; ++b) pw[b] = (1 << b) < R ? (1u << src(TB + b)) : 0u;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         uint32_t off = 0;
@@ -686,7 +688,10 @@ __device__ inline void tile_ops(vec2_t<T> (&x)[1 << RB], const qvmhip_tile_op *_
                 reg_rswap_any<T, R>(x, hdr.y, hdr.z);
                 break;
             case kTileRemap:
-                reg_remap<T, R, TB>(x, sa, op, t);
+                {
+                    const uint32_t src4[4] = {h48.w[4], h48.w[5], h48.w[6], h48.w[7]};
+                    reg_remap<T, R, TB>(x, sa, src4, t);
+                }
                 break;
             case kTileRDiag:
                 reg_rdiag<T, R, FMA>(x, op);
