@@ -60,8 +60,8 @@ def main() -> int:
     ap.add_argument("--big-n", type=int, default=34, help="FP32-only size of the precision report")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(sharding is a design, see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the engine has no multi-GPU path "
+              "(sharding over the top qubits is a design, not built); run with --gpus 1", file=sys.stderr)
         return 2
 
     if args.workload == "precision":
