@@ -48,6 +48,11 @@ enum {
 
 enum { QVMHIP_F64 = 0, QVMHIP_F32 = 1 };
 
+/* Sampling rules: GT = min{b : C(b) > u} (measure-all, bisection sampler,
+ * src/measurement.lisp:179-225); GE = min{i : cdf[i] >= u} (multi-shot,
+ * src/measurement.lisp:246-288).  Both clamp to len-1. */
+enum { QVMHIP_RULE_GT = 0, QVMHIP_RULE_GE = 1 };
+
 /* Tile-pass op kinds (gates applied inside an LDS-resident block). */
 enum {
     QVMHIP_TILE_DENSE1 = 1, /* 2x2 on local qubit q0                          */
@@ -215,6 +220,17 @@ QVMHIP_API int qvmhip_sample_measure_all(qvmhip_state *s, const double *u, uint6
 /* sample-wavefunction-multiple-times (src/measurement.lisp:246-288): per u,
  * min{i : cdf[i] >= u}, clamped to len-1.  No collapse. */
 QVMHIP_API int qvmhip_sample(qvmhip_state *s, const double *u, uint64_t nshots, uint64_t *out);
+/* Either sampler above (rule QVMHIP_RULE_GT / _GE) with boundary reporting,
+ * the sample call of the pure-state path as a whole (measure-all-state
+ * src/measurement.lisp:125-143 + 179-225, sample-wavefunction-multiple-times
+ * :246-288).  A draw u that selects b is near a boundary when |u - C(b)| < eps
+ * or |u - C(b-1)| < eps, with C the engine's own FP64 prefix values that
+ * decided the draw (C(b-1) of a 256-amplitude chunk's first index is the
+ * chunk's exclusive prefix; a clamped draw compares C(len-1) only).
+ * near_flags (optional, nshots bytes) receives 1 per near draw, n_near
+ * (optional) their count; eps = 0 flags nothing.  No collapse. */
+QVMHIP_API int qvmhip_sample_ex(qvmhip_state *s, const double *u, uint64_t nshots, int rule, double eps,
+                                uint64_t *out, uint8_t *near_flags, uint64_t *n_near);
 
 /* --- amplitudes (amplitudes / state-elements / copy-wavefunction) -------- */
 /* amplitudes, nth-amplitude, state-elements: src/qvm.lisp:63-82,

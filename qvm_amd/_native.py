@@ -17,6 +17,8 @@ LIB_PATH = os.path.join(HERE, "_lib", "libqvmhip.so")
 
 F64 = 0
 F32 = 1
+RULE_GT = 0  # sampling rules of qvmhip_sample_ex
+RULE_GE = 1
 FLAG_FMA_TILES = 1  # qvmhip_config.flags
 
 OK = 0
@@ -104,6 +106,7 @@ SIGNATURES = {
     "qvmhip_measure_all": [_vp, C.c_double, _u64p],
     "qvmhip_sample_measure_all": [_vp, _dp, _u64, _u64p],
     "qvmhip_sample": [_vp, _dp, _u64, _u64p],
+    "qvmhip_sample_ex": [_vp, _dp, _u64, C.c_int, C.c_double, _u64p, C.POINTER(C.c_uint8), _u64p],
     "qvmhip_read_amplitudes": [_vp, _u64, _u64, _dp],
     "qvmhip_accumulate": [_vp, _vp],
     "qvmhip_density_diagonal": [_vp, _u32, _dp],

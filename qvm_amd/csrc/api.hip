@@ -80,6 +80,7 @@ void free_state(qvmhip_state *s) {
     if (s->scan_tmp) hipFree(s->scan_tmp);
     if (s->shot_out) hipFree(s->shot_out);
     if (s->shot_u) hipFree(s->shot_u);
+    if (s->shot_near) hipFree(s->shot_near);
     if (s->host) hipHostFree(s->host);
     if (s->stage) hipHostFree(s->stage);
     if (s->stream && s->own_stream) hipStreamDestroy(s->stream);
@@ -499,6 +500,17 @@ int qvmhip_sample(qvmhip_state *s, const double *u, uint64_t nshots, uint64_t *o
     int rc = check_draws(u, nshots);
     if (rc) return rc;
     return sample_shots(s, u, nshots, out, 1);
+}
+
+int qvmhip_sample_ex(qvmhip_state *s, const double *u, uint64_t nshots, int rule, double eps, uint64_t *out,
+                     uint8_t *near_flags, uint64_t *n_near) {
+    if (!s || (nshots && (!u || !out))) return fail(QVMHIP_EINVAL, "null argument");
+    if (rule != QVMHIP_RULE_GT && rule != QVMHIP_RULE_GE) return fail(QVMHIP_EINVAL, "rule must be GT or GE");
+    if (!(eps >= 0.0) || !std::isfinite(eps)) return fail(QVMHIP_EINVAL, "eps must be finite and >= 0");
+    DeviceScope ds(s->device);
+    int rc = check_draws(u, nshots);
+    if (rc) return rc;
+    return sample_shots(s, u, nshots, out, rule == QVMHIP_RULE_GE ? 1 : 0, eps, near_flags, n_near);
 }
 
 int qvmhip_read_amplitudes(qvmhip_state *s, uint64_t start, uint64_t count, double *out) {

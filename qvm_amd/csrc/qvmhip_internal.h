@@ -143,6 +143,7 @@ struct qvmhip_state {
     uint64_t nchunks = 0;
     uint64_t *shot_out = nullptr;   // device shot results
     double *shot_u = nullptr;       // device uniforms
+    uint8_t *shot_near = nullptr;   // device near-boundary flags
     uint64_t shot_cap = 0;
     uint64_t *host_shots = nullptr; // pinned
     double *host_u = nullptr;       // pinned
@@ -197,7 +198,16 @@ int launch_scale(qvmhip_state *s, double factor);
 int reduce_probability(qvmhip_state *s, int mode, uint32_t q, double *out_dev);
 int reduce_norm2(qvmhip_state *s, double *out_dev);
 int launch_force_measure(qvmhip_state *s, uint32_t q, int bit, double inv_norm, int compiled_form);
-int sample_shots(qvmhip_state *s, const double *u, uint64_t nshots, uint64_t *out, int rule_ge);
+int sample_shots(qvmhip_state *s, const double *u, uint64_t nshots, uint64_t *out, int rule_ge, double eps = 0.0,
+                 uint8_t *near_flags = nullptr, uint64_t *n_near = nullptr);
+// Sampling on any device statevector: chunk sums and their exclusive prefix
+// (nchunks = ceil(len / 256) doubles each, scan_tmp ceil(nchunks / 1024)),
+// then one search per draw (all pointers on the device).
+int launch_chunk_prefix(hipStream_t st, int dtype, const void *amps, uint64_t len, double *chunk_sum,
+                        double *scan_tmp, double *chunk_pre);
+int launch_sample_search(hipStream_t st, int dtype, const void *amps, uint64_t len, const double *chunk_pre,
+                         const double *u, uint64_t nshots, int rule_ge, uint64_t *out, double eps,
+                         uint8_t *near_flags, unsigned long long *near_count);
 int collapse_to_basis(qvmhip_state *s, const uint64_t *index_dev);
 // ops_on_device = 0: public ops on the host (scheduled here); 1: an already
 // scheduled program on the device.

@@ -59,6 +59,13 @@ __device__ inline double prob(const vec2_t<T> &a) {
     return re * re + im * im;
 }
 
+// out[0, count) = v (clears counters with a kernel, so the clear can sit in a
+// captured graph).
+__global__ void k_fill_u64(uint64_t *__restrict__ out, uint64_t count, uint64_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = v;
+}
+
 // mode 0: all amplitudes; 1: bit q = 1 half; 2: bit q = 0 half.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_prob_partial(const vec2_t<T> *__restrict__ psi,
@@ -348,10 +355,15 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const double *__restrict_
 // first with cdf > p; len - 1 when none): the last chunk whose exclusive
 // prefix is below p by binary search, then the lanes' in-chunk prefix.
 // Called by a whole wavefront with the same p; the result is valid in
-// every lane.
+// every lane.  *near (every lane) reports whether p lies within eps of the
+// two cumulative values that decided the draw: C(b) and C(b - 1) as this
+// search formed them (C(b - 1) of a chunk's first index is the chunk's
+// exclusive prefix; for the clamped result, C(len - 1)).  eps <= 0 flags
+// nothing.
 template <typename T>
 __device__ inline uint64_t sample_one(const vec2_t<T> *__restrict__ psi, uint64_t len, uint64_t nchunks,
-                                      const double *__restrict__ pre, double p, int rule_ge, int lane) {
+                                      const double *__restrict__ pre, double p, int rule_ge, int lane,
+                                      double eps, bool *near) {
     uint64_t lo = 0, hi = nchunks;  // invariant: answer in [lo, hi)
     while (hi - lo > 1) {
         const uint64_t mid = lo + (hi - lo) / 2;
@@ -362,46 +374,69 @@ __device__ inline uint64_t sample_one(const vec2_t<T> *__restrict__ psi, uint64_
         else
             hi = mid;
     }
+    double last = 0.0;
     for (uint64_t c = lo; c < nchunks; ++c) {
         double pr[4];
         chunk_lane_probs<T>(psi, len, c, lane, pr);
         const double local = ((pr[0] + pr[1]) + pr[2]) + pr[3];
         const double incl = wave_inclusive_scan(local, lane);
         const double up = __shfl_up(incl, 1, kWave);
+        const double pc = pre[c];
         double run = lane ? up : 0.0;  // exclusive lane offset
+        double C[4];
         int hit = -1;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             run += pr[k];
-            const double C = pre[c] + run;
-            const bool ok = rule_ge ? (C >= p) : (C > p);
+            C[k] = pc + run;
+            const bool ok = rule_ge ? (C[k] >= p) : (C[k] > p);
             const uint64_t idx = (c << kChunkLog2) + (uint64_t)lane * 4 + k;
             if (ok && hit < 0 && idx < len) hit = k;
         }
+        const double prev_lane = __shfl_up(C[3], 1, kWave);
         const unsigned long long mask = __ballot(hit >= 0);
         if (mask) {
             const int first = __ffsll((long long)mask) - 1;
+            double cb = 0.0, cp = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (k == hit) {
+                    cb = C[k];
+                    cp = k > 0 ? C[k > 0 ? k - 1 : 0] : (lane ? prev_lane : pc);
+                }
+            const bool nr = eps > 0.0 && (fabs(p - cb) < eps || fabs(p - cp) < eps);
+            *near = __shfl((int)nr, first, kWave) != 0;
             const int k = __shfl(hit, first, kWave);
             return (c << kChunkLog2) + (uint64_t)first * 4 + (uint64_t)k;
         }
+        last = __shfl(C[3], kWave - 1, kWave);
     }
+    *near = eps > 0.0 && fabs(p - last) < eps;
     return len - 1;
 }
 
-// One wavefront per draw.
+// One wavefront per draw.  near_flags (optional) receives one byte per draw,
+// near_count (optional) the number of near-boundary draws (added to).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_sample_search(const vec2_t<T> *__restrict__ psi,
                                                           uint64_t len, uint64_t nchunks,
                                                           const double *__restrict__ pre,
                                                           const double *__restrict__ u,
                                                           uint64_t nshots, int rule_ge,
-                                                          uint64_t *__restrict__ out) {
+                                                          uint64_t *__restrict__ out, double eps,
+                                                          uint8_t *__restrict__ near_flags,
+                                                          unsigned long long *__restrict__ near_count) {
     const int lane = threadIdx.x % kWave;
     const uint64_t sstep = (uint64_t)gridDim.x * (kBlock / kWave);
     for (uint64_t shot = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; shot < nshots;
          shot += sstep) {
-        const uint64_t result = sample_one<T>(psi, len, nchunks, pre, u[shot], rule_ge, lane);
-        if (lane == 0) out[shot] = result;
+        bool nr = false;
+        const uint64_t result = sample_one<T>(psi, len, nchunks, pre, u[shot], rule_ge, lane, eps, &nr);
+        if (lane == 0) {
+            out[shot] = result;
+            if (near_flags) near_flags[shot] = nr ? 1 : 0;
+            if (nr && near_count) atomicAdd(near_count, 1ull);
+        }
     }
 }
 
@@ -554,8 +589,9 @@ __global__ __launch_bounds__(kBlock) void k_bfinish_sample(const vec2_t<T> *__re
     for (uint64_t s = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; s < total; s += sstep) {
         const uint32_t i = (uint32_t)(s / a.shots);
         const uint64_t base = a.chunk_base[i];
+        bool nr = false;
         const uint64_t r = sample_one<T>(buf + a.inst_off[i], a.inst_len[i], a.chunk_base[i + 1] - base,
-                                         a.chunk_pre + base, a.u[s], 1, lane);
+                                         a.chunk_pre + base, a.u[s], 1, lane, 0.0, &nr);
         if (lane == 0) a.shot_out[s] = r;
     }
 }
@@ -684,76 +720,119 @@ int launch_force_measure(qvmhip_state *s, uint32_t q, int bit, double inv_norm, 
     return QVMHIP_OK;
 }
 
-// Builds chunk sums + exclusive chunk prefix in s->chunk_sum / s->chunk_pre.
-static int build_chunk_prefix(qvmhip_state *s) {
-    const uint64_t nch = s->nchunks;
+// Chunk sums + exclusive chunk prefix of a len-amplitude state at `amps`
+// into chunk_sum / chunk_pre (nchunks = ceil(len / 256) entries each);
+// scan_tmp holds ceil(nchunks / kScanSeg) segment totals.
+int launch_chunk_prefix(hipStream_t st, int dtype, const void *amps, uint64_t len, double *chunk_sum,
+                        double *scan_tmp, double *chunk_pre) {
+    const uint64_t nch = (len + (uint64_t(1) << kChunkLog2) - 1) >> kChunkLog2;
     const uint32_t g1 = (uint32_t)std::min<uint64_t>((nch + (kBlock / kWave) - 1) / (kBlock / kWave), kMaxGrid);
-    if (s->dtype == QVMHIP_F64)
-        hipLaunchKernelGGL(k_chunk_sums<double>, dim3(g1), dim3(kBlock), 0, s->stream,
-                           (const double2 *)s->amps, s->len, nch, s->chunk_sum);
+    if (dtype == QVMHIP_F64)
+        hipLaunchKernelGGL(k_chunk_sums<double>, dim3(g1), dim3(kBlock), 0, st, (const double2 *)amps, len, nch,
+                           chunk_sum);
     else
-        hipLaunchKernelGGL(k_chunk_sums<float>, dim3(g1), dim3(kBlock), 0, s->stream,
-                           (const float2 *)s->amps, s->len, nch, s->chunk_sum);
+        hipLaunchKernelGGL(k_chunk_sums<float>, dim3(g1), dim3(kBlock), 0, st, (const float2 *)amps, len, nch,
+                           chunk_sum);
     QH_LAUNCH_CHECK();
     const uint64_t nseg = (nch + kScanSeg - 1) / kScanSeg;
-    hipLaunchKernelGGL(k_scan_seg_totals, dim3((uint32_t)nseg), dim3(kBlock), 0, s->stream, s->chunk_sum,
-                       nch, s->scan_tmp);
+    hipLaunchKernelGGL(k_scan_seg_totals, dim3((uint32_t)nseg), dim3(kBlock), 0, st, chunk_sum, nch, scan_tmp);
     QH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scan_tops, dim3(1), dim3(kBlock), 0, s->stream, s->scan_tmp, nseg);
+    hipLaunchKernelGGL(k_scan_tops, dim3(1), dim3(kBlock), 0, st, scan_tmp, nseg);
     QH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scan_apply, dim3((uint32_t)nseg), dim3(kBlock), 0, s->stream, s->chunk_sum, nch,
-                       s->scan_tmp, s->chunk_pre);
+    hipLaunchKernelGGL(k_scan_apply, dim3((uint32_t)nseg), dim3(kBlock), 0, st, chunk_sum, nch, scan_tmp,
+                       chunk_pre);
     QH_LAUNCH_CHECK();
     return QVMHIP_OK;
+}
+
+// One search per draw over a prefix built by launch_chunk_prefix; u, out,
+// near_flags and near_count are device pointers (the last two optional).
+int launch_sample_search(hipStream_t st, int dtype, const void *amps, uint64_t len, const double *chunk_pre,
+                         const double *u, uint64_t nshots, int rule_ge, uint64_t *out, double eps,
+                         uint8_t *near_flags, unsigned long long *near_count) {
+    if (nshots == 0) return QVMHIP_OK;
+    const uint64_t nch = (len + (uint64_t(1) << kChunkLog2) - 1) >> kChunkLog2;
+    const uint64_t wpb = kBlock / kWave;
+    const uint32_t nb = (uint32_t)std::min<uint64_t>((nshots + wpb - 1) / wpb, kMaxGrid);
+    if (dtype == QVMHIP_F64)
+        hipLaunchKernelGGL(k_sample_search<double>, dim3(nb), dim3(kBlock), 0, st, (const double2 *)amps, len, nch,
+                           chunk_pre, u, nshots, rule_ge, out, eps, near_flags, near_count);
+    else
+        hipLaunchKernelGGL(k_sample_search<float>, dim3(nb), dim3(kBlock), 0, st, (const float2 *)amps, len, nch,
+                           chunk_pre, u, nshots, rule_ge, out, eps, near_flags, near_count);
+    QH_LAUNCH_CHECK();
+    return QVMHIP_OK;
+}
+
+static int build_chunk_prefix(qvmhip_state *s) {
+    return launch_chunk_prefix(s->stream, s->dtype, s->amps, s->len, s->chunk_sum, s->scan_tmp, s->chunk_pre);
 }
 
 static int ensure_shots(qvmhip_state *s, uint64_t nshots) {
     if (nshots <= s->shot_cap) return QVMHIP_OK;
     if (s->shot_out) hipFree(s->shot_out);
     if (s->shot_u) hipFree(s->shot_u);
+    if (s->shot_near) hipFree(s->shot_near);
     s->shot_out = nullptr;
     s->shot_u = nullptr;
+    s->shot_near = nullptr;
     s->shot_cap = 0;
     QH_CHECK(hipMalloc((void **)&s->shot_out, nshots * sizeof(uint64_t)));
     QH_CHECK(hipMalloc((void **)&s->shot_u, nshots * sizeof(double)));
+    QH_CHECK(hipMalloc((void **)&s->shot_near, nshots));
     s->shot_cap = nshots;
     return QVMHIP_OK;
 }
 
-// Draws nshots samples; u and out are host arrays (out may be null when the
-// caller only needs the device copy in s->shot_out).
-int sample_shots(qvmhip_state *s, const double *u, uint64_t nshots, uint64_t *out, int rule_ge) {
+// Draws nshots samples; u, out, near_flags and n_near are host pointers
+// (out may be null when the caller only needs the device copy in
+// s->shot_out; near_flags and n_near are optional).  Draws within eps of the
+// cumulative values that decided them are flagged and counted (eps <= 0:
+// none).
+int sample_shots(qvmhip_state *s, const double *u, uint64_t nshots, uint64_t *out, int rule_ge, double eps,
+                 uint8_t *near_flags, uint64_t *n_near) {
+    if (n_near) *n_near = 0;
     if (nshots == 0) return QVMHIP_OK;
-    // Draws and results travel through the pinned staging buffer in batches.
-    const uint64_t batch = std::min<uint64_t>(nshots, s->stage_bytes / 16);
+    // Draws, results and flags travel through the pinned staging buffer in
+    // batches of 17 bytes per draw.
+    const uint64_t batch = std::min<uint64_t>(nshots, s->stage_bytes / 17);
     int rc = ensure_shots(s, batch);
     if (rc) return rc;
     rc = build_chunk_prefix(s);
     if (rc) return rc;
+    unsigned long long *count_dev = (unsigned long long *)(s->red + kRedBlocks * 2 + 8);
+    if (n_near) {
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s->stream, (uint64_t *)count_dev, uint64_t(1),
+                           uint64_t(0));
+        QH_LAUNCH_CHECK();
+    }
     double *stage_u = (double *)s->stage;
-    uint64_t *stage_out = (uint64_t *)((char *)s->stage + s->stage_bytes / 2);
+    uint64_t *stage_out = (uint64_t *)((char *)s->stage + batch * 8);
+    uint8_t *stage_near = (uint8_t *)s->stage + batch * 16;
     for (uint64_t first = 0; first < nshots; first += batch) {
         const uint64_t c = std::min<uint64_t>(batch, nshots - first);
         QH_CHECK(hipStreamSynchronize(s->stream));  // staging buffer is free
         std::memcpy(stage_u, u + first, c * sizeof(double));
         QH_CHECK(hipMemcpyAsync(s->shot_u, stage_u, c * sizeof(double), hipMemcpyHostToDevice, s->stream));
-        const uint64_t wpb = kBlock / kWave;
-        const uint64_t nb = (c + wpb - 1) / wpb;
-        if (s->dtype == QVMHIP_F64)
-            hipLaunchKernelGGL(k_sample_search<double>, dim3((uint32_t)nb), dim3(kBlock), 0, s->stream,
-                               (const double2 *)s->amps, s->len, s->nchunks, s->chunk_pre, s->shot_u, c,
-                               rule_ge, s->shot_out);
-        else
-            hipLaunchKernelGGL(k_sample_search<float>, dim3((uint32_t)nb), dim3(kBlock), 0, s->stream,
-                               (const float2 *)s->amps, s->len, s->nchunks, s->chunk_pre, s->shot_u, c,
-                               rule_ge, s->shot_out);
-        QH_LAUNCH_CHECK();
-        if (out) {
-            QH_CHECK(hipMemcpyAsync(stage_out, s->shot_out, c * sizeof(uint64_t), hipMemcpyDeviceToHost,
-                                    s->stream));
+        rc = launch_sample_search(s->stream, s->dtype, s->amps, s->len, s->chunk_pre, s->shot_u, c, rule_ge,
+                                  s->shot_out, eps, near_flags ? s->shot_near : nullptr,
+                                  n_near ? count_dev : nullptr);
+        if (rc) return rc;
+        if (out || near_flags) {
+            if (out)
+                QH_CHECK(hipMemcpyAsync(stage_out, s->shot_out, c * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                        s->stream));
+            if (near_flags)
+                QH_CHECK(hipMemcpyAsync(stage_near, s->shot_near, c, hipMemcpyDeviceToHost, s->stream));
             QH_CHECK(hipStreamSynchronize(s->stream));
-            std::memcpy(out + first, stage_out, c * sizeof(uint64_t));
+            if (out) std::memcpy(out + first, stage_out, c * sizeof(uint64_t));
+            if (near_flags) std::memcpy(near_flags + first, stage_near, c);
         }
+    }
+    if (n_near) {
+        QH_CHECK(hipMemcpyAsync(s->host, count_dev, sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream));
+        QH_CHECK(hipStreamSynchronize(s->stream));
+        std::memcpy(n_near, s->host, sizeof(uint64_t));
     }
     return QVMHIP_OK;
 }
@@ -762,16 +841,8 @@ int sample_to_device(qvmhip_state *s, const double *u_dev, uint64_t nshots, uint
     if (nshots == 0) return QVMHIP_OK;
     int rc = build_chunk_prefix(s);
     if (rc) return rc;
-    const uint64_t wpb = kBlock / kWave;
-    const uint32_t nb = (uint32_t)std::min<uint64_t>((nshots + wpb - 1) / wpb, kMaxGrid);
-    if (s->dtype == QVMHIP_F64)
-        hipLaunchKernelGGL(k_sample_search<double>, dim3(nb), dim3(kBlock), 0, s->stream, (const double2 *)s->amps,
-                           s->len, s->nchunks, s->chunk_pre, u_dev, nshots, rule_ge, out_dev);
-    else
-        hipLaunchKernelGGL(k_sample_search<float>, dim3(nb), dim3(kBlock), 0, s->stream, (const float2 *)s->amps,
-                           s->len, s->nchunks, s->chunk_pre, u_dev, nshots, rule_ge, out_dev);
-    QH_LAUNCH_CHECK();
-    return QVMHIP_OK;
+    return launch_sample_search(s->stream, s->dtype, s->amps, s->len, s->chunk_pre, u_dev, nshots, rule_ge, out_dev,
+                                0.0, nullptr, nullptr);
 }
 
 int expectation_zz_to_device(qvmhip_state *s, const uint32_t *edges_dev, uint32_t nedges, double *out_dev) {

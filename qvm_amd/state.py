@@ -186,6 +186,29 @@ class HipPureState:
         N.check(N.lib().qvmhip_sample(self._h, N.dptr(u), u.size, N.u64ptr(out)))
         return out
 
+    def sample_ex(self, u, rule: str = "ge", eps: float = 1e-15):
+        """Sampling with boundary reporting (qvmhip_sample_ex).
+
+        rule "gt" is measure-all's C(b) > u (src/measurement.lisp:179-225),
+        "ge" the multi-shot cdf >= u (:246-288).  Returns (indices, near)
+        where near[i] is True when u[i] lies within eps of C(b) or C(b-1) as
+        the engine's FP64 prefix formed them; near.sum() equals the count the
+        engine reports.
+        """
+        u = np.ascontiguousarray(np.asarray(u, dtype=np.float64))
+        out = np.zeros(u.size, dtype=np.uint64)
+        near = np.zeros(u.size, dtype=np.uint8)
+        count = C.c_uint64()
+        r = N.RULE_GE if rule == "ge" else N.RULE_GT if rule == "gt" else None
+        if r is None:
+            raise ValueError("rule must be 'gt' or 'ge'")
+        N.check(N.lib().qvmhip_sample_ex(self._h, N.dptr(u), u.size, r, float(eps), N.u64ptr(out),
+                                         near.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(count)))
+        flags = near.astype(bool)
+        if int(flags.sum()) != count.value:
+            raise RuntimeError("near-boundary count disagrees with the flags")
+        return out, flags
+
     # -- amplitudes -------------------------------------------------------
     def read_amplitudes(self, start: int = 0, count: int | None = None) -> np.ndarray:
         count = self.length - start if count is None else count
