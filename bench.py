@@ -178,8 +178,12 @@ def main() -> int:
     }
     pmc = os.path.join(REPO, "profiles", f"pmc_{dom}_n30.json")
     if os.path.exists(pmc) and n == 30:
+        # HBM bytes per launch from the committed rocprofv3 --pmc summary of
+        # the same kernels (FETCH_SIZE / WRITE_SIZE scaled by a copy of known
+        # size); counters are not collected inside this timed run
         with open(pmc) as f:
             roofline["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+        roofline["traffic_source"] = f"profiles/pmc_{dom}_n30.json (rocprofv3 --pmc)"
     kernels = {k: {"launches": c, "total_ms": round(t, 2), "avg_ms": round(t / c, 4)} for k, (c, t) in prof.items()}
 
     out = {
@@ -207,6 +211,8 @@ def main() -> int:
         "parity": {"qft_uniform_max_abs_err": qft_err, "norm": norm_b, "shots_vs_closed_form": shot_parity},
     }
 
+    out["ceilings"] = ceilings(st, prof, n, HipPureState)
+
     t_sec = {}
     if not args.no_paths:
         t0 = time.perf_counter()
@@ -228,6 +234,41 @@ def main() -> int:
     out["section_wall_s"] = {k: round(v, 1) for k, v in t_sec.items()}
     print(json.dumps(out))
     return 0
+
+
+def ceilings(st, prof, n, HipPureState):
+    """What the part streams at, measured in this run: a device-to-device
+    copy of the whole state (hipMemcpyAsync, read + write), and the in-place
+    sweep of the full-diagonal gate kernel (RZ: one read and one write per
+    amplitude, four multiplies), from the timed steps' own launches; plus the
+    device as rocminfo names it."""
+    import subprocess
+    res = {}
+    other = HipPureState(n, "f64")
+    other.copy_from(st)
+    other.sync()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        other.copy_from(st)
+    other.sync()
+    dt = (time.perf_counter() - t0) / reps
+    other.close()
+    nbytes = 2 * (1 << n) * 16
+    res["d2d_copy_GBps"] = round(nbytes / dt / 1e9, 1)
+    if "diag_full" in prof:
+        c, t = prof["diag_full"]
+        res["inplace_sweep_GBps"] = round(nbytes / (t / c * 1e-3) / 1e9, 1)
+        res["inplace_sweep_kernel"] = "qvmhip::k_diag_full<double, true, 1> (RZ)"
+    try:
+        txt = subprocess.run(["rocminfo"], capture_output=True, text=True, timeout=60).stdout
+        names = [ln.split(":", 1)[1].strip() for ln in txt.splitlines() if "Marketing Name" in ln]
+        cus = [ln.split(":", 1)[1].strip() for ln in txt.splitlines() if "Compute Unit" in ln]
+        gfx = [ln.split(":", 1)[1].strip() for ln in txt.splitlines() if ln.strip().startswith("Name:") and "gfx" in ln]
+        res["rocminfo"] = {"marketing_names": names, "compute_units": cus, "gfx": gfx}
+    except (OSError, subprocess.SubprocessError) as e:
+        res["rocminfo"] = f"unavailable: {e}"
+    return res
 
 
 def run_paths(args, n, ga, gb, nat_a, upd_a, upd_b, u, st, HipPureState, P, np):
@@ -444,47 +485,23 @@ def batch_key(args):
 
 
 def learned_key(args):
-    """Config 5 in the headline line: the shipped cost model (trained on
-    launch timings the engine recorded, through the engine's MLP kernels)
-    prices each candidate plan of a depth-60 random circuit (60 % Haar U3 /
-    40 % CNOT-or-CZ) at n = 30 and picks one; its time is set against the
-    best fixed rule (tiled at the default width) and fixed-width fusion
-    (k = 4), and its state against the plain FP64 state."""
+    """Config 5 in the headline line: the shipped cost model (an MLP trained
+    on launch timings the engine recorded, through the engine's own HIP
+    kernels) picks, for each circuit of a workload whose best plan differs by
+    circuit (depth-20 random 60 % Haar U3 / 40 % CNOT-or-CZ circuits at n = 30
+    in FP64 and FP32, dense 6-qubit blocks at n = 28, a ZZ-phase heavy circuit
+    at n = 30, a small state), among plain, fused k = 2..6, tiled (three
+    widths x three qubit permutations) and hybrid plans.  Its picks are timed
+    against the fixed rule (tile passes at the default width) and the
+    analytic bytes + issue model, and every picked plan's state is compared
+    with the plain state."""
     from qvm_amd import circuits as C
     from qvm_amd import costmodel as CM
-    from qvm_amd import gates as G
-    from qvm_amd import planner as P
-    from qvm_amd.state import HipPureState
-    n = 30
-    gates = C.random_mix(n, 60, seed=501)
-    upd = C.amp_updates(gates, n)
-    cm = CM.CostModel.shared()
-    ref = HipPureState(n, "f64")
-    ref.sync()
-    t0 = time.perf_counter()
-    for U, q in gates:
-        ref.apply_matrix(U, G.nat_tuple(*q))
-    ref.sync()
-    t_plain = time.perf_counter() - t0
-    res = {"n": n, "gates": len(gates), "amp_updates": upd, "plain_f64_s": t_plain}
-    for dtype in ("f64", "f32"):
-        cands = CM.candidate_plans(gates, n, dtype)
-        name, plan, pred = cm.choose(gates, n, dtype, cands)
-        fixed = f"tiled_m{P.default_tile_m(dtype)}"
-        st = HipPureState(n, dtype, fma_tiles=True)
-        times = {}
-        for key in [k for k in (fixed, "fused_k4") if k != name] + [name]:
-            times[key] = _timed_plan(st, cands[key], P)  # the chosen plan runs last: st holds its state
-        r = {"choice": name, "predicted_ms": {k: round(v, 2) for k, v in pred.items()},
-             "measured_s": times, "speedup_vs_fixed_rule": times[fixed] / times[name],
-             "speedup_vs_fused_k4": times["fused_k4"] / times[name], "speedup_vs_plain_f64": t_plain / times[name],
-             "amp_updates_per_s": upd / times[name], **_vs(st, ref)}
-        res[dtype] = r
-        log(f"learned {dtype}: {name} {times[name]:.3f} s (fixed rule {times[fixed]:.3f} s, k4 "
-            f"{times['fused_k4']:.3f} s, plain f64 {t_plain:.2f} s)")
-        st.close()
-    ref.close()
-    return res
+    rep = CM.evaluate_planners(C.mixed_workload("bench"), CM.CostModel.shared(), CM.AnalyticModel(), log=log)
+    tot = rep["totals_ms"]
+    rep["speedup_vs_fixed_rule"] = tot["fixed_rule"] / tot["learned"]
+    rep["speedup_vs_analytic"] = tot["analytic"] / tot["learned"]
+    return rep
 
 
 def precision(args) -> int:
@@ -691,7 +708,7 @@ def learned(args) -> int:
         a = HipPureState(n, dtype, fma_tiles=True)
         ref = HipPureState(n, dtype, fma_tiles=True)
         t_learned = timed(a, plan)
-        t_k4 = timed(ref, cands["fused_k4"])
+        t_k4 = timed(ref, cands["fused_k4"])  # fixed-width fusion
         t_plain = timed(ref, cands["plain"])  # ref now holds the plain result
         nrm = a.norm()
         rec = {"n": n, "dtype": dtype, "gates": len(gates), "amp_updates": upd, "learned_choice": name,

@@ -2,8 +2,10 @@
 
 This module is the Python twin of the Lisp CFFI binding a qvm maintainer
 would add: one foreign function per entry point, same argument order.  It
-loads the in-tree ``qvm_amd/_lib/libqvmhip.so`` and raises if it is missing;
-there is no CPU fallback anywhere in the package.
+loads the in-tree ``qvm_amd/_lib/libqvmhip.so`` and raises if it is missing
+or was built from other sources than the tree holds (the build writes a hash
+of its sources next to the library); there is no CPU fallback anywhere in
+the package.
 """
 from __future__ import annotations
 
@@ -166,6 +168,12 @@ def lib():
             raise ImportError(
                 f"qvmhip library not built: {LIB_PATH} is missing (run __graft_entry__.build() "
                 "or python qvm_amd/build.py)")
+        from . import build as B
+        stamp = open(B.STAMP).read().strip() if os.path.exists(B.STAMP) else ""
+        if stamp != B.source_hash():
+            raise ImportError(
+                f"qvmhip library {LIB_PATH} was not built from the sources in this tree (its build stamp "
+                "differs from their hash): rebuild with __graft_entry__.build() or python qvm_amd/build.py")
         L = C.CDLL(LIB_PATH)
         for name, args in SIGNATURES.items():
             f = getattr(L, name)

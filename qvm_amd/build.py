@@ -40,6 +40,25 @@ COMMON = [
 ]
 
 
+STAMP = os.path.join(OUT_DIR, "libqvmhip.stamp")
+
+
+def source_hash() -> str:
+    """SHA-256 of everything the library is built from: the HIP sources,
+    the headers and the compile flags."""
+    import hashlib
+    h = hashlib.sha256()
+    files = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".h")))
+    files += [os.path.join(REPO, "include", f) for f in sorted(os.listdir(os.path.join(REPO, "include")))]
+    for path in files:
+        h.update(os.path.relpath(path, REPO).encode())
+        with open(path, "rb") as f:
+            h.update(f.read())
+    flags = [f.replace(REPO, "<repo>") for f in COMMON]  # the tree may sit anywhere
+    h.update(repr((ARCH, flags, sorted(SOURCES.items()))).encode())
+    return h.hexdigest()
+
+
 def hipcc() -> str:
     for p in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc"):
         if p and os.path.exists(p):
@@ -66,6 +85,8 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    with open(STAMP, "w") as f:
+        f.write(source_hash() + "\n")
     if verbose:
         print(f"built {LIB}")
     return LIB

@@ -194,6 +194,77 @@ def random_mix(n: int, depth: int, seed: int = 5) -> List[Gate]:
     return out
 
 
+def haar_unitary(k: int, rng: SplitMix64) -> np.ndarray:
+    """Haar-random 2^k x 2^k unitary (Ginibre QR with the phase fix)."""
+    d = 1 << k
+    Z = np.array([[complex(rng.normal(), rng.normal()) for _ in range(d)] for _ in range(d)]) / math.sqrt(2)
+    Q, R = np.linalg.qr(Z)
+    r = np.diag(R)
+    return Q * (r / np.abs(r))
+
+
+def dense_blocks(n: int, depth: int, k: int, seed: int = 7) -> List[Gate]:
+    """depth layers of a Haar 1-qubit gate on every qubit followed by
+    floor(n / k) Haar k-qubit unitaries on a seeded random partition of the
+    qubits (dense k-qubit blocks, as a fusing compiler would emit)."""
+    rng = SplitMix64(seed)
+    out: List[Gate] = []
+    for _ in range(depth):
+        for q in range(n):
+            out.append((haar_u2(rng), [q]))
+        perm = list(range(n))
+        for i in range(n - 1, 0, -1):
+            j = rng.below(i + 1)
+            perm[i], perm[j] = perm[j], perm[i]
+        for b in range(n // k):
+            out.append((haar_unitary(k, rng), perm[b * k:(b + 1) * k]))
+    return out
+
+
+def diagonal_heavy(n: int, depth: int, seed: int = 8) -> List[Gate]:
+    """QAOA-like: H on every qubit, then per layer ZZ phases (RZZ) on 2n
+    random pairs and RZ, then RX, on every qubit: mostly diagonal gates."""
+    rng = SplitMix64(seed)
+    out: List[Gate] = [(G.gate_matrix("H"), [q]) for q in range(n)]
+    for _ in range(depth):
+        for _ in range(2 * n):
+            a = rng.below(n)
+            b = rng.below(n - 1)
+            b = b + 1 if b >= a else b
+            out.append((G.gate_matrix("RZZ", [math.pi * rng.uniform()]), [a, b]))
+        for q in range(n):
+            out.append((G.gate_matrix("RZ", [2 * math.pi * rng.uniform()]), [q]))
+        for q in range(n):
+            out.append((G.gate_matrix("RX", [math.pi * rng.uniform()]), [q]))
+    return out
+
+
+def mixed_workload(scale: str = "test", seed: int = 900):
+    """Circuits on which the best plan differs: (name, n, dtype, gates).
+    "test" keeps every state at <= 2^26 amplitudes; "bench" has the
+    n = 28..30 members of the families and one small state."""
+    if scale == "bench":
+        spec = [("random_mix", 30, "f64", 20), ("random_mix", 30, "f32", 20), ("dense_k6", 28, "f64", 4),
+                ("diagonal_heavy", 30, "f64", 3), ("layered_small", 18, "f64", 40)]
+    else:
+        spec = [("random_mix", 26, "f64", 30), ("random_mix", 24, "f32", 30), ("dense_k6", 24, "f64", 5),
+                ("dense_k5", 26, "f32", 5), ("diagonal_heavy", 26, "f64", 5), ("layered_small", 16, "f64", 30),
+                ("dense_k6", 18, "f32", 8)]
+    out = []
+    for j, (name, n, dtype, depth) in enumerate(spec):
+        s = seed + j
+        if name == "random_mix":
+            gates = random_mix(n, depth, seed=s)
+        elif name.startswith("dense_k"):
+            gates = dense_blocks(n, depth, int(name[-1]), seed=s)
+        elif name == "diagonal_heavy":
+            gates = diagonal_heavy(n, depth, seed=s)
+        else:
+            gates = layered(n, depth, seed=s)
+        out.append((f"{name}_n{n}_{dtype}", n, dtype, gates))
+    return out
+
+
 def amp_updates(gates: List[Gate], n: int) -> int:
     """Sum of changed(g): amplitudes each gate can modify."""
     return sum(G.changed_amplitudes(U, n) for U, _ in gates)

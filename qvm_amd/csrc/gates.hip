@@ -60,6 +60,10 @@ struct SparseArgs {
     uint64_t src[64];  // permutation: source offsets
     T dr[64];
     T di[64];
+    // general permutation, entries in cycle order: bit 0 = first entry of a
+    // cycle (its destination's old value is kept), bit 1 = last entry (it
+    // takes the kept value instead of reading its source)
+    uint8_t flag[64];
 };
 
 // ---------------------------------------------------------------------------
@@ -349,9 +353,17 @@ __global__ __launch_bounds__(kBlock) void k_perm(vec2_t<T> *__restrict__ psi,
                 stg<NTM>(&psi[base + a.off[0]], x0);
                 stg<NTM>(&psi[base + a.off[1]], x1);
             } else {
-                V tmp[64];
-                for (uint32_t e = 0; e < a.nent; ++e) tmp[e] = psi[base + a.src[e]];
-                for (uint32_t e = 0; e < a.nent; ++e) psi[base + a.off[e]] = tmp[e];
+                // cycle by cycle with one kept value: within a cycle each
+                // destination's source is the next entry's destination, read
+                // before that entry overwrites it
+                V kept;
+                kept.x = kept.y = T(0);
+                for (uint32_t e = 0; e < a.nent; ++e) {
+                    const uint32_t f = a.flag[e];
+                    if (f & 1u) kept = psi[base + a.off[e]];
+                    const V v = (f & 2u) ? kept : psi[base + a.src[e]];
+                    psi[base + a.off[e]] = v;
+                }
             }
         }
     }
@@ -492,6 +504,15 @@ GateSplit split_gate(const cd *U, uint32_t k, const uint32_t *qubits, bool detec
             ident = false;
         }
         if (!(nonzero == 1 && ones == 1)) perm = false;
+    }
+    if (perm) {
+        // a permutation gate moves every amplitude somewhere: rows taking the
+        // same column (a singular 0/1 matrix) go to the dense kernel
+        std::vector<bool> used(Dr, false);
+        for (uint32_t r = 0; r < Dr && perm; ++r) {
+            if (used[s.perm[r]]) perm = false;
+            used[s.perm[r]] = true;
+        }
     }
     if (ident)
         s.kind = 0;
@@ -690,11 +711,23 @@ int run_perm(qvmhip_state *s, const GateSplit &gs) {
     SparseArgs<T> a;
     std::memset(&a, 0, sizeof(a));
     make_group_spec(a.g, gs, s->n);
-    for (uint32_t j = 0; j < Dr; ++j) {
-        if (gs.perm[j] == j) continue;
-        a.off[a.nent] = deposit(j, gs.tq, gs.kt);
-        a.src[a.nent] = deposit(gs.perm[j], gs.tq, gs.kt);
-        ++a.nent;
+    // moved entries in cycle order: d0 <- perm[d0] = d1 <- perm[d1] = d2 ...
+    // <- d0 (psi'[j] = psi[perm[j]])
+    std::vector<bool> seen(Dr, false);
+    for (uint32_t j0 = 0; j0 < Dr; ++j0) {
+        if (gs.perm[j0] == j0 || seen[j0]) continue;
+        uint32_t d = j0;
+        const uint32_t first = a.nent;
+        do {
+            seen[d] = true;
+            a.off[a.nent] = deposit(d, gs.tq, gs.kt);
+            a.src[a.nent] = deposit(gs.perm[d], gs.tq, gs.kt);
+            a.flag[a.nent] = 0;
+            ++a.nent;
+            d = gs.perm[d];
+        } while (d != j0);
+        a.flag[first] |= 1u;
+        a.flag[a.nent - 1] |= 2u;
     }
     if (a.nent == 0) return QVMHIP_OK;
     if (a.nent == 2 && s->n >= 8 && a.g.nsorted && a.g.sorted[a.g.nsorted - 1] < 6 &&

@@ -874,26 +874,16 @@ int collapse_to_basis(qvmhip_state *s, const uint64_t *index_dev) {
 
 int expectation_zz(qvmhip_state *s, const uint32_t *edges, uint32_t nedges, double *out) {
     if (nedges > 256) return fail(QVMHIP_EINVAL, "at most 256 edges");
-    uint32_t *dedges = nullptr;
-    QH_CHECK(hipMalloc((void **)&dedges, sizeof(uint32_t) * 2 * std::max<uint32_t>(nedges, 1)));
+    // the edges go to the state's matrix slot (2 KiB of its 128 KiB), in
+    // stream order after every earlier use of the slot: no allocation
+    uint32_t *dedges = (uint32_t *)s->mat;
     if (nedges)
-        QH_CHECK(hipMemcpyAsync(dedges, edges, sizeof(uint32_t) * 2 * nedges, hipMemcpyHostToDevice,
-                                s->stream));
-    const uint32_t grid = red_grid(s->len);
-    if (s->dtype == QVMHIP_F64)
-        hipLaunchKernelGGL(k_zz_partial<double>, dim3(grid), dim3(kBlock), 0, s->stream,
-                           (const double2 *)s->amps, s->len, dedges, nedges, s->red);
-    else
-        hipLaunchKernelGGL(k_zz_partial<float>, dim3(grid), dim3(kBlock), 0, s->stream,
-                           (const float2 *)s->amps, s->len, dedges, nedges, s->red);
-    QH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(kBlock), 0, s->stream, s->red, (int)grid, 1,
-                       s->red + kRedBlocks * 2);
-    QH_LAUNCH_CHECK();
+        QH_CHECK(hipMemcpyAsync(dedges, edges, sizeof(uint32_t) * 2 * nedges, hipMemcpyHostToDevice, s->stream));
+    int rc = expectation_zz_to_device(s, dedges, nedges, s->red + kRedBlocks * 2);
+    if (rc) return rc;
     QH_CHECK(hipMemcpyAsync(s->host, s->red + kRedBlocks * 2, sizeof(double), hipMemcpyDeviceToHost,
                             s->stream));
     QH_CHECK(hipStreamSynchronize(s->stream));
-    hipFree(dedges);
     *out = s->host[0];
     return QVMHIP_OK;
 }

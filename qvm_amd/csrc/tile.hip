@@ -514,7 +514,7 @@ __device__ __forceinline__ void reg_cut_phase(vec2_t<T> (&x)[R], const qvmhip_ti
         sa[t] = v;
     }
     uint32_t c = 0, A = 0;
-#pragma unroll 4  // fully unrolled, the pair terms were computed ahead and spilled
+#pragma unroll 4  // by four: one scalar load per four pairs (a full unroll computed the pair terms ahead and spilled)
     for (uint32_t i = 0; i < E; ++i) {
         {
             const uint32_t cls = b[72 + i], e0 = b[8 + 2 * i], e1 = b[9 + 2 * i];

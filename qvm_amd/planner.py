@@ -327,12 +327,17 @@ def _merge_runs_1q(gates: List[Gate]) -> List[Gate]:
 
 
 def plan_tiled(gates: List[Gate], nq: int, m: int, low: int = 4, merge_1q: bool = False,
-               reorder: bool = True) -> List[PlanItem]:
-    """LDS tile passes of up to m qubits that always include qubits 0..low-1.
+               reorder: bool = True, wide: str = "gate") -> List[PlanItem]:
+    """Register-tile passes of up to m qubits that always include qubits
+    0..low-1 (low sets how many low-order qubits every block keeps as one
+    contiguous run; the other tile qubits are chosen per pass).
 
     reorder=False keeps the original gate order, so the FP64 result is
     bit-identical to the plain path; reorder=True lets commuting gates move
-    into earlier passes (far fewer passes, results equal to rounding)."""
+    into earlier passes (far fewer passes, results equal to rounding).
+    Gates on more than 2 qubits run outside the passes: as plain gates
+    (wide="gate") or, for 3..6 qubits, as dense blocks on the matrix cores
+    (wide="fused")."""
     if merge_1q:
         gates = _merge_runs_1q(gates)
     if nq < 8:
@@ -353,7 +358,12 @@ def plan_tiled(gates: List[Gate], nq: int, m: int, low: int = 4, merge_1q: bool 
     for kind, T, members in groups:
         if kind == "single":
             i = members[0]
-            plan.append(PlanItem("gate", list(G.nat_tuple(*gates[i][1])), U=gates[i][0], gates=[i]))
+            U, q = gates[i]
+            if wide == "fused" and 3 <= len(q) <= 6:
+                # the fused kernel's convention: block[j] = matrix index bit j
+                plan.append(PlanItem("fused", list(G.nat_tuple(*q)), U=U, gates=[i]))
+            else:
+                plan.append(PlanItem("gate", list(G.nat_tuple(*q)), U=U, gates=[i]))
             continue
         # pad the tile to m qubits with the lowest unused qubits (contiguity)
         T = set(T)
@@ -411,6 +421,11 @@ def default_tile_m(dtype: str) -> int:
 
 def default_low(dtype: str) -> int:
     return 4 if dtype == "f64" else 5
+
+
+def max_tile_m(dtype: str) -> int:
+    """Widest tile the engine runs (FP64 12, FP32 13 qubits)."""
+    return 12 if dtype == "f64" else 13
 
 
 def compile_gates(state, gates: List[Gate], mode: str = "tiled", use_graph: bool = True, m: int = 0,

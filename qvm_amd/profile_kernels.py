@@ -28,7 +28,8 @@ def main(argv=None) -> int:
     ap.add_argument("--reps", type=int, default=8)
     ap.add_argument("--fma", action="store_true", help="tile passes with FMA-contracted products")
     ap.add_argument("--tile-m", type=int, default=0, help="tile width (default: the planner's)")
-    ap.add_argument("--qubits", default="", help="comma-separated target qubits, cycled (default: random)")
+    ap.add_argument("--qubits", default="",
+                    help="comma-separated target qubits, cycled (fused blocks: the first k; default: random)")
     a = ap.parse_args(argv)
     fixed = [int(x) for x in a.qubits.split(",") if x.strip()]
     st = HipPureState(a.n, a.dtype, fma_tiles=a.fma)
@@ -48,7 +49,8 @@ def main(argv=None) -> int:
         elif a.kernel.startswith("fused"):
             k = int(a.kernel[-1])
             Z = rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k))
-            st.apply_fused(np.linalg.qr(Z)[0], [int(x) for x in rng.choice(a.n, k, replace=False)])
+            qs = fixed[:k] if len(fixed) >= k else [int(x) for x in rng.choice(a.n, k, replace=False)]
+            st.apply_fused(np.linalg.qr(Z)[0], qs)
         elif a.kernel == "tile":
             from . import circuits as C
             from . import planner as P

@@ -46,6 +46,33 @@ def test_qaoa_batch_matches_oracle(oracle, fma):
     b.close()
 
 
+def test_qaoa_batch_large_instances_match_oracle(oracle):
+    """Instances of 18, 20 and 22 qubits (the upper half of the bench's
+    range, several tile passes each, FMA tiles at the autotuned width's
+    range) against the oracle: amplitudes, <H_C> and 1024 shots each."""
+    from qvm_amd.batch import qaoa_batch
+    rng = CI.SplitMix64(17)
+    inst = []
+    for n in (18, 20, 22):
+        edges = CI.random_regular3(n, rng)
+        p = 2 + rng.below(3)
+        inst.append((n, edges, [3.0 * rng.uniform() for _ in range(p)], [1.5 * rng.uniform() for _ in range(p)]))
+    shots = 1024
+    for m in (10, 12):
+        b, circuits, u = qaoa_batch(inst, "f64", shots=shots, seed=31, fma_tiles=True, tile_m=m)
+        b.run()
+        hc, samples = b.results()
+        for i, (n, edges, gam, _) in enumerate(inst):
+            say(f"large batch instance n={n} p={len(gam)} m={m}")
+            ref = oracle_state(oracle, n, circuits[i])
+            assert np.max(np.abs(b.amplitudes(i) - ref)) < 1e-12
+            assert abs(hc[i] - zz_expectation(ref, edges)) < 1e-11
+            ui = u[i * shots:(i + 1) * shots]
+            r = check_draws(oracle.cdf(ref), ui, samples[i], oracle.sample_multiple(ref, ui))
+            say(f"  shots: {r}")
+        b.close()
+
+
 def test_qaoa_batch_f32_and_eager(oracle):
     from qvm_amd.batch import qaoa_batch
     inst = CI.qaoa_instances(6, seed=5, nmin=10, nmax=14)

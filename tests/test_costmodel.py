@@ -39,12 +39,33 @@ def test_launch_features_and_candidates():
     n = 14
     gates = CI.random_mix(n, 4, seed=2)
     cands = CM.candidate_plans(gates, n, "f64")
-    assert {"plain", "fused_k4", "tiled_m10"} <= set(cands)
+    assert {"plain", "fused_k4", CM.fixed_rule_name("f64"), "tiled_m10_l3", "tiled_m12_l5", "hybrid_m11"} <= set(cands)
     for name, plan in cands.items():
         for it in plan:
             f = CM.launch_features(it, n, "f64")
-            assert f.shape == (CM.DIN,) and np.all(np.isfinite(f)) and f.max() <= 1.0 + 1e-6
+            assert f.shape == (CM.DIN,) and np.all(np.isfinite(f)) and f.max() <= 2.0 + 1e-6
     assert len(cands["fused_k5"]) < len(cands["plain"])
+    # the low run is the qubit permutation into the block: every pass of
+    # tiled_m*_l{L} keeps qubits 0..L-1
+    for L in (3, 4, 5):
+        for it in cands[f"tiled_m11_l{L}"]:
+            if it.kind == "tile":
+                assert CM.low_run(it.qubits) >= L
+
+
+def test_hybrid_plan_runs_wide_gates_on_the_matrix_cores():
+    n = 14
+    gates = CI.dense_blocks(n, 2, 5, seed=3)
+    hyb = CM.candidate_plans(gates, n, "f64")["hybrid_m11"]
+    wide = [it for it in hyb if it.kind != "tile"]
+    assert wide and all(it.kind == "fused" and len(it.qubits) == 5 for it in wide)
+    # the analytic model prices from first principles: the fused k=6 block
+    # is MFMA-bound at FP64, a 1q gate is HBM-bound
+    am = CM.AnalyticModel()
+    t1 = am.seconds("gate", 30, "f64", 1, changed=1 << 30)
+    t6 = am.seconds("fused", 30, "f64", 6)
+    assert abs(t1 - (am.OVERHEAD_S + 2 * 16 * 2 ** 30 / am.STREAM_BPS)) < 1e-12
+    assert t6 > am.seconds("fused", 30, "f64", 4) >= t1
 
 
 @pytest.mark.gpu
@@ -77,38 +98,36 @@ def test_mlp_training_fits():
 
 
 @pytest.mark.gpu
-def test_learned_planner_beats_fixed_width():
-    """Train on engine-recorded timings, then on held-out circuits compare the
-    measured time of the model's choice with the fixed-width (k=4) fusion
-    planner and with plain gate-by-gate application."""
-    from qvm_amd.state import HipPureState
-    n = 22
-    recs = CM.collect([n], ["f64", "f32"], per_config=70, seed=1)
+def test_cost_model_trains_on_engine_timings():
+    """Timings the engine records train the network on the GPU to a useful
+    accuracy (in-sample median relative error below 50 %)."""
+    recs = CM.collect([20], ["f64", "f32"], per_config=60, seed=1)
     cm = CM.CostModel.trained(recs, epochs=2500)
     X, T = CM.dataset(recs)
-    pred = cm.mlp.predict(X)
-    rel = np.exp(np.abs(pred - T)) - 1
+    rel = np.exp(np.abs(cm.mlp.predict(X) - T)) - 1
     print(f"cost model: {len(recs)} launches, median relative error {np.median(rel):.2f}", flush=True)
     assert np.median(rel) < 0.5
-    totals = {"learned": 0.0, "fixed_k4": 0.0, "plain": 0.0}
-    for i, dtype in enumerate(["f64", "f32", "f64"]):
-        gates = CI.random_mix(n, 12, seed=100 + i)
-        st = HipPureState(n, dtype, fma_tiles=True)
-        cands = CM.candidate_plans(gates, n, dtype)
-        name, plan, costs = cm.choose(gates, n, dtype, cands)
-        measured = {}
-        for key in set([name, "fused_k4", "plain"]):
-            prog = P.build_program(st, cands[key], use_graph=True)
-            prog.run()
-            st.sync()
-            measured[key] = prog.time(3)
-            prog.close()
-        totals["learned"] += measured[name]
-        totals["fixed_k4"] += measured["fused_k4"]
-        totals["plain"] += measured["plain"]
-        print(f"circuit {i} {dtype}: learned picks {name} ({measured[name]:.3f} ms), "
-              f"fixed k=4 {measured['fused_k4']:.3f} ms, plain {measured['plain']:.3f} ms", flush=True)
-        st.close()
-    print(f"totals: {totals}", flush=True)
-    assert totals["learned"] <= 1.05 * min(totals["fixed_k4"], totals["plain"])
     cm.mlp.close()
+
+
+@pytest.mark.gpu
+def test_learned_planner_ablation():
+    """On a held-out workload whose best plan differs by circuit (random
+    1q/2q mixes, dense 5- and 6-qubit blocks, ZZ-phase heavy circuits, small
+    states), every candidate is timed.  The shipped model's picks must run
+    faster in total than the fixed rule's (tile passes at the default width)
+    and no slower than the analytic model's (3 % timing slack), must take at
+    least half of the gain the best candidates offer over the fixed rule, and
+    every picked plan's state must equal the plain state."""
+    cm = CM.CostModel.shared()
+    rep = CM.evaluate_planners(CI.mixed_workload("test", seed=1900), cm, CM.AnalyticModel(), measure_all=True,
+                               log=lambda *a: print(*a, flush=True))
+    tot = rep["totals_ms"]
+    print(f"totals (ms): {tot}", flush=True)
+    for rec in rep["circuits"]:
+        tol = 1e-11 if rec["dtype"] == "f64" else 1e-6
+        for key, d in rec["max_abs_diff_vs_plain"].items():
+            assert d < tol, (rec["circuit"], key, d)
+    assert tot["learned"] < tot["fixed_rule"]
+    assert tot["learned"] <= 1.03 * tot["analytic"]
+    assert tot["fixed_rule"] - tot["learned"] >= 0.5 * (tot["fixed_rule"] - tot["best_candidate"])

@@ -105,3 +105,37 @@ def test_transpositions_low_and_high_bits(S, oracle, dtype):
         oracle.apply_matrix(ref, U, G.nat_tuple(*qs))
     check(st, ref, dtype)
     st.close()
+
+
+def test_general_permutations_by_cycles(S, oracle):
+    """Permutation gates with several moved entries (3-cycles, a 4-cycle plus
+    a transposition, random permutations of 3 qubits, a controlled 3-cycle)
+    run cycle by cycle with one kept value; a singular 0/1 matrix (two rows
+    taking one column) goes to the dense kernel.  Bit-exact against the
+    oracle."""
+    rng = np.random.default_rng(57)
+    n = 13
+    psi = rand_state(n, rng)
+    st = S(n, "f64")
+    st.write_amplitudes(psi)
+    ref = psi.copy()
+
+    def pmat(perm):
+        P = np.zeros((len(perm), len(perm)), dtype=np.complex128)
+        for j, p in enumerate(perm):
+            P[j, p] = 1
+        return P
+
+    mats = [pmat([1, 2, 0, 3]), pmat([3, 0, 1, 2]), pmat([1, 0, 3, 2, 5, 6, 7, 4])]
+    mats += [pmat(list(rng.permutation(8))) for _ in range(4)]
+    ctrl3 = np.eye(8, dtype=np.complex128)
+    ctrl3[4:, 4:] = pmat([1, 2, 0, 3])
+    mats.append(ctrl3)
+    sing = np.array([[1, 0], [1, 0]], dtype=np.complex128)
+    for U in mats + [sing]:
+        k = int(U.shape[0]).bit_length() - 1
+        qs = [int(x) for x in rng.choice(n, k, replace=False)]
+        st.apply_matrix(U, qs)
+        oracle.apply_matrix(ref, U, qs)
+        np.testing.assert_array_equal(st.amplitudes(), ref)
+    st.close()

@@ -596,3 +596,20 @@ def test_tile_schedule_matches_op_list(dtype, m, fma):
     ops = [_op(N.TILE_DENSE1, q, 0, 0, np.eye(2).reshape(-1)) for q in hi]
     ops += [_op(N.TILE_DIAG1, 0, 0, 0, np.array([1, 1j]))]
     assert all(o.kind != N.TILE_REMAP for o in N.tile_schedule(dtype, m, ops, fma))
+
+
+def test_library_load_refuses_a_stale_build(monkeypatch):
+    """The library carries the hash of the sources it was built from; a tree
+    whose sources hash differently refuses to load it (no silently stale
+    binary on a machine that runs the prebuilt library)."""
+    from qvm_amd import _native as N
+    from qvm_amd import build as B
+    assert open(B.STAMP).read().strip() == B.source_hash()
+    saved = N._lib
+    try:
+        N._lib = None
+        monkeypatch.setattr(B, "source_hash", lambda: "0" * 64)
+        with pytest.raises(ImportError, match="not built from the sources"):
+            N.lib()
+    finally:
+        N._lib = saved
