@@ -232,6 +232,21 @@ QVMHIP_API int qvmhip_sample(qvmhip_state *s, const double *u, uint64_t nshots, 
 QVMHIP_API int qvmhip_sample_ex(qvmhip_state *s, const double *u, uint64_t nshots, int rule, double eps,
                                 uint64_t *out, uint8_t *near_flags, uint64_t *n_near);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops that forbid hipMalloc):
+ * qvmhip_sample_ex on a statevector the caller owns, with every pointer on
+ * the current device -- amps (2^n interleaved complex, FP64 or FP32 per
+ * dtype), the nshots draws u, the nshots indices out and the near-boundary
+ * count n_near (optional; cleared, then counted) -- enqueued on `stream` (a
+ * hipStream_t; null = the default stream).  The chunk sums and prefix live in
+ * a static device array of 4 MiB + 2 KiB that each GPU holds until the
+ * process ends, so n is at most 26 (larger n: QVMHIP_EINVAL).  It allocates
+ * nothing and does not synchronise.  Calls that overlap on one GPU share that
+ * array and race.  Draws are not range-checked (below 0 selects 0, above 1
+ * len-1). */
+QVMHIP_API int qvmhip_sample_noalloc(const void *amps, uint32_t n, int dtype, const double *u, uint64_t nshots,
+                                     int rule, double eps, uint64_t *out, uint64_t *n_near, void *stream);
+
 /* --- amplitudes (amplitudes / state-elements / copy-wavefunction) -------- */
 /* amplitudes, nth-amplitude, state-elements: src/qvm.lisp:63-82,
  * src/state-representation.lisp:138-167.  The only D2H point of the path. */

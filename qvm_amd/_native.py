@@ -109,6 +109,8 @@ SIGNATURES = {
     "qvmhip_sample_measure_all": [_vp, _dp, _u64, _u64p],
     "qvmhip_sample": [_vp, _dp, _u64, _u64p],
     "qvmhip_sample_ex": [_vp, _dp, _u64, C.c_int, C.c_double, _u64p, C.POINTER(C.c_uint8), _u64p],
+    "qvmhip_sample_noalloc": [C.c_void_p, _u32, C.c_int, C.c_void_p, _u64, C.c_int, C.c_double, C.c_void_p,
+                              C.c_void_p, C.c_void_p],
     "qvmhip_read_amplitudes": [_vp, _u64, _u64, _dp],
     "qvmhip_accumulate": [_vp, _vp],
     "qvmhip_density_diagonal": [_vp, _u32, _dp],

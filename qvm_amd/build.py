@@ -31,6 +31,7 @@ SOURCES = {
     "program.hip": [],
     "mlp.hip": [],
     "batch.hip": [],
+    "noalloc.hip": [],
 }
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
