@@ -433,6 +433,7 @@ def evaluate_planners(workload, learned: "CostModel", analytic: "AnalyticModel |
         ref = HipPureState(n, dtype, fma_tiles=True)
         st = HipPureState(n, dtype, fma_tiles=True)
         P.execute_plan(ref, cands["plain"])
+        ref.sync()  # the reference's launches must not overlap the timed plans
         measured, diffs = {}, {}
         keys = list(cands) if measure_all else sorted(set(pick.values()))
         for key in keys:
