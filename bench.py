@@ -74,8 +74,9 @@ def main() -> int:
                     help="circuits: the headline line (default); precision: FP32 vs FP64 report; "
                          "batch: 512 QAOA instances in one graph")
     ap.add_argument("--instances", type=int, default=512)
-    ap.add_argument("--learn-samples", type=int, default=60, help="timed launches per (n, dtype) for the cost model")
-    ap.add_argument("--save-model", default="", help="write the trained cost model (params + records) here")
+    ap.add_argument("--learn-samples", type=int, default=120, help="timed launches per (n, dtype) for the cost model")
+    ap.add_argument("--save-model", default="",
+                    help="--workload learned: retrain the cost model first and write it (params + records) here")
     ap.add_argument("--big-n", type=int, default=34, help="FP32-only size of the precision report")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -651,79 +652,30 @@ def batch(args) -> int:
 
 
 def learned(args) -> int:
-    """Config 5: the MLP cost model, trained on launch timings the engine
-    records, picks a plan (fusion width k = 2..6 on MFMA, tile width, or
-    plain) for depth-60 random circuits (60 % Haar U3 / 40 % CNOT-or-CZ); the
-    choice is measured against fixed-width fusion (k = 4) and the plain path,
-    and its state checked against the plain (unfused) result."""
-    import numpy as np
-
+    """Config 5 in full: optionally retrain the cost model on launch timings
+    this GPU records (--learn-samples launches per (n, dtype), n = 16..28,
+    written to --save-model), then time EVERY candidate plan of the
+    bench-scale mixed workload and report what the learned model, the
+    analytic model and the fixed rule pick against the best candidate, with
+    each picked state against the plain state."""
     from qvm_amd import circuits as C
     from qvm_amd import costmodel as CM
-    from qvm_amd import planner as P
-    from qvm_amd.state import HipPureState
 
-    t0 = time.perf_counter()
-    recs = CM.collect([26, 28], ["f64", "f32"], per_config=args.learn_samples, seed=11)
-    t_collect = time.perf_counter() - t0
-    rng = np.random.default_rng(0)
-    idx = rng.permutation(len(recs))
-    ntest = max(1, len(recs) // 5)
-    test = [recs[i] for i in idx[:ntest]]
-    train = [recs[i] for i in idx[ntest:]]
-    t0 = time.perf_counter()
-    cm = CM.CostModel.trained(train, epochs=3000)
-    t_train = time.perf_counter() - t0
-    Xt, Tt = CM.dataset(test)
-    rel = np.exp(np.abs(cm.mlp.predict(Xt) - Tt)) - 1
+    out = {"metric": "learned plan choice vs fixed rule and analytic model, mixed bench-scale workload"}
+    cm = CM.CostModel.shared()
     if args.save_model:
+        t0 = time.perf_counter()
+        recs = CM.collect([16, 20, 24, 28], ["f64", "f32"], per_config=args.learn_samples, seed=21)
+        cm = CM.CostModel.trained(recs, epochs=4000)
         cm.save(args.save_model, recs)
-    out = {"metric": "learned fusion planner vs fixed-width fusion and plain path, depth-60 random circuits",
-           "cost_model": {"launches_recorded": len(recs), "collect_s": t_collect, "train_s": t_train,
-                          "final_train_loss": cm.final_loss, "heldout_median_rel_err": float(np.median(rel)),
-                          "heldout_p90_rel_err": float(np.percentile(rel, 90))},
-           "circuits": []}
-    log(f"cost model: {len(recs)} launches, held-out median relative error {np.median(rel):.2f}")
-
-    def timed(state, plan):
-        prog = P.build_program(state, plan, use_graph=True)
-        state.reset_zero()
-        prog.run()
-        state.sync()
-        state.reset_zero()
-        state.sync()
-        t = time.perf_counter()
-        prog.run()
-        state.sync()
-        dt = time.perf_counter() - t
-        prog.close()
-        return dt
-
-    cases = [(28, "f64"), (28, "f32"), (30, "f64"), (30, "f32"), (32, "f32")]
-    for j, (n, dtype) in enumerate(cases):
-        gates = C.random_mix(n, 60, seed=500 + j)
-        upd = C.amp_updates(gates, n)
-        cands = CM.candidate_plans(gates, n, dtype)
-        name, plan, pred = cm.choose(gates, n, dtype, cands)
-        a = HipPureState(n, dtype, fma_tiles=True)
-        ref = HipPureState(n, dtype, fma_tiles=True)
-        t_learned = timed(a, plan)
-        t_k4 = timed(ref, cands["fused_k4"])  # fixed-width fusion
-        t_plain = timed(ref, cands["plain"])  # ref now holds the plain result
-        nrm = a.norm()
-        rec = {"n": n, "dtype": dtype, "gates": len(gates), "amp_updates": upd, "learned_choice": name,
-               "sweeps": {k: len(v) for k, v in cands.items()},
-               "predicted_ms": {k: round(v, 3) for k, v in pred.items()},
-               "measured_s": {"learned": t_learned, "fixed_k4": t_k4, "plain": t_plain},
-               "speedup_vs_fixed_k4": t_k4 / t_learned, "speedup_vs_plain": t_plain / t_learned,
-               "amp_updates_per_s_learned": upd / t_learned,
-               "max_abs_diff_vs_plain": a.max_abs_diff(ref),
-               "infidelity_vs_plain": max(0.0, 1 - abs(a.inner_product(ref) / (nrm * ref.norm())) ** 2)}
-        log(f"learned n={n} {dtype}: picks {name}: {t_learned:.3f} s vs k4 {t_k4:.3f} s vs plain {t_plain:.3f} s")
-        out["circuits"].append(rec)
-        a.close()
-        ref.close()
-    cm.mlp.close()
+        out["retrained"] = {"launches": len(recs), "s": time.perf_counter() - t0, "path": args.save_model}
+    rep = CM.evaluate_planners(C.mixed_workload("bench"), cm, CM.AnalyticModel(), measure_all=True, log=log)
+    tot = rep["totals_ms"]
+    rep["speedup_vs_fixed_rule"] = tot["fixed_rule"] / tot["learned"]
+    rep["speedup_vs_analytic"] = tot["analytic"] / tot["learned"]
+    rep["share_of_best_gain"] = ((tot["fixed_rule"] - tot["learned"]) / (tot["fixed_rule"] - tot["best_candidate"])
+                                 if tot["fixed_rule"] > tot["best_candidate"] else None)
+    out.update(rep)
     print(json.dumps(out))
     return 0
 
