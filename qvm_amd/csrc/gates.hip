@@ -79,7 +79,8 @@ __global__ __launch_bounds__(kBlock) void k_dense(vec2_t<T> *__restrict__ psi,
     constexpr int D = 1 << KT;
     using V = vec2_t<T>;
     const uint64_t step = (uint64_t)gridDim.x * (kBlock * GPT);
-    for (uint64_t first = (uint64_t)blockIdx.x * (kBlock * GPT) + threadIdx.x; first < a.g.ngroups;
+    for (uint64_t first = (uint64_t)xcd_block(blockIdx.x, gridDim.x) * (kBlock * GPT) + threadIdx.x;
+         first < a.g.ngroups;
          first += step) {
         V x[GPT][D];
         uint64_t base[GPT];
@@ -150,7 +151,7 @@ __global__ __launch_bounds__(kBlock) void k_dense1_lane(vec2_t<T> *__restrict__ 
                                                         const Dense1Coef<T> u) {
     using V = vec2_t<T>;
     const uint64_t step = (uint64_t)gridDim.x * kBlock;
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < len; i += step) {
+    for (uint64_t i = (uint64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; i < len; i += step) {
         const V x = ldg<true>(&psi[i]);
         V p;
         p.x = __shfl_xor(x.x, 1 << q, 64);
@@ -170,7 +171,7 @@ __global__ __launch_bounds__(kBlock) void k_dense1_xchg(vec2_t<T> *__restrict__ 
     __shared__ V sm[kBlock];
     const uint32_t t = threadIdx.x, h = (t >> 6) & 1u;
     const uint64_t nchunk = npairs / 128;
-    for (uint64_t c = blockIdx.x; c < nchunk; c += gridDim.x) {
+    for (uint64_t c = xcd_block(blockIdx.x, gridDim.x); c < nchunk; c += gridDim.x) {
         const uint64_t pair = c * 128 + ((t >> 7) << 6) + (t & 63u);
         const uint64_t a = insert_zero(pair, q) | ((uint64_t)h << q);
         const V x = ldg<true>(&psi[a]);
@@ -202,7 +203,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_big(vec2_t<T> *__restrict__ ps
     const int r = threadIdx.x % D;
     const int gl = threadIdx.x / D;
     const uint64_t nbatch = (g.ngroups + GPB - 1) / GPB;
-    for (uint64_t b = blockIdx.x; b < nbatch; b += gridDim.x) {
+    for (uint64_t b = xcd_block(blockIdx.x, gridDim.x); b < nbatch; b += gridDim.x) {
         const uint64_t grp = b * GPB + gl;
         const bool ok = grp < g.ngroups;
         uint64_t addr = 0;
@@ -267,7 +268,7 @@ __global__ __launch_bounds__(kBlock) void k_diag_full_f32x2(float2 *__restrict__
     f4 *p4 = (f4 *)psi;
     const uint64_t n2 = a.len / 2;
     const uint64_t step = (uint64_t)gridDim.x * kBlock;
-    for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < n2; k += step) {
+    for (uint64_t k = (uint64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; k < n2; k += step) {
         const f4 x = __builtin_nontemporal_load(&p4[k]);
         float m0r, m0i, m1r, m1i;
         diag_entry<float, KT>(a, 2 * k, m0r, m0i);
@@ -286,7 +287,7 @@ __global__ __launch_bounds__(kBlock) void k_diag_full(vec2_t<T> *__restrict__ ps
                                                       const DiagFullArgs<T> a) {
     using V = vec2_t<T>;
     const uint64_t step = (uint64_t)gridDim.x * kBlock;
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < a.len; i += step) {
+    for (uint64_t i = (uint64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; i < a.len; i += step) {
         const V x = ldg<NTM>(&psi[i]);
         T mr, mi;
         diag_entry<T, KT>(a, i, mr, mi);
@@ -306,7 +307,8 @@ __global__ __launch_bounds__(kBlock) void k_diag_sparse(vec2_t<T> *__restrict__ 
                                                         const SparseArgs<T> a) {
     using V = vec2_t<T>;
     const uint64_t step = (uint64_t)gridDim.x * (kBlock * GPT);
-    for (uint64_t first = (uint64_t)blockIdx.x * (kBlock * GPT) + threadIdx.x; first < a.g.ngroups;
+    for (uint64_t first = (uint64_t)xcd_block(blockIdx.x, gridDim.x) * (kBlock * GPT) + threadIdx.x;
+         first < a.g.ngroups;
          first += step) {
 #pragma unroll
         for (int g = 0; g < GPT; ++g) {
@@ -340,7 +342,8 @@ __global__ __launch_bounds__(kBlock) void k_perm(vec2_t<T> *__restrict__ psi,
                                                  const SparseArgs<T> a) {
     using V = vec2_t<T>;
     const uint64_t step = (uint64_t)gridDim.x * (kBlock * GPT);
-    for (uint64_t first = (uint64_t)blockIdx.x * (kBlock * GPT) + threadIdx.x; first < a.g.ngroups;
+    for (uint64_t first = (uint64_t)xcd_block(blockIdx.x, gridDim.x) * (kBlock * GPT) + threadIdx.x;
+         first < a.g.ngroups;
          first += step) {
 #pragma unroll
         for (int g = 0; g < GPT; ++g) {
@@ -383,7 +386,7 @@ __global__ __launch_bounds__(kBlock) void k_perm_lane(vec2_t<T> *__restrict__ ps
     using V = vec2_t<T>;
     const uint32_t flip = o0 ^ o1;
     const uint64_t step = (uint64_t)gridDim.x * kBlock;
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < len; i += step) {
+    for (uint64_t i = (uint64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; i < len; i += step) {
         const V x = ldg<true>(&psi[i]);
         V p;
         p.x = __shfl_xor(x.x, (int)flip, 64);

@@ -25,6 +25,19 @@ constexpr int kChunkLog2 = 8;       // sampling chunk: 256 amplitudes
 // 2^22 workgroups and grid-stride over the rest (34-qubit states).
 constexpr uint64_t kMaxGrid = uint64_t(1) << 22;
 
+// Workgroups of a dispatch go to the 8 XCDs round robin: workgroup b runs on
+// XCD b % 8.  xcd_block(b, nb) renumbers the nb workgroups so that each XCD
+// takes one contiguous share of [0, nb), the x-th share for XCD x: each XCD
+// then streams through its own part of the state (its own pages for its
+// TLB) instead of every eighth 4 KiB piece of all of it.  Measured on a
+// 16 GiB in-place sweep, one amplitude per thread: 6.74 vs 6.60 TB/s.  A
+// bijection for every nb (the shares differ by at most one workgroup), so
+// it changes which workgroup does what, never the result.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 template <typename T> struct Vec2;
 template <> struct Vec2<double> { using type = double2; };
 template <> struct Vec2<float> { using type = float2; };

@@ -730,7 +730,7 @@ __global__ __launch_bounds__(1 << (M - RB)) void k_tile(vec2_t<T> *__restrict__ 
     const uint64_t toff = thread_offset<M - RB>(a, threadIdx.x);
     // One tile per workgroup (grid-striding only past the dispatch limit):
     // many resident workgroups overlap each other's loads, ops and stores.
-    for (uint64_t tile = blockIdx.x; tile < a.g.ngroups; tile += gridDim.x) {
+    for (uint64_t tile = xcd_block(blockIdx.x, gridDim.x); tile < a.g.ngroups; tile += gridDim.x) {
         V x[1 << RB];
         const uint64_t gb = group_base(a.g, tile);
         tile_load<T, M, RB>(psi, a, gb, toff, x);

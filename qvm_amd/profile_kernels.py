@@ -6,6 +6,12 @@ device-to-device copy of the whole state (a transfer of known size:
 it.  Usage:
 
     python -m qvm_amd.profile_kernels --n 30 --kernel dense1 --reps 8
+
+`--kernel tile_ops` times one tile pass of `--count` identical `--op` gates
+on tile-local qubit `--local` (by default the top tile qubit, a register bit
+after the scheduler's remap; low tile qubits are lane bits), `--reps` times,
+and prints the mean time per pass: the difference between two counts is the
+cost of one op beyond the pass's load and store.
 """
 from __future__ import annotations
 
@@ -24,13 +30,18 @@ def main(argv=None) -> int:
     ap.add_argument("--dtype", default="f64")
     ap.add_argument("--kernel", default="dense1",
                     choices=["dense1", "diag_full", "perm", "diag_sparse", "fused3", "fused4", "fused5", "fused6",
-                             "tile"])
+                             "tile", "tile_ops"])
     ap.add_argument("--reps", type=int, default=8)
     ap.add_argument("--fma", action="store_true", help="tile passes with FMA-contracted products")
     ap.add_argument("--tile-m", type=int, default=0, help="tile width (default: the planner's)")
     ap.add_argument("--qubits", default="",
                     help="comma-separated target qubits, cycled (fused blocks: the first k; default: random)")
+    ap.add_argument("--op", default="RX", choices=["RX", "U3", "RZ"], help="tile_ops: the gate")
+    ap.add_argument("--count", type=int, default=8, help="tile_ops: gates in the pass")
+    ap.add_argument("--local", type=int, default=-1, help="tile_ops: tile-local target (default: m - 1)")
     a = ap.parse_args(argv)
+    if a.kernel == "tile_ops":
+        return tile_ops(a)
     fixed = [int(x) for x in a.qubits.split(",") if x.strip()]
     st = HipPureState(a.n, a.dtype, fma_tiles=a.fma)
     rng = np.random.default_rng(0)
@@ -64,6 +75,27 @@ def main(argv=None) -> int:
     print(f"profiled {a.reps} x {a.kernel} at n={a.n} {a.dtype}; copy of {(1 << a.n) * (16 if a.dtype == 'f64' else 8)} B",
           flush=True)
     other.close()
+    st.close()
+    return 0
+
+
+def tile_ops(a) -> int:
+    from . import planner as P
+    st = HipPureState(a.n, a.dtype, fma_tiles=a.fma)
+    m = a.tile_m or P.default_tile_m(a.dtype)
+    q = a.local if a.local >= 0 else m - 1
+    U = {"RX": G.gate_matrix("RX", [0.3]), "U3": G.gate_matrix("U3", [0.3, 0.7, 1.1]),
+         "RZ": G.gate_matrix("RZ", [0.3])}[a.op]
+    tq = list(range(m))
+    local = {t: t for t in tq}
+    ops = [P.tile_op(U, [q], local) for _ in range(a.count)]
+    prog = P.build_program(st, [P.PlanItem("tile", tq, ops=ops)], use_graph=False)
+    prog.run()
+    st.sync()
+    ms = prog.time(a.reps)
+    print(f"tile pass of {a.count} x {a.op} on tile qubit {q}: m={m} n={a.n} {a.dtype} "
+          f"{'fma' if a.fma else 'exact'}: {ms:.4f} ms per pass", flush=True)
+    prog.close()
     st.close()
     return 0
 
