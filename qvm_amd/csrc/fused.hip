@@ -17,8 +17,9 @@
 // and 16 consecutive complement groups x those rows are contiguous runs.  The
 // A fragments (U_re, U_im per 16x4 block and lane) are built once per
 // workgroup in LDS from U in nat-tuple order.  Each wave takes 16 complement
-// groups at a time, grid-striding, with the next group's loads issued before
-// the current group's MFMAs.  The four real products per complex product are
+// groups at a time, striding through the contiguous share of column groups
+// that its XCD owns, with the next group's loads issued before the current
+// group's MFMAs.  The four real products per complex product are
 // all kept, so rounding is that of a complex matvec with FMA.
 #include <algorithm>
 #include <cstring>
@@ -90,7 +91,10 @@ __global__ __launch_bounds__(256) void k_fused(vec2_t<T> *__restrict__ psi, cons
     const int j = lane & 15, kk = lane >> 4;
     const uint64_t ncols = g.ngroups;
     const uint64_t ncg = (ncols + 15) / 16;
-    const uint64_t wstride = (uint64_t)gridDim.x * 4;
+    // each XCD takes a contiguous share of the column groups; the waves of
+    // its workgroups stride through that share
+    const XcdShare sh = xcd_share(blockIdx.x, gridDim.x, ncg);
+    const uint64_t wstride = (uint64_t)sh.count * 4;
     // lane parts of the row offsets: loads take row 4*ks + kk; the result
     // rows of register e are kk + 4e (f64) or 4*kk + e (f32)
     const uint64_t lo_ld = off.o[kk];
@@ -116,14 +120,14 @@ __global__ __launch_bounds__(256) void k_fused(vec2_t<T> *__restrict__ psi, cons
     // MFMAs (register double buffer); k = 6 has no registers to spare for it
     // and relies on the other waves of the SIMD instead.
     constexpr bool PF = K <= 5;
-    uint64_t cg = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    uint64_t cg = sh.lo + (uint64_t)sh.rank * 4 + (threadIdx.x >> 6);
     V xn[PF ? KS : 1];
     uint64_t bn = 0;
     bool okn = false;
     if constexpr (PF) {
-        if (cg < ncg) load(cg, xn, bn, okn);
+        if (cg < sh.hi) load(cg, xn, bn, okn);
     }
-    for (; cg < ncg; cg += wstride) {
+    for (; cg < sh.hi; cg += wstride) {
         V xv[KS];
         uint64_t base;
         bool ok;
@@ -132,7 +136,7 @@ __global__ __launch_bounds__(256) void k_fused(vec2_t<T> *__restrict__ psi, cons
             for (int ks = 0; ks < KS; ++ks) xv[ks] = xn[ks];
             base = bn;
             ok = okn;
-            if (cg + wstride < ncg) load(cg + wstride, xn, bn, okn);  // in flight during the MFMAs
+            if (cg + wstride < sh.hi) load(cg + wstride, xn, bn, okn);  // in flight during the MFMAs
         } else {
             load(cg, xv, base, ok);
         }

@@ -38,6 +38,32 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
     return x * q + (x < r ? x : r) + (b >> 3);
 }
 
+// The persistent-grid form: items [0, n) are cut into 8 contiguous shares,
+// share x for the workgroups on XCD x; a workgroup takes items rank, rank +
+// count, ... of its share (rank among the count workgroups of its XCD).  With
+// fewer than 8 workgroups every workgroup strides over all items.
+struct XcdShare {
+    uint64_t lo, hi;
+    uint32_t rank, count;
+};
+__device__ __forceinline__ XcdShare xcd_share(uint32_t b, uint32_t nb, uint64_t n) {
+    XcdShare s;
+    if (nb < 8) {
+        s.lo = 0;
+        s.hi = n;
+        s.rank = b;
+        s.count = nb;
+        return s;
+    }
+    const uint32_t x = b & 7u, r = nb & 7u;
+    s.rank = b >> 3;
+    s.count = (nb >> 3) + (x < r ? 1u : 0u);
+    const uint64_t per = n >> 3, rem = n & 7u;
+    s.lo = x * per + (x < rem ? x : rem);
+    s.hi = s.lo + per + (x < rem ? 1u : 0u);
+    return s;
+}
+
 template <typename T> struct Vec2;
 template <> struct Vec2<double> { using type = double2; };
 template <> struct Vec2<float> { using type = float2; };

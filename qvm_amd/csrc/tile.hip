@@ -769,6 +769,9 @@ __global__ __launch_bounds__(1 << (M - RB)) void k_tile_batch(vec2_t<T> *__restr
                                                              const char *__restrict__ op_buf) {
     using V = vec2_t<T>;
     __shared__ V sa[(1 << M) + (1 << RB)];
+    // round robin over the XCDs (no xcd_block): work blocks of different
+    // entries differ in cost, and contiguous shares per XCD measured 5-6 %
+    // slower on the 512-instance QAOA batch
     for (uint64_t b = blockIdx.x; b < total_blocks; b += gridDim.x) {
         const TileEntry &e = ent[blk_ent[b]];
         const uint64_t toff = thread_offset<M - RB>(e.args, threadIdx.x);
