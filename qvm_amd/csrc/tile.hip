@@ -549,15 +549,15 @@ __device__ __forceinline__ uint32_t lds_addr(uint32_t s) {
 }
 
 // src4: the op's first 16 operand bytes as four words; byte j is the old
-// slot bit of new slot bit j.
+// slot bit of new slot bit j.  FP64 blocks go through LDS one component at a
+// time (real parts, then imaginary parts): the exchange buffer is then 2^M
+// doubles, half a block, and five workgroups fit on a CU instead of four
+// (measured at n = 30: 8.23 -> 7.84 ms per exact pass, 7.50 -> 7.13 FMA).
+// FP32 blocks move whole amplitudes (split, they lost 2.5-6 %).
 template <typename T, int R, int TB>
-__device__ __forceinline__ void reg_remap(vec2_t<T> (&x)[R], vec2_t<T> *sa, const uint32_t (&src4)[4], uint32_t t) {
+__device__ __forceinline__ void reg_remap(vec2_t<T> (&x)[R], T *sa, const uint32_t (&src4)[4], uint32_t t) {
     constexpr uint32_t NT = 1u << TB;
     auto src = [&](int j) -> uint32_t { return (src4[j >> 2] >> (8 * (j & 3))) & 0xFFu; };
-    __syncthreads();  // readers of the previous layout are done with sa
-#pragma unroll
-    for (int r = 0; r < R; ++r) sa[t + (uint32_t)r * (NT + 1)] = x[r];
-    __syncthreads();
     uint32_t base = 0;
 #pragma unroll
     for (int j = 0; j < TB; ++j)
@@ -565,13 +565,38 @@ __device__ __forceinline__ void reg_remap(vec2_t<T> (&x)[R], vec2_t<T> *sa, cons
     uint32_t pw[8];
 #pragma unroll
     for (int b = 0; b < 8; ++b) pw[b] = (1 << b) < R ? (1u << src(TB + b)) : 0u;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
+    auto addr = [&](int r) -> uint32_t {
         uint32_t off = 0;
 #pragma unroll
         for (int b = 0; (1 << b) < R; ++b)
             if ((r >> b) & 1) off |= pw[b];
-        x[r] = sa[lds_addr<TB>(base | off)];
+        return lds_addr<TB>(base | off);
+    };
+    if constexpr (sizeof(T) != 8) {
+        vec2_t<T> *sv = reinterpret_cast<vec2_t<T> *>(sa);
+        __syncthreads();  // readers of the previous layout are done with sa
+#pragma unroll
+        for (int r = 0; r < R; ++r) sv[t + (uint32_t)r * (NT + 1)] = x[r];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r) x[r] = sv[addr(r)];
+        return;
+    }
+    __syncthreads();  // readers of the previous layout are done with sa
+#pragma unroll
+    for (int r = 0; r < R; ++r) sa[t + (uint32_t)r * (NT + 1)] = x[r].x;
+    __syncthreads();
+    T re[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) re[r] = sa[addr(r)];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r) sa[t + (uint32_t)r * (NT + 1)] = x[r].y;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        x[r].x = re[r];
+        x[r].y = sa[addr(r)];
     }
 }
 
@@ -602,7 +627,7 @@ __device__ __forceinline__ void tile_store(vec2_t<T> *__restrict__ psi, const Ti
 // other op down.
 template <typename T, int M, int RB, bool FMA, bool CUT>
 __device__ inline void tile_ops(vec2_t<T> (&x)[1 << RB], const qvmhip_tile_op *__restrict__ ops, uint32_t nops,
-                                vec2_t<T> *sa, uint64_t gb) {
+                                T *sa, uint64_t gb) {
     constexpr int R = 1 << RB, TB = M - RB;
     const uint32_t t = threadIdx.x;
     for (uint32_t oi = 0; oi < nops; ++oi) {
@@ -707,7 +732,7 @@ __device__ inline void tile_ops(vec2_t<T> (&x)[1 << RB], const qvmhip_tile_op *_
                       // deepens the dispatch of every op)
                 if constexpr (FMA && CUT) {
                     if (kind == kTileCutPhase) {
-                        reg_cut_phase<T, R, TB, FMA>(x, ops + oi, sa, t, gb);
+                        reg_cut_phase<T, R, TB, FMA>(x, ops + oi, reinterpret_cast<vec2_t<T> *>(sa), t, gb);
                         oi += ((const uint8_t *)op.m)[1];  // its DATA ops
                         break;
                     }
@@ -726,7 +751,9 @@ template <typename T, int M, int RB, bool FMA, bool CUT>
 __global__ __launch_bounds__(1 << (M - RB)) void k_tile(vec2_t<T> *__restrict__ psi, const TileArgs a,
                                                        const qvmhip_tile_op *__restrict__ ops, uint32_t nops) {
     using V = vec2_t<T>;
-    __shared__ V sa[(1 << M) + (1 << RB)];
+    // the remap buffer: 2^M + 2^RB reals for FP64 (one component at a time),
+    // as many amplitudes for FP32
+    __shared__ __align__(16) T sa[(sizeof(T) == 8 ? 1 : 2) * ((1 << M) + (1 << RB))];
     const uint64_t toff = thread_offset<M - RB>(a, threadIdx.x);
     // One tile per workgroup (grid-striding only past the dispatch limit):
     // many resident workgroups overlap each other's loads, ops and stores.
@@ -768,7 +795,9 @@ __global__ __launch_bounds__(1 << (M - RB)) void k_tile_batch(vec2_t<T> *__restr
                                                              uint64_t total_blocks, uint32_t tiles_per_block,
                                                              const char *__restrict__ op_buf) {
     using V = vec2_t<T>;
-    __shared__ V sa[(1 << M) + (1 << RB)];
+    // the remap buffer: 2^M + 2^RB reals for FP64 (one component at a time),
+    // as many amplitudes for FP32
+    __shared__ __align__(16) T sa[(sizeof(T) == 8 ? 1 : 2) * ((1 << M) + (1 << RB))];
     // round robin over the XCDs (no xcd_block): work blocks of different
     // entries differ in cost, and contiguous shares per XCD measured 5-6 %
     // slower on the 512-instance QAOA batch
