@@ -54,23 +54,19 @@ struct FusedOff {
     uint32_t nat[6]; // position in the nat-tuple of the b-th lowest target
 };
 
-template <typename T, int K, bool NTM>
-__global__ __launch_bounds__(256) void k_fused(vec2_t<T> *__restrict__ psi, const double *__restrict__ U,
-                                               GroupSpec g, FusedOff off) {
-    using V = vec2_t<T>;
-    using v4 = typename Mfma<T>::v4;
+// A fragments in LDS: fr[(mt*KS + ks)*64 + lane] = U_s[mt*16 + i][ks*4 + k],
+// lane = i + 16k, U_s = U with rows and columns in sorted-target order
+// (rows past 2^K, for K = 3, are zero).  Ends with a barrier.
+template <typename T, int K>
+__device__ __forceinline__ void build_fragments(vec2_t<T> *fr, const double *__restrict__ U, const FusedOff &off) {
     constexpr int D = 1 << K;
-    constexpr int MT = D < 16 ? 1 : D / 16;  // 16-row output blocks
-    constexpr int KS = D / 4;                // 4-row k-steps
-    __shared__ V fr[MT * KS * 64];
-
-    // A fragments: fr[(mt*KS + ks)*64 + lane] = U_s[mt*16 + i][ks*4 + k],
-    // lane = i + 16k, U_s = U with rows and columns in sorted-target order.
+    constexpr int MT = D < 16 ? 1 : D / 16;
+    constexpr int KS = D / 4;
     for (int e = threadIdx.x; e < MT * KS * 64; e += 256) {
         const int ln = e & 63, blk = e >> 6;
         const int mt = blk / KS, ks = blk % KS;
         const int rs = mt * 16 + (ln & 15), cs = ks * 4 + (ln >> 4);
-        V v;
+        vec2_t<T> v;
         v.x = T(0);
         v.y = T(0);
         if (rs < D) {
@@ -86,6 +82,18 @@ __global__ __launch_bounds__(256) void k_fused(vec2_t<T> *__restrict__ psi, cons
         fr[e] = v;
     }
     __syncthreads();
+}
+
+template <typename T, int K, bool NTM>
+__global__ __launch_bounds__(256) void k_fused(vec2_t<T> *__restrict__ psi, const double *__restrict__ U,
+                                               GroupSpec g, FusedOff off) {
+    using V = vec2_t<T>;
+    using v4 = typename Mfma<T>::v4;
+    constexpr int D = 1 << K;
+    constexpr int MT = D < 16 ? 1 : D / 16;  // 16-row output blocks
+    constexpr int KS = D / 4;                // 4-row k-steps
+    __shared__ V fr[MT * KS * 64];
+    build_fragments<T, K>(fr, U, off);
 
     const int lane = threadIdx.x & 63;
     const int j = lane & 15, kk = lane >> 4;
@@ -191,6 +199,135 @@ __global__ __launch_bounds__(256) void k_fused(vec2_t<T> *__restrict__ psi, cons
     }
 }
 
+// 4x4 transpose of 16-lane blocks over four registers: afterwards block g of
+// register k holds what block k of register g held.  Two rounds of the gfx950
+// lane swaps (permlane32_swap exchanges the upper 32 lanes of its first
+// operand with the lower 32 of its second, permlane16_swap the odd 16-lane
+// blocks of the first with the even blocks of the second), one word at a
+// time.  An involution.
+__device__ __forceinline__ void xpose4_words(uint32_t &a0, uint32_t &a1, uint32_t &a2, uint32_t &a3) {
+    const auto p = __builtin_amdgcn_permlane32_swap(a0, a2, false, false);
+    const auto q = __builtin_amdgcn_permlane32_swap(a1, a3, false, false);
+    const auto r = __builtin_amdgcn_permlane16_swap(p[0], q[0], false, false);
+    const auto t = __builtin_amdgcn_permlane16_swap(p[1], q[1], false, false);
+    a0 = r[0];
+    a1 = r[1];
+    a2 = t[0];
+    a3 = t[1];
+}
+
+template <typename X>
+__device__ __forceinline__ void xpose4(X &a0, X &a1, X &a2, X &a3) {
+    static_assert(sizeof(X) % 4 == 0, "whole words");
+    constexpr int W = sizeof(X) / 4;
+    uint32_t w0[W], w1[W], w2[W], w3[W];
+    __builtin_memcpy(w0, &a0, sizeof(X));
+    __builtin_memcpy(w1, &a1, sizeof(X));
+    __builtin_memcpy(w2, &a2, sizeof(X));
+    __builtin_memcpy(w3, &a3, sizeof(X));
+#pragma unroll
+    for (int i = 0; i < W; ++i) xpose4_words(w0[i], w1[i], w2[i], w3[i]);
+    __builtin_memcpy(&a0, w0, sizeof(X));
+    __builtin_memcpy(&a1, w1, sizeof(X));
+    __builtin_memcpy(&a2, w2, sizeof(X));
+    __builtin_memcpy(&a3, w3, sizeof(X));
+}
+
+// The same block product with one whole row per memory instruction: a wave
+// takes 64 consecutive complement groups (lane l holds column 64 cb + l) and
+// each load or store moves one row of them, a 1 KiB run (FP64) when the
+// columns are contiguous, where the MFMA operand layout above moves 4 rows x
+// 256 B.  The four rows of a k-step are turned into the B fragments of the
+// four 16-column groups by one 4x4 block transpose across lanes (xpose4),
+// and the results back into rows by the same transpose, so the products, and
+// their rounding, are those of k_fused.  Needs 2^(n-K) >= 64 columns.  Used
+// for K = 4, and K = 5 in FP32 (the four column groups' accumulators of FP64
+// K = 5 and of K = 6 leave room for one wave per SIMD).
+template <typename T, int K, bool NTM>
+__global__ __launch_bounds__(256) void k_fused_rows(vec2_t<T> *__restrict__ psi, const double *__restrict__ U,
+                                                    GroupSpec g, FusedOff off) {
+    using V = vec2_t<T>;
+    using v4 = typename Mfma<T>::v4;
+    constexpr int D = 1 << K;
+    constexpr int MT = D < 16 ? 1 : D / 16;
+    constexpr int KS = D / 4;
+    __shared__ V fr[MT * KS * 64];
+    build_fragments<T, K>(fr, U, off);
+
+    const int lane = threadIdx.x & 63;
+    const uint64_t ncb = g.ngroups / 64;
+    const XcdShare sh = xcd_share(blockIdx.x, gridDim.x, ncb);
+    const uint64_t wstride = (uint64_t)sh.count * 4;
+    for (uint64_t cb = sh.lo + (uint64_t)sh.rank * 4 + (threadIdx.x >> 6); cb < sh.hi; cb += wstride) {
+        const uint64_t base = group_base(g, cb * 64 + (uint64_t)lane);
+        v4 ar[4][MT], ai[4][MT];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                ar[c][mt] = v4{0, 0, 0, 0};
+                ai[c][mt] = v4{0, 0, 0, 0};
+            }
+        // the next k-step's four rows are in flight during this one's MFMAs
+        V xn[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) xn[k] = ldg<NTM>(&psi[base + off.o[k]]);
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            V b[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) b[k] = xn[k];
+            if (ks + 1 < KS) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) xn[k] = ldg<NTM>(&psi[base + off.o[4 * (ks + 1) + k]]);
+            }
+            // rows 4ks..4ks+3 of 64 columns -> B fragments of column groups 0..3
+            xpose4(b[0], b[1], b[2], b[3]);
+            V a[MT];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) a[mt] = fr[(mt * KS + ks) * 64 + lane];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const T br = b[c].x, bi = b[c].y, nbi = -b[c].y;
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) {
+                    ar[c][mt] = Mfma<T>::mma(a[mt].x, br, ar[c][mt]);
+                    ai[c][mt] = Mfma<T>::mma(a[mt].y, br, ai[c][mt]);
+                    ar[c][mt] = Mfma<T>::mma(a[mt].y, nbi, ar[c][mt]);
+                    ai[c][mt] = Mfma<T>::mma(a[mt].x, bi, ai[c][mt]);
+                }
+            }
+        }
+        // result register e of column group c, lane block b, holds row
+        // 16mt + b + 4e (f64) or 16mt + 4b + e (f32); transposed across c and
+        // b, register b then holds that row for all 64 columns
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                T yr[4], yi[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    yr[c] = ar[c][mt][e];
+                    yi[c] = ai[c][mt][e];
+                }
+                xpose4(yr[0], yr[1], yr[2], yr[3]);
+                xpose4(yi[0], yi[1], yi[2], yi[3]);
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const int row = 16 * mt + (sizeof(T) == 8 ? b + 4 * e : 4 * b + e);
+                    if (row < D) {
+                        V y;
+                        y.x = yr[b];
+                        y.y = yi[b];
+                        stg<NTM>(&psi[base + off.o[row]], y);
+                    }
+                }
+            }
+        }
+    }
+}
+
 namespace {
 template <typename T, int K>
 int run_fused(qvmhip_state *s, const void *mat_dev, const uint32_t *qubits) {
@@ -238,6 +375,41 @@ int run_fused(qvmhip_state *s, const void *mat_dev, const uint32_t *qubits) {
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess || ncu <= 0)
         ncu = 256;
+    if constexpr (K == 4 || (K == 5 && sizeof(T) == 4)) {
+        // one row per memory instruction when every target lies above the
+        // 128 B line, so that each row's lines are read whole by one
+        // instruction.  Measured at n = 30 (ms per launch, this kernel vs
+        // k_fused, same box): FP64 K = 4 on ten target sets 5.80-6.95 vs
+        // 6.73-7.00, 5.3 % less in total, from -1 % (3,6,16,29) to +18 %
+        // (12,19,20,22); FP32 K = 4 / 5 on 5,11,17,23(,27) 3.46 / 3.66 vs
+        // 3.80 / 3.91.  With a target inside the line (0,5,11,17 FP64; 0,1,2,
+        // 9,20 FP32) it lost 11-27 %, and at K = 3 4 %.
+        if (g.ngroups >= 64 && pos[0] >= line_bits) {
+            static int per_cu_rows = 0;
+            if (per_cu_rows == 0) {
+                int nb = 0;
+                per_cu_rows = (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fused_rows<T, K, false>, 256, 0) ==
+                                   hipSuccess &&
+                               nb > 0)
+                                  ? nb
+                                  : 2;
+            }
+            const bool nt_rows = sizeof(T) == 8;
+            const uint64_t ncb = g.ngroups / 64;
+            const uint32_t grid_rows =
+                (uint32_t)std::min<uint64_t>((ncb + 3) / 4, (uint64_t)per_cu_rows * (uint64_t)ncu);
+            prof_begin(s, QVMHIP_K_FUSED);
+            if (nt_rows)
+                hipLaunchKernelGGL((k_fused_rows<T, K, true>), dim3(grid_rows), dim3(256), 0, s->stream,
+                                   (vec2_t<T> *)s->amps, (const double *)mat_dev, g, off);
+            else
+                hipLaunchKernelGGL((k_fused_rows<T, K, false>), dim3(grid_rows), dim3(256), 0, s->stream,
+                                   (vec2_t<T> *)s->amps, (const double *)mat_dev, g, off);
+            prof_end(s);
+            QH_LAUNCH_CHECK();
+            return QVMHIP_OK;
+        }
+    }
     const uint64_t ncg = (g.ngroups + 15) / 16;
     const uint32_t grid = (uint32_t)std::min<uint64_t>((ncg + 3) / 4, (uint64_t)per_cu * (uint64_t)ncu);
     prof_begin(s, QVMHIP_K_FUSED);

@@ -602,18 +602,23 @@ def test_tile_passes_fma(S, oracle, dtype, m):
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 @pytest.mark.parametrize("k", [3, 4, 5, 6])
 def test_fused_mfma_block(S, oracle, k, dtype):
+    """Both column layouts: targets 0..k-1 or n = k + 5 (32 complement
+    groups) take the MFMA operand layout (16 groups per wave); targets above
+    the line at n = 13 take one row of 64 groups per memory instruction for
+    k = 4 and FP32 k = 5."""
     rng = np.random.default_rng(200 + k)
-    n = 13
-    psi = rand_state(n, rng)
-    for qs in (list(range(k)), [int(x) for x in rng.choice(n, size=k, replace=False)]):
-        U = rand_unitary(k, rng)
-        say(f"fused k={k} dtype={dtype} qubits={qs}")
-        st = load(S, psi, dtype)
-        st.apply_fused(U, qs)
-        ref = oracle.apply_matrix(psi.copy(), U, qs)
-        err = np.max(np.abs(st.amplitudes() - ref))
-        assert err < (1e-14 if dtype == "f64" else 3e-6), err
-        st.close()
+    for n in (k + 5, 13):
+        psi = rand_state(n, rng)
+        high = [int(x) for x in 4 + rng.choice(n - 4, size=k, replace=False)]
+        for qs in (list(range(k)), [int(x) for x in rng.choice(n, size=k, replace=False)], high):
+            U = rand_unitary(k, rng)
+            say(f"fused k={k} dtype={dtype} n={n} qubits={qs}")
+            st = load(S, psi, dtype)
+            st.apply_fused(U, qs)
+            ref = oracle.apply_matrix(psi.copy(), U, qs)
+            err = np.max(np.abs(st.amplitudes() - ref))
+            assert err < (1e-14 if dtype == "f64" else 3e-6), err
+            st.close()
 
 
 def test_fused_plan_random_mix(S, oracle):
