@@ -327,20 +327,20 @@ def run_paths(args, n, ga, gb, nat_a, upd_a, upd_b, u, st, HipPureState, P, np):
         return r
 
     s2 = HipPureState(n, "f64")
-    m64 = P.default_tile_m("f64")
-    res["tiled_exact_f64"] = record("tiled_exact_f64", s2, P.plan_tiled(ga, n, m64, 4, reorder=False),
-                                    P.plan_tiled(gb, n, m64, 4, reorder=False))
+    m64, l64 = P.default_tile_m("f64"), P.default_low("f64")
+    res["tiled_exact_f64"] = record("tiled_exact_f64", s2, P.plan_tiled(ga, n, m64, l64, reorder=False),
+                                    P.plan_tiled(gb, n, m64, l64, reorder=False))
     res["tiled_exact_f64"]["bit_identical_to_plain"] = res["tiled_exact_f64"]["max_abs_diff_vs_plain_f64"] == 0.0
     res["fused_f64"] = record("fused_f64", s2, P.plan_fused(ga, n, 4), P.plan_fused(gb, n, 4))
     s2.close()
     s2 = HipPureState(n, "f64", fma_tiles=True)  # reordered passes, FMA-contracted
-    res["tiled_f64"] = record("tiled_f64", s2, P.plan_tiled(ga, n, m64, 4, merge_1q=True),
-                              P.plan_tiled(gb, n, m64, 4, merge_1q=True))
+    res["tiled_f64"] = record("tiled_f64", s2, P.plan_tiled(ga, n, m64, l64, merge_1q=True),
+                              P.plan_tiled(gb, n, m64, l64, merge_1q=True))
     s2.close()
     s3 = HipPureState(n, "f32", fma_tiles=True)
     m32 = P.default_tile_m("f32")
-    res["tiled_f32"] = record("tiled_f32", s3, P.plan_tiled(ga, n, m32, 5, merge_1q=True),
-                              P.plan_tiled(gb, n, m32, 5, merge_1q=True))
+    res["tiled_f32"] = record("tiled_f32", s3, P.plan_tiled(ga, n, m32, P.default_low("f32"), merge_1q=True),
+                              P.plan_tiled(gb, n, m32, P.default_low("f32"), merge_1q=True))
     res["fused_f32"] = record("fused_f32", s3, P.plan_fused(ga, n, 5), P.plan_fused(gb, n, 5))
     s3.close()
     ref_a.close()

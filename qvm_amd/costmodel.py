@@ -378,8 +378,9 @@ def fixed_rule_name(dtype: str) -> str:
 
 def candidate_plans(gates, n: int, dtype: str) -> Dict[str, List["P.PlanItem"]]:
     """plain; fused_k2..6; tiled_m{m}_l{low} for three tile widths and three
-    low runs (the qubit permutation into the block); hybrid_m{m}: the default
-    tile passes with the gates on more than two qubits as MFMA blocks."""
+    low runs (the qubit permutation into the block: the default run of one
+    128 B line and two longer ones); hybrid_m{m}: the default tile passes with
+    the gates on more than two qubits as MFMA blocks."""
     cands = {"plain": P.plan_plain(gates, n)}
     for k in (2, 3, 4, 5, 6):
         if k <= n:
@@ -389,7 +390,7 @@ def candidate_plans(gates, n: int, dtype: str) -> Dict[str, List["P.PlanItem"]]:
         for m in (m0 - 1, m0, m0 + 1):
             if m > min(n, P.max_tile_m(dtype)):
                 continue
-            for low in (l0 - 1, l0, l0 + 1):
+            for low in (l0, l0 + 1, l0 + 2):
                 cands[f"tiled_m{m}_l{low}"] = P.plan_tiled(gates, n, m, low, merge_1q=True)
         m = min(m0, n)
         cands[f"hybrid_m{m}"] = P.plan_tiled(gates, n, m, l0, merge_1q=True, wide="fused")
