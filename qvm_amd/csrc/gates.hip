@@ -397,6 +397,71 @@ __global__ __launch_bounds__(kBlock) void k_perm_lane(vec2_t<T> *__restrict__ ps
     }
 }
 
+// Gates whose lowest bit lies inside a 128 B line touch every line of the
+// lines they change but only some amplitudes of each, so the pair and entry
+// forms above read and write partial lines.  The line forms sweep whole
+// lines instead.  The gate's bits split into those inside the line (mask
+// lmask) and those above it; `hg` enumerates the amplitudes whose bits above
+// the line are zero (count = 2^(n - their number)), and ORing h0 (h1) selects
+// the lines of entry E0 (E1).  One amplitude per thread, consecutive threads
+// on consecutive amplitudes of a line; count is a multiple of kBlock, so every
+// lane of a wavefront takes part in each shuffle.
+//
+// Single-entry diagonal (CZ, CPHASE with a bit inside the line): every
+// amplitude of the lines with the entry's upper bits is read, the ones whose
+// in-line bits match (l0) are multiplied as in k_diag_sparse, and the lines
+// are written back whole.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_diag_line(vec2_t<T> *__restrict__ psi, const GroupSpec hg,
+                                                      uint64_t count, uint64_t h0, uint32_t lmask, uint32_t l0,
+                                                      T dr, T di) {
+    using V = vec2_t<T>;
+    const uint64_t step = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t j = (uint64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; j < count; j += step) {
+        const uint64_t i = group_base(hg, j) | h0;
+        const V x = ldg<true>(&psi[i]);
+        V y = x;
+        if (((uint32_t)i & lmask) == l0) {
+            y.x = x.x * dr - x.y * di;
+            y.y = x.x * di + x.y * dr;
+        }
+        stg<true>(&psi[i], y);
+    }
+}
+
+// Single transposition E0 <-> E1 (CNOT, SWAP, CCNOT, CSWAP) with a bit inside
+// the line and one at or above bit 6 (the all-lane case is k_perm_lane).  The
+// moving amplitude at in-line position l of line A (upper bits h0) pairs with
+// position l ^ (l0 ^ l1) of line B (upper bits h1): a lane shuffle.  With
+// h0 == h1 both entries lie in the same lines.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_perm_line(vec2_t<T> *__restrict__ psi, const GroupSpec hg,
+                                                      uint64_t count, uint64_t h0, uint64_t h1, uint32_t lmask,
+                                                      uint32_t l0, uint32_t l1) {
+    using V = vec2_t<T>;
+    const int d = (int)(l0 ^ l1);
+    const bool two = h0 != h1;
+    const uint64_t step = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t j = (uint64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; j < count; j += step) {
+        const uint64_t g = group_base(hg, j);
+        const uint64_t ia = g | h0, ib = g | h1;
+        const V xa = ldg<true>(&psi[ia]);
+        const V xb = two ? ldg<true>(&psi[ib]) : xa;
+        V pa, pb;
+        pa.x = __shfl_xor(xa.x, d, 64);
+        pa.y = __shfl_xor(xa.y, d, 64);
+        pb.x = __shfl_xor(xb.x, d, 64);
+        pb.y = __shfl_xor(xb.y, d, 64);
+        const uint32_t ls = (uint32_t)ia & lmask;
+        if (two) {
+            stg<true>(&psi[ia], ls == l0 ? pb : xa);
+            stg<true>(&psi[ib], ls == l1 ? pa : xb);
+        } else {
+            stg<true>(&psi[ia], (ls == l0 || ls == l1) ? pa : xa);
+        }
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fill_zero_state(vec2_t<T> *__restrict__ psi,
                                                             uint64_t len) {
@@ -552,6 +617,29 @@ uint32_t grid_for(uint64_t work, uint64_t per_block) {
     return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(b, 1), kMaxGrid);
 }
 
+// The gate's bits above the 128 B line as a GroupSpec of their own (no
+// controls), and the mask of its bits inside the line, for the line forms.
+struct LineSplit {
+    GroupSpec hg;
+    uint64_t hmask;
+    uint32_t lmask;
+};
+LineSplit line_split(const GroupSpec &g, uint32_t n, size_t amp_bytes) {
+    const uint32_t line_bits = amp_bytes == 16 ? 3u : 4u;  // 128 B lines
+    LineSplit ls;
+    std::memset(&ls, 0, sizeof(ls));
+    for (uint32_t i = 0; i < g.nsorted; ++i) {
+        if (g.sorted[i] < line_bits) {
+            ls.lmask |= 1u << g.sorted[i];
+        } else {
+            ls.hg.sorted[ls.hg.nsorted++] = g.sorted[i];
+            ls.hmask |= uint64_t(1) << g.sorted[i];
+        }
+    }
+    ls.hg.ngroups = uint64_t(1) << (n - ls.hg.nsorted);
+    return ls;
+}
+
 // Non-temporal access pays off when every contiguous run of the gate's
 // access pattern spans >= 128 B: the lowest target/control qubit q gives
 // runs of 2^q amplitudes.
@@ -693,9 +781,25 @@ int run_diag(qvmhip_state *s, const GateSplit &gs) {
     }
     constexpr int GPT = 1;
     const dim3 grid(grid_for(a.g.ngroups, (uint64_t)kBlock * GPT));
-    prof_begin(s, QVMHIP_K_DIAG_SPARSE);
     const bool lr = long_runs(a.g, s->amp_bytes);
     V2Ptr<T> psi = (vec2_t<T> *)s->amps;
+    // FP64 only: one 8-byte FP32 amplitude per lane makes 512 B wavefront
+    // accesses, and the line form measured slower there (CPHASE(2, 9) at
+    // n = 30: 1.64 vs 1.50 ms); FP64 CPHASE(1, 8) 3.14 -> 2.65 ms
+    if (sizeof(T) == 8 && a.nent == 1 && !lr) {
+        const LineSplit ls = line_split(a.g, s->n, s->amp_bytes);
+        if (ls.hg.ngroups % kBlock == 0) {
+            const uint64_t e0 = a.off[0] | a.g.ctrl_mask;
+            prof_begin(s, QVMHIP_K_DIAG_SPARSE);
+            hipLaunchKernelGGL((k_diag_line<T>), dim3(grid_for(ls.hg.ngroups, kBlock)), dim3(kBlock), 0, s->stream,
+                               psi, ls.hg, ls.hg.ngroups, e0 & ls.hmask, ls.lmask, (uint32_t)e0 & ls.lmask, a.dr[0],
+                               a.di[0]);
+            prof_end(s);
+            QH_LAUNCH_CHECK();
+            return QVMHIP_OK;
+        }
+    }
+    prof_begin(s, QVMHIP_K_DIAG_SPARSE);
     if (a.nent == 1) {
         if (lr) hipLaunchKernelGGL((k_diag_sparse<T, GPT, true, true>), grid, dim3(kBlock), 0, s->stream, psi, a);
         else hipLaunchKernelGGL((k_diag_sparse<T, GPT, false, true>), grid, dim3(kBlock), 0, s->stream, psi, a);
@@ -752,9 +856,24 @@ int run_perm(qvmhip_state *s, const GateSplit &gs) {
     }
     constexpr int GPT = 1;
     const dim3 grid(grid_for(a.g.ngroups, (uint64_t)kBlock * GPT));
-    prof_begin(s, QVMHIP_K_PERM);
     const bool lr = long_runs(a.g, s->amp_bytes);
     V2Ptr<T> psi = (vec2_t<T> *)s->amps;
+    // FP64 (as for diagonals): CNOT(2, 9) at n = 30 3.14 -> 2.60 ms, SWAP(0,
+    // 29) 6.71 -> 5.49 ms
+    if (sizeof(T) == 8 && a.nent == 2 && !lr) {
+        const LineSplit ls = line_split(a.g, s->n, s->amp_bytes);
+        if (ls.hg.ngroups % kBlock == 0) {
+            const uint64_t e0 = a.off[0] | a.g.ctrl_mask, e1 = a.off[1] | a.g.ctrl_mask;
+            prof_begin(s, QVMHIP_K_PERM);
+            hipLaunchKernelGGL((k_perm_line<T>), dim3(grid_for(ls.hg.ngroups, kBlock)), dim3(kBlock), 0, s->stream,
+                               psi, ls.hg, ls.hg.ngroups, e0 & ls.hmask, e1 & ls.hmask, ls.lmask,
+                               (uint32_t)e0 & ls.lmask, (uint32_t)e1 & ls.lmask);
+            prof_end(s);
+            QH_LAUNCH_CHECK();
+            return QVMHIP_OK;
+        }
+    }
+    prof_begin(s, QVMHIP_K_PERM);
     if (a.nent == 2) {
         if (lr) hipLaunchKernelGGL((k_perm<T, GPT, true, true>), grid, dim3(kBlock), 0, s->stream, psi, a);
         else hipLaunchKernelGGL((k_perm<T, GPT, false, true>), grid, dim3(kBlock), 0, s->stream, psi, a);

@@ -3,8 +3,9 @@ dense gates on every target (one amplitude per thread with lane shuffles for
 targets < 6, LDS exchange for 6 and 7, the pair form above; controlled gates
 always take the pair form), full diagonals of width 1..4 (compile-time entry
 selects for 1..3, indexed entries for 4; two amplitudes per access in FP32),
-and single transpositions whose bits are all low (whole-state lane sweep) or
-not.  FP64 results are bit-identical to the oracle's serial arithmetic; FP32
+single transpositions whose bits are all low (whole-state lane sweep) or
+not, and whole-line sweeps for single-entry diagonals and transpositions with
+a bit inside a 128 B line.  FP64 results are bit-identical to the oracle's serial arithmetic; FP32
 within 1e-6."""
 import sys
 
@@ -104,6 +105,37 @@ def test_transpositions_low_and_high_bits(S, oracle, dtype):
         st.apply_matrix(U, G.nat_tuple(*qs))
         oracle.apply_matrix(ref, U, G.nat_tuple(*qs))
     check(st, ref, dtype)
+    st.close()
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_line_sweeps_for_bits_inside_a_line(S, oracle, dtype):
+    """FP64 single-entry diagonals and single transpositions with a bit
+    inside a 128 B line (below bit 3) and one above bit 5 sweep whole lines
+    (k_diag_line, k_perm_line): entries in two line sets (SWAP, CNOT with the
+    control inside the line) or in one (CNOT with the target inside), controls
+    inside and outside, every bit inside the line (PHASE, CPHASE 1 0).  FP32
+    keeps the entry and pair forms for the same gates."""
+    n = 13
+    rng = np.random.default_rng(61)
+    say(f"line sweeps {dtype}")
+    psi = rand_state(n, rng)
+    st = S(n, dtype)
+    st.write_amplitudes(psi)
+    ref = psi.astype(np.complex64).astype(np.complex128) if dtype == "f32" else psi.copy()
+    ccp = np.eye(8, dtype=np.complex128)
+    ccp[7, 7] = np.exp(0.7j)
+    cases = [(G.gate_matrix("CZ"), [0, 9]), (G.gate_matrix("CPHASE", [0.3]), [2, 12]),
+             (G.gate_matrix("CPHASE", [1.1]), [1, 0]), (G.gate_matrix("PHASE", [0.9]), [1]),
+             (G.gate_matrix("CZ"), [3, 9]), (ccp, [1, 7, 11]), (G.gate_matrix("CPHASE", [0.5]), [11, 0]),
+             (G.gate_matrix("SWAP"), [0, 12]), (G.gate_matrix("SWAP"), [2, 7]), (G.gate_matrix("CNOT"), [1, 11]),
+             (G.gate_matrix("CNOT"), [10, 2]), (G.gate_matrix("CSWAP"), [9, 0, 11]),
+             (G.gate_matrix("CCNOT"), [0, 1, 8]), (G.gate_matrix("SWAP"), [3, 6]), (G.gate_matrix("CNOT"), [3, 12])]
+    for U, qs in cases:
+        say(f"  {dtype} {U.shape[0]}x{U.shape[0]} on {qs}")
+        st.apply_matrix(U, G.nat_tuple(*qs))
+        oracle.apply_matrix(ref, U, G.nat_tuple(*qs))
+        check(st, ref, dtype)
     st.close()
 
 
