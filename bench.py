@@ -165,7 +165,7 @@ def main() -> int:
     unit_bytes = 2 * (1 << n) * 16  # dense 1q / diagonal sweep: read + write every amplitude
     roofline = {
         "kernel": {"dense1": "1-qubit dense family: qvmhip::k_dense1_lane<double> (target < 6), "
-                             "qvmhip::k_dense1_xchg<double> (6, 7), qvmhip::k_dense<double, 1, 1, NT> (>= 8)",
+                             "qvmhip::k_dense1_xchg<double> (>= 6)",
                    "diag_full": "qvmhip::k_diag_full<double, true, 1>"}.get(dom, dom),
         "bound": "hbm",
         "achieved": round(unit_bytes / (per_launch_ms * 1e-3) / 1e9, 1),

@@ -161,9 +161,10 @@ __global__ __launch_bounds__(kBlock) void k_dense1_lane(vec2_t<T> *__restrict__ 
     }
 }
 
-// q >= 6: a workgroup takes 128 pairs; waves 0 and 2 load 64 lower members
-// each (contiguous), waves 1 and 3 the matching upper members, and the
-// partners are exchanged through LDS.  npairs is a multiple of 128.
+// q >= 6 (any target up to n - 1): a workgroup takes 128 pairs; waves 0 and
+// 2 load 64 lower members each (contiguous), waves 1 and 3 the matching upper
+// members, and the partners are exchanged through LDS.  npairs is a multiple
+// of 128.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dense1_xchg(vec2_t<T> *__restrict__ psi, uint64_t npairs, uint32_t q,
                                                         const Dense1Coef<T> u) {
@@ -663,8 +664,11 @@ int run_dense_small(qvmhip_state *s, const GateSplit &gs) {
     if constexpr (KT == 1 && sizeof(T) == 8) {
         // FP64 only: with 8-byte FP32 amplitudes one per lane is a 512 B
         // wavefront access, slower than the pair form (5.8 vs 6.6 TB/s)
-        if (a.g.ctrl_mask == 0 && s->n >= 8 && gs.tq[0] < 8) {
-            // low target: one amplitude per thread (k_dense1_lane / k_dense1_xchg)
+        if (a.g.ctrl_mask == 0 && s->n >= 8) {
+            // one amplitude per thread: k_dense1_lane below 6, k_dense1_xchg
+            // from 6 up.  With XCD-contiguous workgroups the exchange form
+            // beat the pair form on every target 8..29 at n = 30 (5.08-5.84
+            // against 5.18-6.11 ms, 4 % less on average)
             Dense1Coef<T> u;
             for (int i = 0; i < 4; ++i) {
                 u.ur[i] = (T)gs.V[i].real();

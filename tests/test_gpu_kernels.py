@@ -1,7 +1,7 @@
 """Plain-kernel dispatch paths against the oracle, one case per path: 1-qubit
 dense gates on every target (one amplitude per thread with lane shuffles for
-targets < 6, LDS exchange for 6 and 7, the pair form above; controlled gates
-always take the pair form), full diagonals of width 1..4 (compile-time entry
+targets < 6, LDS exchange from 6 up in FP64; FP32 and controlled gates take
+the pair form), full diagonals of width 1..4 (compile-time entry
 selects for 1..3, indexed entries for 4; two amplitudes per access in FP32),
 single transpositions whose bits are all low (whole-state lane sweep) or
 not, and whole-line sweeps for single-entry diagonals and transpositions with
