@@ -398,6 +398,33 @@ __global__ __launch_bounds__(kBlock) void k_perm_lane(vec2_t<T> *__restrict__ ps
     }
 }
 
+// A single transposition E0 <-> E1 whose bits are all at or above 6 (CNOT,
+// SWAP, CCNOT, CSWAP on high qubits), in the layout of k_dense1_xchg: a
+// workgroup takes 128 consecutive groups, waves 0 and 2 load the E0 members
+// of 64 groups each (a contiguous run), waves 1 and 3 the E1 members, the
+// partners are exchanged through LDS and each thread stores its partner's
+// amplitude at its own address.  Copies only, so results are exact.
+// ngroups is a multiple of 128.  Measured at n = 30 against the pair form
+// (k_perm): CNOT(q, q + 7) for q = 6, 8, 12, 15, 20, 22 2.54-2.89 ms against
+// 2.59-3.01, SWAP(q, 29 - q) for q = 8, 12 2.58 / 2.62 against 2.77 / 2.80.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_perm_xchg(vec2_t<T> *__restrict__ psi, const GroupSpec g, uint64_t e0,
+                                                      uint64_t e1) {
+    using V = vec2_t<T>;
+    __shared__ V sm[kBlock];
+    const uint32_t t = threadIdx.x, h = (t >> 6) & 1u;
+    const uint64_t nchunk = g.ngroups / 128;
+    for (uint64_t c = xcd_block(blockIdx.x, gridDim.x); c < nchunk; c += gridDim.x) {
+        const uint64_t grp = c * 128 + ((t >> 7) << 6) + (t & 63u);
+        const uint64_t a = group_base(g, grp) | (h ? e1 : e0);
+        sm[t] = ldg<true>(&psi[a]);
+        __syncthreads();
+        const V p = sm[t ^ 64u];
+        __syncthreads();  // sm is rewritten by the next chunk
+        stg<true>(&psi[a], p);
+    }
+}
+
 // Gates whose lowest bit lies inside a 128 B line touch every line of the
 // lines they change but only some amplitudes of each, so the pair and entry
 // forms above read and write partial lines.  The line forms sweep whole
@@ -862,6 +889,14 @@ int run_perm(qvmhip_state *s, const GateSplit &gs) {
     const dim3 grid(grid_for(a.g.ngroups, (uint64_t)kBlock * GPT));
     const bool lr = long_runs(a.g, s->amp_bytes);
     V2Ptr<T> psi = (vec2_t<T> *)s->amps;
+    if (sizeof(T) == 8 && a.nent == 2 && a.g.nsorted && a.g.sorted[0] >= 6 && a.g.ngroups % 128 == 0) {
+        prof_begin(s, QVMHIP_K_PERM);
+        hipLaunchKernelGGL((k_perm_xchg<T>), dim3(grid_for(a.g.ngroups, 128)), dim3(kBlock), 0, s->stream,
+                           (vec2_t<T> *)s->amps, a.g, a.off[0], a.off[1]);
+        prof_end(s);
+        QH_LAUNCH_CHECK();
+        return QVMHIP_OK;
+    }
     // FP64 (as for diagonals): CNOT(2, 9) at n = 30 3.14 -> 2.60 ms, SWAP(0,
     // 29) 6.71 -> 5.49 ms
     if (sizeof(T) == 8 && a.nent == 2 && !lr) {

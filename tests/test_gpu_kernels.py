@@ -139,6 +139,28 @@ def test_line_sweeps_for_bits_inside_a_line(S, oracle, dtype):
     st.close()
 
 
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_transpositions_with_every_bit_from_6_up(S, oracle, dtype):
+    """FP64 single transpositions whose bits are all >= 6 run in the LDS
+    exchange form (k_perm_xchg): CNOT both ways, SWAP, CCNOT, CSWAP.  Bit-exact
+    against the oracle; FP32 takes the pair form."""
+    n = 13
+    rng = np.random.default_rng(67)
+    say(f"exchange-form transpositions {dtype}")
+    psi = rand_state(n, rng)
+    st = S(n, dtype)
+    st.write_amplitudes(psi)
+    ref = psi.astype(np.complex64).astype(np.complex128) if dtype == "f32" else psi.copy()
+    cases = [("CNOT", [7, 12]), ("CNOT", [12, 6]), ("SWAP", [6, 11]), ("CCNOT", [8, 10, 12]),
+             ("CSWAP", [12, 6, 9]), ("SWAP", [9, 10])]
+    for name, qs in cases:
+        U = G.gate_matrix(name)
+        st.apply_matrix(U, G.nat_tuple(*qs))
+        oracle.apply_matrix(ref, U, G.nat_tuple(*qs))
+        check(st, ref, dtype)
+    st.close()
+
+
 def test_general_permutations_by_cycles(S, oracle):
     """Permutation gates with several moved entries (3-cycles, a 4-cycle plus
     a transposition, random permutations of 3 qubits, a controlled 3-cycle)
