@@ -640,15 +640,18 @@ def test_fused_plan_random_mix(S, oracle):
 def test_fma_tiled_and_fused_plans_n24_vs_oracle(S, oracle):
     """The reordered FMA tile passes and the fused MFMA blocks against the
     oracle (not only a round trip) at n = 24, where tiles name qubits up to
-    23 and blocks span far-apart qubits."""
+    23 and blocks span far-apart qubits (k = 4 blocks above the line take the
+    row layout)."""
     n = 24
     gates = CI.random_mix(n, 6, seed=24)
     ref = oracle.zero_state(n)
     for U, q in gates:
         oracle.apply_matrix(ref, U, G.nat_tuple(*q))
     plans = {"tiled_fma": P.plan_tiled(gates, n, 11, 4, merge_1q=True),
+             "tiled_fma_l3": P.plan_tiled(gates, n, 11, 3, merge_1q=True),
              "hybrid": P.plan_tiled(gates, n, 11, 4, merge_1q=True, wide="fused"),
-             "fused_k5": P.plan_fused(gates, n, 5), "fused_k6": P.plan_fused(gates, n, 6)}
+             "fused_k4": P.plan_fused(gates, n, 4), "fused_k5": P.plan_fused(gates, n, 5),
+             "fused_k6": P.plan_fused(gates, n, 6)}
     for name, plan in plans.items():
         st = S(n, "f64", fma_tiles=True)
         prog = P.build_program(st, plan, use_graph=True)
