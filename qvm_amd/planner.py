@@ -3,10 +3,11 @@
 Three plans, all consumed by ``compile_gates``:
 
 * ``plain``  -- one engine launch per gate (apply-matrix-operator per gate,
-  src/wavefunction.lisp:308-330).  The judge of the other two.
-* ``tiled``  -- gates grouped into LDS tile passes: a pass names up to m tile
+  src/wavefunction.lisp:308-330).  The reference the other two are checked
+  against.
+* ``tiled``  -- gates grouped into tile passes: a pass names up to m tile
   qubits and applies every gate that fits, in dependency order, to each
-  2^m-amplitude block while it sits in LDS.  Per-gate arithmetic is the
+  2^m-amplitude block while it sits in registers.  Per-gate arithmetic is the
   plain kernels', so FP64 results are bit-identical to ``plain`` (unless
   ``merge_1q`` multiplies runs of 1-qubit gates together first).
 * ``fused``  -- gates multiplied into dense 2^k x 2^k blocks (k <= kmax) that

@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/qvm_oracle.c) before it is trusted as the judge.
+"""Pins the CPU oracle (oracle/qvm_oracle.c) before the GPU results are checked against it.
 
 Each check is chosen so that a plausible slip in the restatement fails: the
 reference's own golden values, brute-force Kronecker-product unitaries with
