@@ -708,6 +708,47 @@ def test_program_graph_replay(S, oracle, mode):
     eager.close()
 
 
+def test_path_sum_and_xy_piswap_programs(S, oracle):
+    """The reference's MADPI programs (tests/path-simulate-tests.lisp:21-30)
+    run on the engine equal their path sums, and XY(pi/2) 1 0 equals
+    PISWAP(pi/2) 1 0 on random 2-qubit states (tests/gate-tests.lisp:115-127),
+    both also against the oracle."""
+    import json
+    import os
+
+    from qvm_amd import qvm as QV
+    from qvm_amd import quil as Q
+
+    from tests.test_oracle import path_sum
+
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.json")))
+    for lines in gold["path_sum_programs"]["programs"]:
+        n = 1 + max(q for ins in Q.parse_quil("\n".join(lines)).executable_code for q in ins.qubits)
+        q = QV.run_program(n, "\n".join(lines))
+        err = np.max(np.abs(q.amplitudes() - path_sum(lines, n)))
+        say(f"path sum {lines}: {err:.1e}")
+        assert err < 1e-15, lines
+        q.close()
+    rng = np.random.default_rng(71)
+    xy, pis = ("\n".join(p) for p in gold["xy_piswap"]["programs"])
+    for _ in range(10):
+        psi = rand_state(2, rng)
+        outs = []
+        for prog in (xy, pis):
+            q = QV.make_qvm(2)
+            q.set_amplitudes(psi)
+            q.load_program(prog)
+            q.run()
+            outs.append(q.amplitudes())
+            q.close()
+            p = Q.parse_quil(prog)
+            ref = psi.copy()
+            for ins in p.executable_code:
+                oracle.apply_matrix(ref, Q.resolve_gate_matrix(ins, p.gate_definitions, {}), G.nat_tuple(*ins.qubits))
+            assert np.max(np.abs(outs[-1] - ref)) < 1e-15
+        assert np.max(np.abs(outs[0] - outs[1])) < 1e-15
+
+
 def test_qvm_api_end_to_end(S):
     import json
     import os

@@ -294,6 +294,51 @@ def test_dagger_round_trip_golden(oracle):
     assert abs(psi[0] - 1) < 1e-14 and np.max(np.abs(psi[1:])) < 1e-14
 
 
+def path_sum(lines, n):
+    """Amplitudes of a program from |0...0> as a sum over computational-basis
+    paths: each gate maps every basis state x with amplitude a to the states
+    y that differ from x on the gate's qubits only, with weight U[y, x], the
+    gate matrix indexed with the gate's first qubit as the most significant
+    bit (Quil's convention).  No tensor products and no nat-tuple reversal,
+    so it checks the engine's qubit ordering independently."""
+    prog = Q.parse_quil("\n".join(lines))
+    amps = {0: 1.0 + 0j}
+    for ins in prog.executable_code:
+        U = Q.resolve_gate_matrix(ins, prog.gate_definitions, {})
+        qs = list(ins.qubits)
+        k = len(qs)
+        nxt = {}
+        for x, a in amps.items():
+            col = sum(((x >> q) & 1) << (k - 1 - j) for j, q in enumerate(qs))
+            base = x
+            for q in qs:
+                base &= ~(1 << q)
+            for row in range(1 << k):
+                w = U[row, col]
+                if w == 0:
+                    continue
+                y = base
+                for j, q in enumerate(qs):
+                    if (row >> (k - 1 - j)) & 1:
+                        y |= 1 << q
+                nxt[y] = nxt.get(y, 0) + w * a
+        amps = nxt
+    out = np.zeros(1 << n, dtype=np.complex128)
+    for x, a in amps.items():
+        out[x] = a
+    return out
+
+
+def test_path_sum_programs_oracle(oracle):
+    """The reference's MADPI programs (tests/path-simulate-tests.lisp:21-30):
+    the oracle's wavefunction equals the path sum."""
+    for lines in GOLD["path_sum_programs"]["programs"]:
+        n = 1 + max(q for ins in Q.parse_quil("\n".join(lines)).executable_code for q in ins.qubits)
+        assert np.max(np.abs(run_quil_oracle(oracle, lines, n) - path_sum(lines, n))) < 1e-15, lines
+    # a swapped control and target changes the result, so the check has teeth
+    assert np.max(np.abs(path_sum(["H 0", "CNOT 0 1"], 2) - path_sum(["H 0", "CNOT 1 0"], 2))) > 0.5
+
+
 def test_modifier_identities():
     X = G.gate_matrix("X")
     assert np.array_equal(G.controlled(X), G.gate_matrix("CNOT"))
