@@ -328,6 +328,137 @@ __global__ __launch_bounds__(256) void k_fused_rows(vec2_t<T> *__restrict__ psi,
     }
 }
 
+// Two registers each holding two rows of 32 columns (lanes 0-31 one row,
+// lanes 32-63 the next) <-> the B fragments of two 16-column groups: one
+// permlane32_swap and one permlane16_swap per word; the inverse applies them
+// in the other order (each swap is an involution).
+__device__ __forceinline__ void rows_to_frag2_words(uint32_t &a0, uint32_t &a1) {
+    const auto p = __builtin_amdgcn_permlane32_swap(a0, a1, false, false);
+    const auto r = __builtin_amdgcn_permlane16_swap(p[0], p[1], false, false);
+    a0 = r[0];
+    a1 = r[1];
+}
+__device__ __forceinline__ void frag2_to_rows_words(uint32_t &a0, uint32_t &a1) {
+    const auto r = __builtin_amdgcn_permlane16_swap(a0, a1, false, false);
+    const auto p = __builtin_amdgcn_permlane32_swap(r[0], r[1], false, false);
+    a0 = p[0];
+    a1 = p[1];
+}
+template <bool TO_FRAG, typename X>
+__device__ __forceinline__ void xpose2(X &a0, X &a1) {
+    static_assert(sizeof(X) % 4 == 0, "whole words");
+    constexpr int W = sizeof(X) / 4;
+    uint32_t w0[W], w1[W];
+    __builtin_memcpy(w0, &a0, sizeof(X));
+    __builtin_memcpy(w1, &a1, sizeof(X));
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+        if constexpr (TO_FRAG)
+            rows_to_frag2_words(w0[i], w1[i]);
+        else
+            frag2_to_rows_words(w0[i], w1[i]);
+    }
+    __builtin_memcpy(&a0, w0, sizeof(X));
+    __builtin_memcpy(&a1, w1, sizeof(X));
+}
+
+// FP64 K = 5 in rows of 32 columns: a wave takes 32 consecutive complement
+// groups, and each load or store moves two rows of them (512 B runs, against
+// 256 B in the operand layout); half the accumulators of k_fused_rows, so the
+// 32-row blocks fit.  Same products as k_fused.  Measured at n = 30 on six
+// target sets above the line: 5.85-6.74 ms against 6.81-7.01 for k_fused,
+// 5.5 % less in total (+20 % on 5,11,17,23,27).
+template <typename T, int K, bool NTM>
+__global__ __launch_bounds__(256) void k_fused_rows2(vec2_t<T> *__restrict__ psi, const double *__restrict__ U,
+                                                     GroupSpec g, FusedOff off) {
+    using V = vec2_t<T>;
+    using v4 = typename Mfma<T>::v4;
+    constexpr int D = 1 << K;
+    constexpr int MT = D < 16 ? 1 : D / 16;
+    constexpr int KS = D / 4;
+    __shared__ V fr[MT * KS * 64];
+    build_fragments<T, K>(fr, U, off);
+
+    const int lane = threadIdx.x & 63;
+    const int half = lane >> 5;
+    const uint64_t ncb = g.ngroups / 32;
+    const XcdShare sh = xcd_share(blockIdx.x, gridDim.x, ncb);
+    const uint64_t wstride = (uint64_t)sh.count * 4;
+    // lane parts of the row offsets: loads take row bit 0 = half; the rows of
+    // result lane block b = 2r + half have half at bit 0 (f64) or bit 2 (f32)
+    const uint64_t lh_ld = off.o[half];
+    const uint64_t lh_st = off.o[sizeof(T) == 8 ? half : 4 * half];
+    for (uint64_t cb = sh.lo + (uint64_t)sh.rank * 4 + (threadIdx.x >> 6); cb < sh.hi; cb += wstride) {
+        const uint64_t base = group_base(g, cb * 32 + (uint64_t)(lane & 31));
+        // the A fragments are read per column block: hoisted out of the loop
+        // they would hold 16 registers per k-step
+        const V *frl = fr + lane;
+        asm volatile("" : "+v"(frl));
+        v4 ar[2][MT], ai[2][MT];
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                ar[c][mt] = v4{0, 0, 0, 0};
+                ai[c][mt] = v4{0, 0, 0, 0};
+            }
+        // register 0 holds rows 4ks, 4ks+1, register 1 rows 4ks+2, 4ks+3 (one
+        // per half-wave); the next k-step's are in flight during the MFMAs.
+        // A row's offset is the OR of its bits' offsets: the half-wave's bit
+        // (row bit 0) is the lane part, the rest is uniform.
+        V xn[2];
+        xn[0] = ldg<NTM>(&psi[base + (off.o[0] | lh_ld)]);
+        xn[1] = ldg<NTM>(&psi[base + (off.o[2] | lh_ld)]);
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            V b[2] = {xn[0], xn[1]};
+            if (ks + 1 < KS) {
+                xn[0] = ldg<NTM>(&psi[base + (off.o[4 * (ks + 1)] | lh_ld)]);
+                xn[1] = ldg<NTM>(&psi[base + (off.o[4 * (ks + 1) + 2] | lh_ld)]);
+            }
+            xpose2<true>(b[0], b[1]);  // -> B fragments of column groups 0, 1
+            V a[MT];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) a[mt] = frl[(mt * KS + ks) * 64];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const T br = b[c].x, bi = b[c].y, nbi = -b[c].y;
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) {
+                    ar[c][mt] = Mfma<T>::mma(a[mt].x, br, ar[c][mt]);
+                    ai[c][mt] = Mfma<T>::mma(a[mt].y, br, ai[c][mt]);
+                    ar[c][mt] = Mfma<T>::mma(a[mt].y, nbi, ar[c][mt]);
+                    ai[c][mt] = Mfma<T>::mma(a[mt].x, bi, ai[c][mt]);
+                }
+            }
+        }
+        // result register e of column group c, lane block b: row 16mt + b +
+        // 4e (f64) or 16mt + 4b + e (f32); back to rows, register r holds
+        // lane block 2r + half's row for the wave's 32 columns
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                T yr[2] = {ar[0][mt][e], ar[1][mt][e]};
+                T yi[2] = {ai[0][mt][e], ai[1][mt][e]};
+                xpose2<false>(yr[0], yr[1]);
+                xpose2<false>(yi[0], yi[1]);
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+                    // uniform part of the row of lane block b = 2r + half
+                    const int hi = 16 * mt + (sizeof(T) == 8 ? 2 * r + 4 * e : 8 * r + e);
+                    if (hi < D) {
+                        V y;
+                        y.x = yr[r];
+                        y.y = yi[r];
+                        stg<NTM>(&psi[base + (off.o[hi] | lh_st)], y);
+                    }
+                }
+            }
+        }
+    }
+}
+
 namespace {
 template <typename T, int K>
 int run_fused(qvmhip_state *s, const void *mat_dev, const uint32_t *qubits) {
@@ -375,6 +506,27 @@ int run_fused(qvmhip_state *s, const void *mat_dev, const uint32_t *qubits) {
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess || ncu <= 0)
         ncu = 256;
+    if constexpr (K == 5 && sizeof(T) == 8) {
+        if (g.ngroups >= 32 && pos[0] >= line_bits) {
+            static int per_cu_rows2 = 0;
+            if (per_cu_rows2 == 0) {
+                int nb = 0;
+                per_cu_rows2 = (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fused_rows2<T, K, true>, 256, 0) ==
+                                    hipSuccess &&
+                                nb > 0)
+                                   ? nb
+                                   : 2;
+            }
+            const uint64_t ncb = g.ngroups / 32;
+            const uint32_t grid2 = (uint32_t)std::min<uint64_t>((ncb + 3) / 4, (uint64_t)per_cu_rows2 * (uint64_t)ncu);
+            prof_begin(s, QVMHIP_K_FUSED);
+            hipLaunchKernelGGL((k_fused_rows2<T, K, true>), dim3(grid2), dim3(256), 0, s->stream,
+                               (vec2_t<T> *)s->amps, (const double *)mat_dev, g, off);
+            prof_end(s);
+            QH_LAUNCH_CHECK();
+            return QVMHIP_OK;
+        }
+    }
     if constexpr (K == 4 || (K == 5 && sizeof(T) == 4)) {
         // one row per memory instruction when every target lies above the
         // 128 B line, so that each row's lines are read whole by one
