@@ -613,10 +613,13 @@ __device__ __forceinline__ void tile_load(const vec2_t<T> *__restrict__ psi, con
 template <typename T, int M, int RB>
 __device__ __forceinline__ void tile_store(vec2_t<T> *__restrict__ psi, const TileArgs &a, uint64_t gb,
                                            uint64_t toff, const vec2_t<T> (&x)[1 << RB]) {
-    vec2_t<T> *p = psi + gb + toff;
     // recompute the store addresses instead of keeping them live across
-    // the op loop
-    asm volatile("" : "+v"(p));
+    // the op loop (the barrier is on the offset: on the pointer itself it
+    // would hide the global address space, and flat stores count against
+    // the LDS and scalar-load waits of the next tile)
+    uint64_t o = gb + toff;
+    asm volatile("" : "+v"(o));
+    vec2_t<T> *p = psi + o;
 #pragma unroll
     for (int r = 0; r < (1 << RB); ++r) stg<true>(&p[a.roff[r]], x[r]);
 }
@@ -789,7 +792,7 @@ struct TileEntry {
 };
 
 template <typename T, int M, int RB, bool FMA, bool CUT>
-__global__ __launch_bounds__(1 << (M - RB)) void k_tile_batch(vec2_t<T> *__restrict__ base,
+__global__ __launch_bounds__(1 << (M - RB)) __attribute__((amdgpu_waves_per_eu(FMA && sizeof(T) == 8 ? 5 : 1))) void k_tile_batch(vec2_t<T> *__restrict__ base,
                                                              const TileEntry *__restrict__ ent,
                                                              const uint32_t *__restrict__ blk_ent,
                                                              uint64_t total_blocks, uint32_t tiles_per_block,
