@@ -391,9 +391,13 @@ __global__ __launch_bounds__(256) void k_fused_rows2(vec2_t<T> *__restrict__ psi
     for (uint64_t cb = sh.lo + (uint64_t)sh.rank * 4 + (threadIdx.x >> 6); cb < sh.hi; cb += wstride) {
         const uint64_t base = group_base(g, cb * 32 + (uint64_t)(lane & 31));
         // the A fragments are read per column block: hoisted out of the loop
-        // they would hold 16 registers per k-step
-        const V *frl = fr + lane;
-        asm volatile("" : "+v"(frl));
+        // they would hold 16 registers per k-step (the barrier is on the
+        // index: on the pointer it would hide the LDS address space, and the
+        // reads would become flat loads that also wait for the next k-step's
+        // global loads)
+        uint32_t fo = (uint32_t)lane;
+        asm volatile("" : "+v"(fo));
+        const V *frl = fr + fo;
         v4 ar[2][MT], ai[2][MT];
 #pragma unroll
         for (int c = 0; c < 2; ++c)

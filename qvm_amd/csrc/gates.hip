@@ -360,13 +360,19 @@ __global__ __launch_bounds__(kBlock) void k_perm(vec2_t<T> *__restrict__ psi,
                 // cycle by cycle with one kept value: within a cycle each
                 // destination's source is the next entry's destination, read
                 // before that entry overwrites it
+                // (the source is read even where the kept value replaces
+                // it: a choice between two loads would be merged into one
+                // flat load through a scratch copy of kept)
                 V kept;
                 kept.x = kept.y = T(0);
                 for (uint32_t e = 0; e < a.nent; ++e) {
                     const uint32_t f = a.flag[e];
-                    if (f & 1u) kept = psi[base + a.off[e]];
-                    const V v = (f & 2u) ? kept : psi[base + a.src[e]];
-                    psi[base + a.off[e]] = v;
+                    if (f & 1u) kept = ldg<NTM>(&psi[base + a.off[e]]);
+                    const V sv = ldg<NTM>(&psi[base + a.src[e]]);
+                    V v;
+                    v.x = (f & 2u) ? kept.x : sv.x;
+                    v.y = (f & 2u) ? kept.y : sv.y;
+                    stg<NTM>(&psi[base + a.off[e]], v);
                 }
             }
         }
