@@ -22,6 +22,7 @@
 // group's MFMAs.  The four real products per complex product are
 // all kept, so rounding is that of a complex matvec with FMA.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <vector>
 
@@ -464,6 +465,20 @@ __global__ __launch_bounds__(256) void k_fused_rows2(vec2_t<T> *__restrict__ psi
 }
 
 namespace {
+// Resident 256-thread workgroups per CU of a kernel, asked once per kernel
+// and kept in `cache` (concurrent first calls on several handles may each ask
+// and store the same value).
+template <typename F>
+int resident_per_cu(std::atomic<int> &cache, F kernel) {
+    int v = cache.load(std::memory_order_relaxed);
+    if (v == 0) {
+        int nb = 0;
+        v = (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 256, 0) == hipSuccess && nb > 0) ? nb : 2;
+        cache.store(v, std::memory_order_relaxed);
+    }
+    return v;
+}
+
 template <typename T, int K>
 int run_fused(qvmhip_state *s, const void *mat_dev, const uint32_t *qubits) {
     GroupSpec g;
@@ -487,14 +502,8 @@ int run_fused(qvmhip_state *s, const void *mat_dev, const uint32_t *qubits) {
     // Every wave grid-strides over equal shares of column groups, so the grid
     // is exactly the workgroups that fit on the device at once: a second,
     // partial round would run at a fraction of the occupancy.
-    static int per_cu = 0;
-    if (per_cu == 0) {
-        int nb = 0;
-        per_cu =
-            (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fused<T, K, false>, 256, 0) == hipSuccess && nb > 0)
-                ? nb
-                : 2;
-    }
+    static std::atomic<int> per_cu_cache{0};
+    const int per_cu = resident_per_cu(per_cu_cache, k_fused<T, K, false>);
     // Non-temporal accesses when every 128 B line a k-step touches is read
     // whole in that k-step: the k-step's rows vary the two lowest targets, so
     // no other target may lie below the line size.  Otherwise a line is read
@@ -512,15 +521,8 @@ int run_fused(qvmhip_state *s, const void *mat_dev, const uint32_t *qubits) {
         ncu = 256;
     if constexpr (K == 5 && sizeof(T) == 8) {
         if (g.ngroups >= 32 && pos[0] >= line_bits) {
-            static int per_cu_rows2 = 0;
-            if (per_cu_rows2 == 0) {
-                int nb = 0;
-                per_cu_rows2 = (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fused_rows2<T, K, true>, 256, 0) ==
-                                    hipSuccess &&
-                                nb > 0)
-                                   ? nb
-                                   : 2;
-            }
+            static std::atomic<int> rows2_cache{0};
+            const int per_cu_rows2 = resident_per_cu(rows2_cache, k_fused_rows2<T, K, true>);
             const uint64_t ncb = g.ngroups / 32;
             const uint32_t grid2 = (uint32_t)std::min<uint64_t>((ncb + 3) / 4, (uint64_t)per_cu_rows2 * (uint64_t)ncu);
             prof_begin(s, QVMHIP_K_FUSED);
@@ -541,15 +543,8 @@ int run_fused(qvmhip_state *s, const void *mat_dev, const uint32_t *qubits) {
         // 3.80 / 3.91.  With a target inside the line (0,5,11,17 FP64; 0,1,2,
         // 9,20 FP32) it lost 11-27 %, and at K = 3 4 %.
         if (g.ngroups >= 64 && pos[0] >= line_bits) {
-            static int per_cu_rows = 0;
-            if (per_cu_rows == 0) {
-                int nb = 0;
-                per_cu_rows = (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fused_rows<T, K, false>, 256, 0) ==
-                                   hipSuccess &&
-                               nb > 0)
-                                  ? nb
-                                  : 2;
-            }
+            static std::atomic<int> rows_cache{0};
+            const int per_cu_rows = resident_per_cu(rows_cache, k_fused_rows<T, K, false>);
             const bool nt_rows = sizeof(T) == 8;
             const uint64_t ncb = g.ngroups / 64;
             const uint32_t grid_rows =
