@@ -791,12 +791,14 @@ struct TileEntry {
     uint32_t from_zero;    // 1: the state is |0...0>, built in registers instead of loaded
 };
 
+// FP64 FMA forms held to five waves per SIMD (96 VGPRs, 11 spilled; left
+// alone they take 108 and four waves, 5 % slower on the 512-instance QAOA
+// batch); the exact forms would spill ~100 values at five.
 template <typename T, int M, int RB, bool FMA, bool CUT>
-__global__ __launch_bounds__(1 << (M - RB)) __attribute__((amdgpu_waves_per_eu(FMA && sizeof(T) == 8 ? 5 : 1))) void k_tile_batch(vec2_t<T> *__restrict__ base,
-                                                             const TileEntry *__restrict__ ent,
-                                                             const uint32_t *__restrict__ blk_ent,
-                                                             uint64_t total_blocks, uint32_t tiles_per_block,
-                                                             const char *__restrict__ op_buf) {
+__global__ __launch_bounds__(1 << (M - RB))
+__attribute__((amdgpu_waves_per_eu(FMA && sizeof(T) == 8 ? 5 : 1))) void k_tile_batch(
+    vec2_t<T> *__restrict__ base, const TileEntry *__restrict__ ent, const uint32_t *__restrict__ blk_ent,
+    uint64_t total_blocks, uint32_t tiles_per_block, const char *__restrict__ op_buf) {
     using V = vec2_t<T>;
     // the remap buffer: 2^M + 2^RB reals for FP64 (one component at a time),
     // as many amplitudes for FP32
