@@ -161,16 +161,22 @@ def test_transpositions_with_every_bit_from_6_up(S, oracle, dtype):
     st.close()
 
 
-def test_general_permutations_by_cycles(S, oracle):
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_general_permutations_by_cycles(S, oracle, dtype):
     """Permutation gates with several moved entries (3-cycles, a 4-cycle plus
     a transposition, random permutations of 3 qubits, a controlled 3-cycle)
-    run cycle by cycle with one kept value; a singular 0/1 matrix (two rows
-    taking one column) goes to the dense kernel.  Bit-exact against the
-    oracle."""
+    run cycle by cycle with one kept value, on random qubits and on fixed
+    sets whose moved amplitudes come in long runs (all bits >= 5) or short
+    ones (bits 0..2); a singular 0/1 matrix (two rows taking one column) goes
+    to the dense kernel.  A permutation only moves amplitudes, so both dtypes
+    are bit-exact against the oracle (FP32 starts from an FP32-rounded
+    state)."""
     rng = np.random.default_rng(57)
     n = 13
     psi = rand_state(n, rng)
-    st = S(n, "f64")
+    if dtype == "f32":
+        psi = psi.astype(np.complex64).astype(np.complex128)
+    st = S(n, dtype)
     st.write_amplitudes(psi)
     ref = psi.copy()
 
@@ -186,10 +192,17 @@ def test_general_permutations_by_cycles(S, oracle):
     ctrl3[4:, 4:] = pmat([1, 2, 0, 3])
     mats.append(ctrl3)
     sing = np.array([[1, 0], [1, 0]], dtype=np.complex128)
-    for U in mats + [sing]:
+    cases = [(U, None) for U in mats + [sing]]
+    cases += [(mats[0], [7, 12]), (mats[2], [5, 11, 8]), (ctrl3, [9, 6, 12]),
+              (mats[0], [1, 0]), (mats[2], [2, 0, 1]), (ctrl3, [0, 2, 1])]
+    for U, fixed in cases:
         k = int(U.shape[0]).bit_length() - 1
-        qs = [int(x) for x in rng.choice(n, k, replace=False)]
+        qs = fixed if fixed is not None else [int(x) for x in rng.choice(n, k, replace=False)]
         st.apply_matrix(U, qs)
         oracle.apply_matrix(ref, U, qs)
-        np.testing.assert_array_equal(st.amplitudes(), ref)
+        got = st.amplitudes()
+        if dtype == "f32":
+            # the singular matrix is arithmetic (1*a + 0*b): FP32-exact here too
+            got = np.asarray(got, dtype=np.complex128)
+        np.testing.assert_array_equal(got, ref)
     st.close()
