@@ -161,6 +161,43 @@ __global__ __launch_bounds__(kBlock) void k_dense1_lane(vec2_t<T> *__restrict__ 
     }
 }
 
+// FP32, q < 7: two consecutive amplitudes per lane (one 16-byte access, 1 KB
+// per wavefront; one 8-byte amplitude per lane gives 512 B accesses, which
+// sweep slower).  Element j holds amplitudes 2j and 2j + 1: for q = 0 they
+// are the pair; otherwise each pairs with the same half of lane j ^ 2^(q-1).
+// n4 (elements) is a multiple of kBlock, so every lane of every wave is live.
+__global__ __launch_bounds__(kBlock) void k_dense1_lane2_f32(float2 *__restrict__ psi, uint64_t n4, uint32_t q,
+                                                             const Dense1Coef<float> u) {
+    typedef float F4 __attribute__((ext_vector_type(4)));
+    F4 *p4 = reinterpret_cast<F4 *>(psi);
+    const uint64_t step = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t j = (uint64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; j < n4; j += step) {
+        const F4 v = __builtin_nontemporal_load(&p4[j]);
+        const float2 a0 = make_float2(v.x, v.y), a1 = make_float2(v.z, v.w);
+        float2 y0, y1;
+        if (q == 0) {
+            y0 = dense1_row<float>(a0, a1, 0u, u);
+            y1 = dense1_row<float>(a0, a1, 1u, u);
+        } else {
+            const int d = 1 << (q - 1);
+            float2 p0, p1;
+            p0.x = __shfl_xor(a0.x, d, 64);
+            p0.y = __shfl_xor(a0.y, d, 64);
+            p1.x = __shfl_xor(a1.x, d, 64);
+            p1.y = __shfl_xor(a1.y, d, 64);
+            const uint32_t r = (uint32_t)(j >> (q - 1)) & 1u;
+            y0 = dense1_row<float>(r ? p0 : a0, r ? a0 : p0, r, u);
+            y1 = dense1_row<float>(r ? p1 : a1, r ? a1 : p1, r, u);
+        }
+        F4 o;
+        o.x = y0.x;
+        o.y = y0.y;
+        o.z = y1.x;
+        o.w = y1.y;
+        __builtin_nontemporal_store(o, &p4[j]);
+    }
+}
+
 // q >= 6 (any target up to n - 1): a workgroup takes 128 pairs; waves 0 and
 // 2 load 64 lower members each (contiguous), waves 1 and 3 the matching upper
 // members, and the partners are exchanged through LDS.  npairs is a multiple
@@ -715,6 +752,21 @@ int run_dense_small(qvmhip_state *s, const GateSplit &gs) {
             else
                 hipLaunchKernelGGL((k_dense1_xchg<T>), dim3(grid_for(s->len / 2, 128)), dim3(kBlock), 0, s->stream,
                                    (vec2_t<T> *)s->amps, s->len / 2, q, u);
+            prof_end(s);
+            QH_LAUNCH_CHECK();
+            return QVMHIP_OK;
+        }
+    }
+    if constexpr (KT == 1 && sizeof(T) == 4) {
+        if (a.g.ctrl_mask == 0 && s->n >= 9 && gs.tq[0] < 7) {
+            Dense1Coef<float> u;
+            for (int i = 0; i < 4; ++i) {
+                u.ur[i] = (float)gs.V[i].real();
+                u.ui[i] = (float)gs.V[i].imag();
+            }
+            prof_begin(s, KT);
+            hipLaunchKernelGGL(k_dense1_lane2_f32, dim3(grid_for(s->len / 2, kBlock)), dim3(kBlock), 0, s->stream,
+                               (float2 *)s->amps, s->len / 2, gs.tq[0], u);
             prof_end(s);
             QH_LAUNCH_CHECK();
             return QVMHIP_OK;
