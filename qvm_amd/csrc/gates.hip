@@ -478,6 +478,38 @@ __global__ __launch_bounds__(kBlock) void k_perm_xchg(vec2_t<T> *__restrict__ ps
 // on consecutive amplitudes of a line; count is a multiple of kBlock, so every
 // lane of a wavefront takes part in each shuffle.
 //
+// FP32 form of k_perm_lane with every bit below 7: two consecutive amplitudes
+// per lane as one 16-byte access (element j holds amplitudes 2j, 2j + 1); the
+// partner of amplitude 2j + e is in lane j ^ (flip >> 1), in the other half
+// when flip has bit 0.  n4 (elements) is a multiple of kBlock.
+__global__ __launch_bounds__(kBlock) void k_perm_lane2_f32(float2 *__restrict__ psi, uint64_t n4, uint32_t tmask,
+                                                           uint32_t o0, uint32_t o1, uint32_t ctrl) {
+    typedef float F4 __attribute__((ext_vector_type(4)));
+    F4 *p4 = reinterpret_cast<F4 *>(psi);
+    const uint32_t flip = o0 ^ o1;
+    const int d = (int)(flip >> 1);
+    const bool odd = (flip & 1u) != 0;
+    const uint64_t step = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t j = (uint64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; j < n4; j += step) {
+        const F4 v = __builtin_nontemporal_load(&p4[j]);
+        const float s0x = __shfl_xor(v.x, d, 64), s0y = __shfl_xor(v.y, d, 64);
+        const float s1x = __shfl_xor(v.z, d, 64), s1y = __shfl_xor(v.w, d, 64);
+        const uint32_t lo0 = (uint32_t)(j << 1) & 127u, lo1 = lo0 | 1u;
+        const bool m0 = ((lo0 & tmask) == o0 || (lo0 & tmask) == o1) && (lo0 & ctrl) == ctrl;
+        const bool m1 = ((lo1 & tmask) == o0 || (lo1 & tmask) == o1) && (lo1 & ctrl) == ctrl;
+        F4 o = v;
+        if (m0) {
+            o.x = odd ? s1x : s0x;
+            o.y = odd ? s1y : s0y;
+        }
+        if (m1) {
+            o.z = odd ? s0x : s1x;
+            o.w = odd ? s0y : s1y;
+        }
+        __builtin_nontemporal_store(o, &p4[j]);
+    }
+}
+
 // Single-entry diagonal (CZ, CPHASE with a bit inside the line): every
 // amplitude of the lines with the entry's upper bits is read, the ones whose
 // in-line bits match (l0) are multiplied as in k_diag_sparse, and the lines
@@ -926,6 +958,18 @@ int run_perm(qvmhip_state *s, const GateSplit &gs) {
         a.flag[a.nent - 1] |= 2u;
     }
     if (a.nent == 0) return QVMHIP_OK;
+    if (sizeof(T) == 4 && a.nent == 2 && s->n >= 9 && a.g.nsorted && a.g.sorted[a.g.nsorted - 1] < 7 &&
+        !long_runs(a.g, s->amp_bytes)) {
+        uint32_t tmask = 0;
+        for (uint32_t t = 0; t < gs.kt; ++t) tmask |= 1u << gs.tq[t];
+        prof_begin(s, QVMHIP_K_PERM);
+        hipLaunchKernelGGL(k_perm_lane2_f32, dim3(grid_for(s->len / 2, kBlock)), dim3(kBlock), 0, s->stream,
+                           (float2 *)s->amps, s->len / 2, tmask, (uint32_t)a.off[0], (uint32_t)a.off[1],
+                           (uint32_t)a.g.ctrl_mask);
+        prof_end(s);
+        QH_LAUNCH_CHECK();
+        return QVMHIP_OK;
+    }
     if (a.nent == 2 && s->n >= 8 && a.g.nsorted && a.g.sorted[a.g.nsorted - 1] < 6 &&
         !long_runs(a.g, s->amp_bytes)) {
         // every bit involved is a lane bit and the moved amplitudes come in
