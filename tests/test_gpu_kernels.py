@@ -70,6 +70,46 @@ def test_dense1_every_target(S, oracle, dtype, n):
     st.close()
 
 
+def f32_apply_1q(psi, U, q):
+    """The serial 1q kernel's arithmetic (src/serial-kernels.lisp:19-168: each
+    output row u0*a + u1*b, every complex product as (ar*br - ai*bi,
+    ar*bi + ai*br), no fused multiply-add) evaluated in float32 with numpy,
+    one rounding per operation."""
+    x = psi.reshape(-1, 2, 1 << q)
+    a, b = x[:, 0, :], x[:, 1, :]
+    ur = U.real.astype(np.float32)
+    ui = U.imag.astype(np.float32)
+    out = np.empty_like(x)
+    for r in (0, 1):
+        yr = a.real * ur[r, 0] - a.imag * ui[r, 0]
+        yi = a.real * ui[r, 0] + a.imag * ur[r, 0]
+        pr = b.real * ur[r, 1] - b.imag * ui[r, 1]
+        pi = b.real * ui[r, 1] + b.imag * ur[r, 1]
+        out[:, r, :].real = yr + pr
+        out[:, r, :].imag = yi + pi
+    return out.reshape(-1)
+
+
+@pytest.mark.parametrize("n", [9, 12])
+def test_fp32_1q_every_target_bit_exact(S, n):
+    """FP32 1q gates, one at a time on every target (two amplitudes per lane
+    below target 7 from n = 9, the pair form above), against the serial
+    kernel's arithmetic in float32: bit-exact, so the lane pairing and the
+    row choice are pinned, not just the value to a tolerance."""
+    rng = np.random.default_rng(70 + n)
+    say(f"fp32 1q bit-exact n={n}")
+    psi = rand_state(n, rng).astype(np.complex64)
+    st = S(n, "f32")
+    st.write_amplitudes(psi.astype(np.complex128))
+    ref = psi.copy()
+    for q in list(range(n)) + [0, 6, 3]:
+        U = rand_u(1, rng)
+        st.apply_matrix(U, [q])
+        ref = f32_apply_1q(ref, U, q)
+        np.testing.assert_array_equal(np.asarray(st.amplitudes(), dtype=np.complex64), ref)
+    st.close()
+
+
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_full_diagonals_of_every_width(S, oracle, dtype):
     n = 12
