@@ -68,22 +68,22 @@ int scalar_to_host(qvmhip_state *s, double *dev, double *out) {
 void free_state(qvmhip_state *s) {
     if (!s) return;
     for (auto &r : s->prof) {
-        if (r.a) hipEventDestroy(r.a);
-        if (r.b) hipEventDestroy(r.b);
+        if (r.a) (void)hipEventDestroy(r.a);
+        if (r.b) (void)hipEventDestroy(r.b);
     }
-    for (auto e : s->event_pool) hipEventDestroy(e);
-    if (s->amps) hipFree(s->amps);
-    if (s->red) hipFree(s->red);
-    if (s->mat) hipFree(s->mat);
-    if (s->chunk_sum) hipFree(s->chunk_sum);
-    if (s->chunk_pre) hipFree(s->chunk_pre);
-    if (s->scan_tmp) hipFree(s->scan_tmp);
-    if (s->shot_out) hipFree(s->shot_out);
-    if (s->shot_u) hipFree(s->shot_u);
-    if (s->shot_near) hipFree(s->shot_near);
-    if (s->host) hipHostFree(s->host);
-    if (s->stage) hipHostFree(s->stage);
-    if (s->stream && s->own_stream) hipStreamDestroy(s->stream);
+    for (auto e : s->event_pool) (void)hipEventDestroy(e);
+    if (s->amps) (void)hipFree(s->amps);
+    if (s->red) (void)hipFree(s->red);
+    if (s->mat) (void)hipFree(s->mat);
+    if (s->chunk_sum) (void)hipFree(s->chunk_sum);
+    if (s->chunk_pre) (void)hipFree(s->chunk_pre);
+    if (s->scan_tmp) (void)hipFree(s->scan_tmp);
+    if (s->shot_out) (void)hipFree(s->shot_out);
+    if (s->shot_u) (void)hipFree(s->shot_u);
+    if (s->shot_near) (void)hipFree(s->shot_near);
+    if (s->host) (void)hipHostFree(s->host);
+    if (s->stage) (void)hipHostFree(s->stage);
+    if (s->stream && s->own_stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
 
@@ -177,7 +177,7 @@ int qvmhip_create(uint32_t n, int dtype, const qvmhip_config *cfg, qvmhip_state 
 int qvmhip_destroy(qvmhip_state *s) {
     if (!s) return QVMHIP_OK;
     DeviceScope ds(s->device);
-    hipStreamSynchronize(s->stream);
+    (void)hipStreamSynchronize(s->stream);
     for (qvmhip_program *p : s->programs) program_detach(p);
     free_state(s);
     return QVMHIP_OK;
@@ -548,8 +548,8 @@ int qvmhip_write_amplitudes(qvmhip_state *s, uint64_t start, uint64_t count, con
         hipStream_t st = nullptr;
         ~Wide() {
             if (p) {
-                hipStreamSynchronize(st);
-                hipFree(p);
+                (void)hipStreamSynchronize(st);
+                (void)hipFree(p);
             }
         }
     } wide_buf;

@@ -119,15 +119,15 @@ namespace {
 
 void free_batch(qvmhip_batch *b) {
     if (!b) return;
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
     for (void *p : {(void *)b->buf, (void *)b->red, (void *)b->chunk_sum, (void *)b->chunk_pre, (void *)b->scan_tmp,
                     b->mat, b->blob_dev, (void *)b->edges_dev, (void *)b->exp_dev, (void *)b->u_dev,
                     (void *)b->out_dev, (void *)b->starts_dev, b->entries_dev, (void *)b->blkmap_dev})
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     for (void *p : b->fin_dev)
-        if (p) hipFree(p);
-    if (b->stream) hipStreamDestroy(b->stream);
+        if (p) (void)hipFree(p);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
 
@@ -497,7 +497,7 @@ int qvmhip_batch_finalize(qvmhip_batch *b, int use_graph) {
         hipGraph_t g = nullptr;
         e = hipStreamEndCapture(b->stream, &g);
         if (rc) {
-            if (g) hipGraphDestroy(g);
+            if (g) (void)hipGraphDestroy(g);
             return rc;
         }
         if (e != hipSuccess) return fail(QVMHIP_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -556,7 +556,7 @@ int qvmhip_batch_read_amplitudes(qvmhip_batch *b, uint32_t inst, uint64_t start,
 int qvmhip_batch_destroy(qvmhip_batch *b) {
     if (!b) return QVMHIP_OK;
     DeviceScope ds(b->device);
-    hipStreamSynchronize(b->stream);
+    (void)hipStreamSynchronize(b->stream);
     free_batch(b);
     return QVMHIP_OK;
 }

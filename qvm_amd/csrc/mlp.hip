@@ -142,7 +142,7 @@ namespace {
 int ensure_batch(qvmhip_mlp *h, int nb) {
     if (nb <= h->cap) return QVMHIP_OK;
     for (float *q : {h->X, h->T, h->G, h->lossb, h->out})
-        if (q) hipFree(q);
+        if (q) (void)hipFree(q);
     QH_CHECK(hipMalloc((void **)&h->X, sizeof(float) * nb * h->S.din));
     QH_CHECK(hipMalloc((void **)&h->T, sizeof(float) * nb));
     QH_CHECK(hipMalloc((void **)&h->G, sizeof(float) * (size_t)nb * h->S.np));
@@ -274,10 +274,10 @@ int qvmhip_mlp_train(qvmhip_mlp *h, const float *X, const float *T, int nb, int 
 int qvmhip_mlp_destroy(qvmhip_mlp *h) {
     if (!h) return QVMHIP_OK;
     DeviceScope ds(h->device);
-    if (h->stream) hipStreamSynchronize(h->stream);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (float *q : {h->p, h->m, h->v, h->grad, h->loss1, h->X, h->T, h->G, h->lossb, h->out})
-        if (q) hipFree(q);
-    if (h->stream) hipStreamDestroy(h->stream);
+        if (q) (void)hipFree(q);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return QVMHIP_OK;
 }

@@ -183,7 +183,7 @@ int qvmhip_program_finalize(qvmhip_program *p, int use_graph) {
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(s->stream, &g);
         if (rc) {
-            if (g) hipGraphDestroy(g);
+            if (g) (void)hipGraphDestroy(g);
             return rc;
         }
         if (e != hipSuccess) return fail(QVMHIP_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -210,13 +210,13 @@ int qvmhip_program_destroy(qvmhip_program *p) {
     if (!p) return QVMHIP_OK;
     if (p->s) {
         DeviceScope ds(p->s->device);
-        hipStreamSynchronize(p->s->stream);
+        (void)hipStreamSynchronize(p->s->stream);
         auto &v = p->s->programs;
         v.erase(std::remove(v.begin(), v.end(), p), v.end());
     }
-    if (p->exec) hipGraphExecDestroy(p->exec);
-    if (p->graph) hipGraphDestroy(p->graph);
-    if (p->dev) hipFree(p->dev);
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->dev) (void)hipFree(p->dev);
     delete p;
     return QVMHIP_OK;
 }
@@ -229,8 +229,8 @@ int qvmhip_time_program(qvmhip_program *p, int reps, double *ms) {
     struct Events {  // destroyed on every return
         hipEvent_t a = nullptr, b = nullptr;
         ~Events() {
-            if (a) hipEventDestroy(a);
-            if (b) hipEventDestroy(b);
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
         }
     } ev;
     QH_CHECK(hipEventCreate(&ev.a));
@@ -239,7 +239,7 @@ int qvmhip_time_program(qvmhip_program *p, int reps, double *ms) {
     for (int r = 0; r < reps; ++r) {
         int rc = qvmhip_program_run(p);
         if (rc) {
-            hipStreamSynchronize(s->stream);  // no pending record of the events
+            (void)hipStreamSynchronize(s->stream);  // no pending record of the events
             return rc;
         }
     }

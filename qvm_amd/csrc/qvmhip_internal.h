@@ -140,7 +140,7 @@ struct DeviceScope {
         if (hipGetDevice(&cur) == hipSuccess && cur != dev && hipSetDevice(dev) == hipSuccess) prev = cur;
     }
     ~DeviceScope() {
-        if (prev >= 0) hipSetDevice(prev);
+        if (prev >= 0) (void)hipSetDevice(prev);
     }
     DeviceScope(const DeviceScope &) = delete;
     DeviceScope &operator=(const DeviceScope &) = delete;
@@ -156,7 +156,7 @@ inline hipError_t stream_after(hipStream_t waiter, hipStream_t done) {
     if (e != hipSuccess) return e;
     e = hipEventRecord(ev, done);
     if (e == hipSuccess) e = hipStreamWaitEvent(waiter, ev, 0);
-    hipEventDestroy(ev);  // released once the recorded work has completed
+    (void)hipEventDestroy(ev);  // released once the recorded work has completed
     return e;
 }
 
@@ -219,12 +219,12 @@ inline hipEvent_t prof_event(qvmhip_state *s) {
 inline void prof_begin(qvmhip_state *s, int kind) {
     if (!s->profiling) return;
     qvmhip_state::ProfRec r{kind, prof_event(s), prof_event(s)};
-    if (r.a) hipEventRecord(r.a, s->stream);
+    if (r.a) (void)hipEventRecord(r.a, s->stream);
     s->prof.push_back(r);
 }
 inline void prof_end(qvmhip_state *s) {
     if (!s->profiling || s->prof.empty()) return;
-    if (s->prof.back().b) hipEventRecord(s->prof.back().b, s->stream);
+    if (s->prof.back().b) (void)hipEventRecord(s->prof.back().b, s->stream);
 }
 
 // Launch helpers implemented in the kernel files.

@@ -770,9 +770,9 @@ static int build_chunk_prefix(qvmhip_state *s) {
 
 static int ensure_shots(qvmhip_state *s, uint64_t nshots) {
     if (nshots <= s->shot_cap) return QVMHIP_OK;
-    if (s->shot_out) hipFree(s->shot_out);
-    if (s->shot_u) hipFree(s->shot_u);
-    if (s->shot_near) hipFree(s->shot_near);
+    if (s->shot_out) (void)hipFree(s->shot_out);
+    if (s->shot_u) (void)hipFree(s->shot_u);
+    if (s->shot_near) (void)hipFree(s->shot_near);
     s->shot_out = nullptr;
     s->shot_u = nullptr;
     s->shot_near = nullptr;
@@ -984,8 +984,8 @@ int export_range(qvmhip_state *s, uint64_t start, uint64_t count, int mode, void
     int rc = QVMHIP_OK;
     auto cleanup = [&]() {
         for (int h = 0; h < 2; ++h) {
-            if (dev[h]) hipFree(dev[h]);
-            if (ev[h]) hipEventDestroy(ev[h]);
+            if (dev[h]) (void)hipFree(dev[h]);
+            if (ev[h]) (void)hipEventDestroy(ev[h]);
         }
     };
     for (int h = 0; h < 2; ++h) {
@@ -1039,7 +1039,7 @@ int export_range(qvmhip_state *s, uint64_t start, uint64_t count, int mode, void
             std::memcpy(out + off * eb, half[h], c * eb);
         }
     }
-    if (rc != QVMHIP_OK) hipStreamSynchronize(s->stream);  // no copy may still target the stage
+    if (rc != QVMHIP_OK) (void)hipStreamSynchronize(s->stream);  // no copy may still target the stage
     cleanup();
     return rc;
 }
@@ -1074,7 +1074,7 @@ int density_diagonal(qvmhip_state *s, uint32_t nq, double *host_out) {
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(host_out, dev, dim * sizeof(double), hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-    hipFree(dev);
+    (void)hipFree(dev);
     if (e != hipSuccess) return fail(QVMHIP_EDEVICE, std::string("density diagonal: ") + hipGetErrorString(e));
     return QVMHIP_OK;
 }

@@ -1452,8 +1452,8 @@ int launch_tile_pass(qvmhip_state *s, uint32_t m, const uint32_t *tile_qubits, u
     int rc = s->dtype == QVMHIP_F64 ? run_tile_m<double>(s, m, tile_qubits, dev, nops, cut)
                                     : run_tile_m<float>(s, m, tile_qubits, dev, nops, cut);
     if (tmp) {
-        hipStreamSynchronize(s->stream);
-        hipFree(tmp);
+        (void)hipStreamSynchronize(s->stream);
+        (void)hipFree(tmp);
     }
     return rc;
 }
