@@ -47,6 +47,7 @@ struct BLaunch {
     uint32_t inst;      // gate launches
     const BItem *item;  // gate launches
     bool cut;           // tile launches: an entry holds CUT_PHASE ops
+    bool zero;          // tile launches: an entry starts from |0...0> or has all-zero blocks
 };
 
 template <typename T>
@@ -163,9 +164,9 @@ int enqueue(qvmhip_batch *b) {
     for (const BLaunch &L : b->launches) {
         int rc;
         if (L.kind == kItemTile)
-            rc = launch_tile_batch(b->stream, b->dtype, (b->flags & QVMHIP_FLAG_FMA_TILES) != 0, L.cut, L.m, b->buf,
-                                   (const char *)b->entries_dev + L.ent_off, b->blkmap_dev + L.map_off, L.blocks,
-                                   kTilesPerBlock, b->blob_dev);
+            rc = launch_tile_batch(b->stream, b->dtype, (b->flags & QVMHIP_FLAG_FMA_TILES) != 0, L.cut, L.zero, L.m,
+                                   b->buf, (const char *)b->entries_dev + L.ent_off, b->blkmap_dev + L.map_off,
+                                   L.blocks, kTilesPerBlock, b->blob_dev);
         else
             rc = launch_gate(&b->views[L.inst], L.item->U.data(), L.item->k, L.item->qubits, 0);
         if (rc) return rc;
@@ -455,11 +456,11 @@ int qvmhip_batch_finalize(qvmhip_batch *b, int use_graph) {
             if (it.kind == kItemTile)
                 by_m[it.k].push_back(i);
             else
-                b->launches.push_back({kItemGate, 0, 0, 0, 0, 0, i, &it, false});
+                b->launches.push_back({kItemGate, 0, 0, 0, 0, 0, i, &it, false, false});
         }
         for (auto &kv : by_m) {
             BLaunch bl{kItemTile, kv.first, b->entries.size(), (uint32_t)kv.second.size(), 0, b->blkmap.size(), 0,
-                       nullptr, false};
+                       nullptr, false, false};
             uint64_t blocks = 0;
             uint32_t local = 0;
             for (uint32_t i : kv.second) {
@@ -469,10 +470,12 @@ int qvmhip_batch_finalize(qvmhip_batch *b, int use_graph) {
                 uint64_t tmask = 0;
                 for (uint32_t j = 0; j < it.k; ++j) tmask |= uint64_t(1) << it.qubits[j];
                 const uint64_t all = (uint64_t(1) << b->nq[i]) - 1;
+                const bool from_zero = L == 0 && b->first_pass_resets;
+                const uint64_t zero_mask = all & ~touched[i][L] & ~tmask;
+                bl.zero = bl.zero || from_zero || zero_mask != 0;
                 const uint64_t nb = build_tile_entry(
                     ent.data(), b->dtype, (b->flags & QVMHIP_FLAG_FMA_TILES) != 0, b->nq[i], it.k, it.qubits,
-                    b->offset[i], blocks, it.op_off, it.nops, kTilesPerBlock, L == 0 && b->first_pass_resets,
-                    all & ~touched[i][L] & ~tmask);
+                    b->offset[i], blocks, it.op_off, it.nops, kTilesPerBlock, from_zero, zero_mask);
                 b->entries.insert(b->entries.end(), ent.begin(), ent.end());
                 b->blkmap.insert(b->blkmap.end(), (size_t)nb, local);
                 blocks += nb;

@@ -309,7 +309,8 @@ uint64_t build_tile_entry(void *entry, int dtype, bool fma, uint32_t n, uint32_t
                           uint32_t tiles_per_block, bool from_zero, uint64_t zero_mask);
 // op_buf: the device buffer the entries' op offsets point into
 // cut: some entry of the launch holds CUT_PHASE ops
-int launch_tile_batch(hipStream_t st, int dtype, bool fma, bool cut, uint32_t m, void *base,
+// zero: some entry starts from |0...0> or has a nonzero zero_mask
+int launch_tile_batch(hipStream_t st, int dtype, bool fma, bool cut, bool zero, uint32_t m, void *base,
                       const void *entries_dev, const uint32_t *blk_ent_dev, uint64_t total_blocks,
                       uint32_t tiles_per_block, const void *op_buf);
 }  // namespace qvmhip

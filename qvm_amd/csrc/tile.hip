@@ -791,10 +791,13 @@ struct TileEntry {
     uint32_t from_zero;    // 1: the state is |0...0>, built in registers instead of loaded
 };
 
+// ZERO: some entry of the launch starts from |0...0> or has all-zero
+// blocks; launches without (every level after the first few) are built
+// without that code, which costs the exact FP64 forms 60 VGPRs.
 // FP64 FMA forms held to five waves per SIMD (96 VGPRs, 11 spilled; left
 // alone they take 108 and four waves, 5 % slower on the 512-instance QAOA
 // batch); the exact forms would spill ~100 values at five.
-template <typename T, int M, int RB, bool FMA, bool CUT>
+template <typename T, int M, int RB, bool FMA, bool CUT, bool ZERO>
 __global__ __launch_bounds__(1 << (M - RB))
 __attribute__((amdgpu_waves_per_eu(FMA && sizeof(T) == 8 ? 5 : 1))) void k_tile_batch(
     vec2_t<T> *__restrict__ base, const TileEntry *__restrict__ ent, const uint32_t *__restrict__ blk_ent,
@@ -815,6 +818,12 @@ __attribute__((amdgpu_waves_per_eu(FMA && sizeof(T) == 8 ? 5 : 1))) void k_tile_
         for (uint64_t tile = first; tile < last; ++tile) {
             V x[1 << RB];
             const uint64_t gb = group_base(e.args.g, tile);
+            if (!ZERO) {
+                tile_load<T, M, RB>(psi, e.args, gb, toff, x);
+                tile_ops<T, M, RB, FMA, CUT>(x, (const qvmhip_tile_op *)(op_buf + e.op_off), e.nops, sa, gb);
+                tile_store<T, M, RB>(psi, e.args, gb, toff, x);
+                continue;
+            }
             if (gb & e.zero_mask) {
                 // an all-zero block stays zero: the first pass of a run stores
                 // it (the buffer holds the previous run), later passes leave
@@ -1479,25 +1488,29 @@ uint64_t build_tile_entry(void *entry, int dtype, bool fma, uint32_t n, uint32_t
 namespace {
 template <typename T, int M, bool FMA, bool CUT>
 void tile_batch_go(hipStream_t st, void *base, const void *ent, const uint32_t *blk_ent, uint64_t total,
-                   uint32_t tpb, const void *op_buf) {
+                   uint32_t tpb, const void *op_buf, bool zero) {
     constexpr int RB = tile_reg_bits(M, sizeof(T) == 8, FMA);
     constexpr int NT = 1 << (M - RB);
     const uint32_t grid = (uint32_t)std::min<uint64_t>(total, kMaxGrid);
-    hipLaunchKernelGGL((k_tile_batch<T, M, RB, FMA, CUT>), dim3(grid), dim3(NT), 0, st, (vec2_t<T> *)base,
-                       (const TileEntry *)ent, blk_ent, total, tpb, (const char *)op_buf);
+    if (zero)
+        hipLaunchKernelGGL((k_tile_batch<T, M, RB, FMA, CUT, true>), dim3(grid), dim3(NT), 0, st, (vec2_t<T> *)base,
+                           (const TileEntry *)ent, blk_ent, total, tpb, (const char *)op_buf);
+    else
+        hipLaunchKernelGGL((k_tile_batch<T, M, RB, FMA, CUT, false>), dim3(grid), dim3(NT), 0, st,
+                           (vec2_t<T> *)base, (const TileEntry *)ent, blk_ent, total, tpb, (const char *)op_buf);
 }
 template <typename T, bool FMA, bool CUT>
 int tile_batch_m(hipStream_t st, uint32_t m, void *base, const void *ent, const uint32_t *blk_ent, uint64_t total,
-                 uint32_t tpb, const void *op_buf) {
+                 uint32_t tpb, const void *op_buf, bool zero) {
     switch (m) {
-        case 8: tile_batch_go<T, 8, FMA, CUT>(st, base, ent, blk_ent, total, tpb, op_buf); break;
-        case 9: tile_batch_go<T, 9, FMA, CUT>(st, base, ent, blk_ent, total, tpb, op_buf); break;
-        case 10: tile_batch_go<T, 10, FMA, CUT>(st, base, ent, blk_ent, total, tpb, op_buf); break;
-        case 11: tile_batch_go<T, 11, FMA, CUT>(st, base, ent, blk_ent, total, tpb, op_buf); break;
-        case 12: tile_batch_go<T, 12, FMA, CUT>(st, base, ent, blk_ent, total, tpb, op_buf); break;
+        case 8: tile_batch_go<T, 8, FMA, CUT>(st, base, ent, blk_ent, total, tpb, op_buf, zero); break;
+        case 9: tile_batch_go<T, 9, FMA, CUT>(st, base, ent, blk_ent, total, tpb, op_buf, zero); break;
+        case 10: tile_batch_go<T, 10, FMA, CUT>(st, base, ent, blk_ent, total, tpb, op_buf, zero); break;
+        case 11: tile_batch_go<T, 11, FMA, CUT>(st, base, ent, blk_ent, total, tpb, op_buf, zero); break;
+        case 12: tile_batch_go<T, 12, FMA, CUT>(st, base, ent, blk_ent, total, tpb, op_buf, zero); break;
         case 13:
             if constexpr (sizeof(T) == 4) {
-                tile_batch_go<T, 13, FMA, CUT>(st, base, ent, blk_ent, total, tpb, op_buf);
+                tile_batch_go<T, 13, FMA, CUT>(st, base, ent, blk_ent, total, tpb, op_buf, zero);
                 break;
             }
             return fail(QVMHIP_EINVAL, "tile size 13 is FP32 only");
@@ -1509,12 +1522,12 @@ int tile_batch_m(hipStream_t st, uint32_t m, void *base, const void *ent, const 
 }
 }  // namespace
 
-int launch_tile_batch(hipStream_t st, int dtype, bool fma, bool cut, uint32_t m, void *base,
+int launch_tile_batch(hipStream_t st, int dtype, bool fma, bool cut, bool zero, uint32_t m, void *base,
                       const void *entries_dev, const uint32_t *blk_ent_dev, uint64_t total_blocks,
                       uint32_t tiles_per_block, const void *op_buf) {
     const bool f64 = dtype == QVMHIP_F64;
 #define QH_TB(TT, F, C) \
-    tile_batch_m<TT, F, C>(st, m, base, entries_dev, blk_ent_dev, total_blocks, tiles_per_block, op_buf)
+    tile_batch_m<TT, F, C>(st, m, base, entries_dev, blk_ent_dev, total_blocks, tiles_per_block, op_buf, zero)
     if (!fma) return f64 ? QH_TB(double, false, false) : QH_TB(float, false, false);
     if (!cut) return f64 ? QH_TB(double, true, false) : QH_TB(float, true, false);
     return f64 ? QH_TB(double, true, true) : QH_TB(float, true, true);
