@@ -532,6 +532,30 @@ __global__ __launch_bounds__(kBlock) void k_diag_line(vec2_t<T> *__restrict__ ps
     }
 }
 
+// FP32 single-entry sparse diagonal whose bits are all >= 1: the changed
+// amplitudes come in pairs (2j, 2j + 1), so the kernel works on 16-byte pairs
+// (g2: the gate's bits and controls shifted down by one over 2^(n-1) pairs,
+// off2 the entry's offset in pairs), each thread one pair, non-temporal (the
+// caller uses it for runs of >= 128 B); the products are those of
+// k_diag_sparse.
+__global__ __launch_bounds__(kBlock) void k_diag_sparse2_f32(float2 *__restrict__ psi, const GroupSpec g2,
+                                                             uint64_t off2, float dr, float di) {
+    typedef float F4 __attribute__((ext_vector_type(4)));
+    F4 *p4 = reinterpret_cast<F4 *>(psi);
+    const uint64_t step = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; i < g2.ngroups;
+         i += step) {
+        const uint64_t e = group_base(g2, i) | off2;
+        const F4 v = __builtin_nontemporal_load(&p4[e]);
+        F4 y;
+        y.x = v.x * dr - v.y * di;
+        y.y = v.x * di + v.y * dr;
+        y.z = v.z * dr - v.w * di;
+        y.w = v.z * di + v.w * dr;
+        __builtin_nontemporal_store(y, &p4[e]);
+    }
+}
+
 // Single transposition E0 <-> E1 (CNOT, SWAP, CCNOT, CSWAP) with a bit inside
 // the line and one at or above bit 6 (the all-lane case is k_perm_lane).  The
 // moving amplitude at in-line position l of line A (upper bits h0) pairs with
@@ -919,6 +943,22 @@ int run_diag(qvmhip_state *s, const GateSplit &gs) {
             QH_LAUNCH_CHECK();
             return QVMHIP_OK;
         }
+    }
+    // FP32, one entry, runs of >= 128 B: 16-byte pairs per thread (n = 30:
+    // CPHASE(11, 4) 0.80 -> 0.68 ms, (29, 22) 0.80 -> 0.64, (15, 8) 0.97 ->
+    // 0.85; two or four groups per thread gained less, and with shorter runs
+    // every variant lost)
+    if (sizeof(T) == 4 && a.nent == 1 && lr) {
+        GroupSpec g2 = a.g;
+        for (uint32_t j = 0; j < g2.nsorted; ++j) g2.sorted[j] -= 1;
+        g2.ctrl_mask >>= 1;
+        g2.ngroups >>= 1;
+        prof_begin(s, QVMHIP_K_DIAG_SPARSE);
+        hipLaunchKernelGGL(k_diag_sparse2_f32, dim3(grid_for(g2.ngroups, (uint64_t)kBlock)), dim3(kBlock), 0,
+                           s->stream, (float2 *)s->amps, g2, a.off[0] >> 1, (float)a.dr[0], (float)a.di[0]);
+        prof_end(s);
+        QH_LAUNCH_CHECK();
+        return QVMHIP_OK;
     }
     prof_begin(s, QVMHIP_K_DIAG_SPARSE);
     if (a.nent == 1) {

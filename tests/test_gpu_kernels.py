@@ -110,6 +110,40 @@ def test_fp32_1q_every_target_bit_exact(S, n):
     st.close()
 
 
+def test_fp32_single_entry_diagonals_bit_exact(S):
+    """FP32 CZ / CPHASE / controlled CPHASE: one changed entry, the amplitudes whose
+    listed bits are all 1 multiplied by the entry (serial-kernel products,
+    evaluated in float32 here).  Bits all >= 4 take the 16-byte pair kernel,
+    lower ones the per-amplitude form; bit-exact for both."""
+    n = 14
+    rng = np.random.default_rng(83)
+    say("fp32 single-entry diagonals bit-exact")
+    psi = rand_state(n, rng).astype(np.complex64)
+    st = S(n, "f32")
+    st.write_amplitudes(psi.astype(np.complex128))
+    ref = psi.copy()
+    idx = np.arange(1 << n, dtype=np.int64)
+    ccphase = G.controlled(G.gate_matrix("CPHASE", [0.4]))
+    cases = [(G.gate_matrix("CPHASE", [0.7]), [9, 4]), (G.gate_matrix("CZ"), [13, 5]),
+             (G.gate_matrix("CPHASE", [1.9]), [4, 12]), (ccphase, [6, 11, 8]),
+             (G.gate_matrix("CPHASE", [2.3]), [9, 1]), (G.gate_matrix("CZ"), [0, 7]),
+             (G.gate_matrix("CPHASE", [0.2]), [3, 2])]
+    for U, qs in cases:
+        st.apply_matrix(U, G.nat_tuple(*qs))
+        d = np.complex64(U[-1, -1])
+        mask = np.ones(1 << n, dtype=bool)
+        for q in qs:
+            mask &= ((idx >> q) & 1) == 1
+        x = ref[mask]
+        dr, di = np.float32(d.real), np.float32(d.imag)
+        y = np.empty_like(x)
+        y.real = x.real * dr - x.imag * di
+        y.imag = x.real * di + x.imag * dr
+        ref[mask] = y
+        np.testing.assert_array_equal(np.asarray(st.amplitudes(), dtype=np.complex64), ref)
+    st.close()
+
+
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_full_diagonals_of_every_width(S, oracle, dtype):
     n = 12
