@@ -29,8 +29,8 @@ def main(argv=None) -> int:
     ap.add_argument("--n", type=int, default=30)
     ap.add_argument("--dtype", default="f64")
     ap.add_argument("--kernel", default="dense1",
-                    choices=["dense1", "diag_full", "perm", "swap", "diag_sparse", "fused3", "fused4", "fused5",
-                             "fused6", "tile", "tile_ops"])
+                    choices=["dense1", "diag_full", "perm", "perm_adj", "swap", "diag_sparse", "fused3", "fused4",
+                             "fused5", "fused6", "tile", "tile_ops"])
     ap.add_argument("--reps", type=int, default=8)
     ap.add_argument("--fma", action="store_true", help="tile passes with FMA-contracted products")
     ap.add_argument("--tile-m", type=int, default=0, help="tile width (default: the planner's)")
@@ -55,6 +55,8 @@ def main(argv=None) -> int:
             st.apply_matrix(G.gate_matrix("RZ", [0.1 + r]), [q])
         elif a.kernel == "perm":
             st.apply_matrix(G.gate_matrix("CNOT"), [q, (q + 7) % a.n])
+        elif a.kernel == "perm_adj":
+            st.apply_matrix(G.gate_matrix("CNOT"), [q, (q + 1) % a.n])
         elif a.kernel == "swap":
             st.apply_matrix(G.gate_matrix("SWAP"), [q, a.n - 1 - q])
         elif a.kernel == "diag_sparse":
