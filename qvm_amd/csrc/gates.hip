@@ -513,22 +513,26 @@ __global__ __launch_bounds__(kBlock) void k_perm_lane2_f32(float2 *__restrict__ 
 // Single-entry diagonal (CZ, CPHASE with a bit inside the line): every
 // amplitude of the lines with the entry's upper bits is read, the ones whose
 // in-line bits match (l0) are multiplied as in k_diag_sparse, and the lines
-// are written back whole.
+// are written back whole, or with `masked` only the changed amplitudes: the
+// caller sets it when they form 64-byte halves of each line (in-line bit 2
+// only; n = 30: CPHASE(29, 2) 2.57 -> 2.11 ms), since 16- and 32-byte pieces
+// wrote slower than whole lines (CPHASE(9, 0) 2.61 -> 4.26 ms).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_diag_line(vec2_t<T> *__restrict__ psi, const GroupSpec hg,
                                                       uint64_t count, uint64_t h0, uint32_t lmask, uint32_t l0,
-                                                      T dr, T di) {
+                                                      T dr, T di, uint32_t masked) {
     using V = vec2_t<T>;
     const uint64_t step = (uint64_t)gridDim.x * kBlock;
     for (uint64_t j = (uint64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; j < count; j += step) {
         const uint64_t i = group_base(hg, j) | h0;
         const V x = ldg<true>(&psi[i]);
         V y = x;
-        if (((uint32_t)i & lmask) == l0) {
+        const bool hit = ((uint32_t)i & lmask) == l0;
+        if (hit) {
             y.x = x.x * dr - x.y * di;
             y.y = x.x * di + x.y * dr;
         }
-        stg<true>(&psi[i], y);
+        if (hit || !masked) stg<true>(&psi[i], y);
     }
 }
 
@@ -938,7 +942,7 @@ int run_diag(qvmhip_state *s, const GateSplit &gs) {
             prof_begin(s, QVMHIP_K_DIAG_SPARSE);
             hipLaunchKernelGGL((k_diag_line<T>), dim3(grid_for(ls.hg.ngroups, kBlock)), dim3(kBlock), 0, s->stream,
                                psi, ls.hg, ls.hg.ngroups, e0 & ls.hmask, ls.lmask, (uint32_t)e0 & ls.lmask, a.dr[0],
-                               a.di[0]);
+                               a.di[0], (ls.lmask & 3u) == 0 ? 1u : 0u);
             prof_end(s);
             QH_LAUNCH_CHECK();
             return QVMHIP_OK;

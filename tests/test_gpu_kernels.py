@@ -206,7 +206,8 @@ def test_line_sweeps_for_bits_inside_a_line(S, oracle, dtype):
              (G.gate_matrix("CZ"), [3, 9]), (ccp, [1, 7, 11]), (G.gate_matrix("CPHASE", [0.5]), [11, 0]),
              (G.gate_matrix("SWAP"), [0, 12]), (G.gate_matrix("SWAP"), [2, 7]), (G.gate_matrix("CNOT"), [1, 11]),
              (G.gate_matrix("CNOT"), [10, 2]), (G.gate_matrix("CSWAP"), [9, 0, 11]),
-             (G.gate_matrix("CCNOT"), [0, 1, 8]), (G.gate_matrix("SWAP"), [3, 6]), (G.gate_matrix("CNOT"), [3, 12])]
+             (G.gate_matrix("CCNOT"), [0, 1, 8]), (G.gate_matrix("SWAP"), [3, 6]), (G.gate_matrix("CNOT"), [3, 12]),
+             (G.gate_matrix("PHASE", [1.3]), [2]), (ccp, [2, 5, 10])]  # bit 2 only inside: changed halves stored
     for U, qs in cases:
         say(f"  {dtype} {U.shape[0]}x{U.shape[0]} on {qs}")
         st.apply_matrix(U, G.nat_tuple(*qs))
