@@ -426,7 +426,7 @@ __global__ __launch_bounds__(kBlock) void k_perm(vec2_t<T> *__restrict__ psi,
 // line when the bits are this low).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_perm_lane(vec2_t<T> *__restrict__ psi, uint64_t len, uint32_t tmask,
-                                                      uint32_t o0, uint32_t o1, uint32_t ctrl) {
+                                                      uint32_t o0, uint32_t o1, uint32_t ctrl, uint32_t masked) {
     using V = vec2_t<T>;
     const uint32_t flip = o0 ^ o1;
     const uint64_t step = (uint64_t)gridDim.x * kBlock;
@@ -437,7 +437,7 @@ __global__ __launch_bounds__(kBlock) void k_perm_lane(vec2_t<T> *__restrict__ ps
         p.y = __shfl_xor(x.y, (int)flip, 64);
         const uint32_t lo = (uint32_t)i & 63u;
         const bool moved = ((lo & tmask) == o0 || (lo & tmask) == o1) && (lo & ctrl) == ctrl;
-        stg<true>(&psi[i], moved ? p : x);
+        if (moved || !masked) stg<true>(&psi[i], moved ? p : x);
     }
 }
 
@@ -1024,9 +1024,12 @@ int run_perm(qvmhip_state *s, const GateSplit &gs) {
         uint32_t tmask = 0;
         for (uint32_t t = 0; t < gs.kt; ++t) tmask |= 1u << gs.tq[t];
         prof_begin(s, QVMHIP_K_PERM);
+        // FP64 with every bit >= 2: the moved amplitudes are 64-byte halves
+        // of lines, and only they are stored (as in k_diag_line)
+        const uint32_t masked = (sizeof(T) == 8 && ((tmask | (uint32_t)a.g.ctrl_mask) & 3u) == 0) ? 1u : 0u;
         hipLaunchKernelGGL((k_perm_lane<T>), dim3(grid_for(s->len, kBlock)), dim3(kBlock), 0, s->stream,
                            (vec2_t<T> *)s->amps, s->len, tmask, (uint32_t)a.off[0], (uint32_t)a.off[1],
-                           (uint32_t)a.g.ctrl_mask);
+                           (uint32_t)a.g.ctrl_mask, masked);
         prof_end(s);
         QH_LAUNCH_CHECK();
         return QVMHIP_OK;
